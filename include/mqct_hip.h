@@ -1,0 +1,253 @@
+/*
+ * mqct_hip.h -- C ABI of libmqct_hip.so, the MI355X (gfx950) library that
+ * replaces MQCT's two data-parallel hot paths on one GPU.
+ *
+ * Conventions (match the Fortran host so iso_c_binding can bind directly):
+ *  - all pointers are host memory owned by the caller unless a name ends in
+ *    _dev; arrays are in Fortran column-major order as the reference keeps
+ *    them (Mat_el(n_r, total_size) is n_r contiguous R values per element);
+ *  - state numbers, channel numbers and ind_mat entries are 1-based;
+ *  - energies and matrix elements are Hartree, distances bohr, angles rad;
+ *  - every call returns 0 on success and a negative code on failure; the
+ *    message is available through mqct_hip_last_error (the Fortran bridge
+ *    turns a failure into STOP "<message>", the reference's convention);
+ *  - one host thread per handle; a handle owns its device buffers and its
+ *    stream and is not re-entrant.
+ */
+#ifndef MQCT_HIP_H
+#define MQCT_HIP_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mqct_hip mqct_hip_t;
+
+enum {
+  MQCT_OK = 0,
+  MQCT_ERR_ARG = -1,       /* bad argument / unsupported configuration */
+  MQCT_ERR_HIP = -2,       /* HIP runtime failure */
+  MQCT_ERR_STATE = -3,     /* call order (e.g. build before set_basis) */
+  MQCT_ERR_NOMEM = -4
+};
+
+/* precision forms of the matrix build */
+enum {
+  MQCT_PREC_FP64 = 0,      /* FP64 MFMA, FP64 accumulation */
+  MQCT_PREC_FP32 = 1,      /* FP32 MFMA, FP32 chunk sums added in FP64 */
+  MQCT_PREC_BF16X3 = 2,    /* split-BF16 hi*hi + hi*lo + lo*hi on MFMA */
+  MQCT_PREC_SURROGATE = 3  /* potential from the MLP surrogate, then BF16X3 */
+};
+
+/* synthetic potentials evaluated on the GPU (parameter layouts in
+ * mqct_amd/workloads.py) */
+enum { MQCT_PES_LEGENDRE = 0, MQCT_PES_REALSH = 1, MQCT_PES_SITES = 2 };
+
+/* propagation modes (PROPAGATOR= keyword x AT_APPROX= keyword) */
+enum {
+  MQCT_PROP_CC_RK4 = 0,    /* DERIVS_BK + rk4, fixed step */
+  MQCT_PROP_CC_ODEINT = 1, /* DERIVS_BK + odeint per outer step */
+  MQCT_PROP_AT_RK4 = 2,    /* DERIVS_BK_adia + rk4 on a recorded path */
+  MQCT_PROP_AT_ODEINT = 3  /* DERIVS_BK_adia + odeint over the path */
+};
+
+/* ---------------------------------------------------------------------- */
+/* Library lifetime.  No reference counterpart (the reference has no device). */
+int mqct_hip_create(int device, mqct_hip_t **h);
+int mqct_hip_destroy(mqct_hip_t *h);
+int mqct_hip_last_error(char *buf, int len);
+int mqct_hip_version(void);
+
+/* ---------------------------------------------------------------------- */
+/* Matrix build: replaces the loop of MATRIX_MAKE over k and R that calls
+ * INTEGRATOR_TEMP (MQCT_SOURCE_CODES/matrix_ini.f:1916-2016, element
+ * integrand pot_comp.f:1790-3737) together with the V-grid fill of
+ * INI_POT_STORAGE (pot_comp.f:333-1788).                                  */
+
+/* Basis: the state tables INI_ARRAYS builds (matrix_ini.f:255-545) and the
+ * channel quantum numbers and eigenvector table ASYM_TOP_VECTORS keeps
+ * (pot_comp.f:1-254, M_VECTORS / M1_VECTORS / M2_VECTORS).
+ * coll_type: 1 (diatom+atom), 4 (asym. top+atom), 0 (asym.+asym. top,
+ * distinguishable).  eig1/eig2 are [n_channels][ldvec] row-major with c_K,
+ * K=-J..J in the first 2J+1 slots (NULL for type 1; eig2 only for type 0). */
+int mqct_hip_set_basis(mqct_hip_t *h, int coll_type, int n_states,
+                       const int *j12, const int *m12, const int *indx_chann,
+                       int n_channels, const int *j1_ch, const int *j2_ch,
+                       int ldvec, const double *eig1, const double *eig2);
+
+/* Angular quadrature: GRID_INTEGR module state (iotest.f:62-90) as
+ * INI_POT_STORAGE fills it (pot_comp.f:374-382, 586-611, 1628-1709).  Nodes
+ * x* on [-1,1] with weights w*, mapped as angle = x*m + x*l * x (or angle =
+ * x for equidistant gamma/alpha grids, eq_gamma=1).  Unused dimensions have
+ * n=1, x=0, w=1, m=0, l=1.  ref_xz=0 selects the REF_XZ=NO integrand of
+ * pot_comp.f:2544-2553. */
+typedef struct {
+  int n_beta;   const double *xb;  const double *wb;  double xbm, xbl;
+  int n_gamma;  const double *xg;  const double *wg;  double xgm, xgl;
+  int n_alpha2; const double *xaa; const double *waa; double xaam, xaal;
+  int n_beta2;  const double *xbb; const double *wbb; double xbbm, xbbl;
+  int n_gamma2; const double *xgg; const double *wgg; double xggm, xggl;
+  int eq_gamma;
+  int ref_xz;
+} mqct_grid_t;
+int mqct_hip_set_grid(mqct_hip_t *h, const mqct_grid_t *g);
+
+/* V on the grid from the host, in E_UNITS, in the POT_STORE layouts
+ * (iotest.f:33-48): type 1 V_2_1(n_beta, n_r); type 4 V_3_1(n_gamma,
+ * n_beta, n_r); type 0 V_3_3(n_gamma2, n_gamma1, n_beta2, n_beta1, n_alpha2,
+ * n_r).  R_com in bohr (the PES was called with R_COM*conv_unit_r). */
+int mqct_hip_set_vgrid(mqct_hip_t *h, int n_r, const double *R_com,
+                       const double *V);
+
+/* Same V grid computed on the GPU from a synthetic potential (stands in for
+ * USER_DEFINED_PES called through pes_sys_type.f:2-97). */
+int mqct_hip_eval_pes(mqct_hip_t *h, int n_r, const double *R_com,
+                      int pes_kind, int n_par, const double *par);
+
+/* The build.  Produces Mat_el(n_r, total_size) = quadrature * conv_unit_e,
+ * zeroes element k when max(|M(ir_cut1)|, |M(ir_cut2)|) < mij_zero_cut
+ * (Hartree; matrix_ini.f:1991-1997; ir_cut* 1-based), and Mat_el_der =
+ * clamped-spline second derivatives (matrix_ini.f:2005-2016).  ind_mat is
+ * 2 x total_size (matrix_ini.f:553-630) and must be INI_ARRAYS' order.
+ * mat_el / mat_el_der may be NULL: the result then stays on the device
+ * (mqct_hip_matrix_get copies it out; mqct_hip_use_built_matrix hands it to
+ * the propagator).  skipped (optional, total_size bytes) receives the
+ * K_SKIPPED_BY_ROUTINE mask. */
+int mqct_hip_matrix_build(mqct_hip_t *h, int precision, int64_t total_size,
+                          const int *ind_mat, double conv_unit_e,
+                          double mij_zero_cut, int ir_cut1, int ir_cut2,
+                          double *mat_el, double *mat_el_der,
+                          uint8_t *skipped);
+int mqct_hip_matrix_get(mqct_hip_t *h, double *mat_el, double *mat_el_der);
+
+/* Per-stage device times of the last build, milliseconds:
+ * [0] basis functions on the grid, [1] potential/weights, [2] contraction,
+ * [3] cutoff+spline, [4] total. */
+int mqct_hip_build_timing(mqct_hip_t *h, double *ms5);
+
+/* ---------------------------------------------------------------------- */
+/* Propagation: replaces the stepping of DOTRAJECT (propagator.f:3205-4137)
+ * with DERIVS_BK / DERIVS_BK_adia (mqct_eqtns.f:112-499, 720-879), the
+ * spline interpolation potential_data_bk / splint_bk (mqct_eqtns.f:501-611)
+ * and rk4 / rkqs / rkck / odeint (suppl_routines.f:1076-1463) for a batch
+ * of trajectories.                                                        */
+
+/* Spline-coefficient residency: Mat_el / Mat_el_der as READ_MIJ_USER leaves
+ * them (matrix_ini.f:5794-6640) plus the state energies Ej (Hartree) and
+ * quantum numbers.  mass = massAB_M (a.u.).  theta_on = bikram_theta. */
+int mqct_hip_load_matrix(mqct_hip_t *h, int n_states, const double *Ej,
+                         const int *j12, const int *m12, int64_t total_size,
+                         const int *ind_mat, int n_r, const double *R_com,
+                         const double *mat_el, const double *mat_el_der,
+                         double mass, int theta_on);
+/* Same, taking the matrix of the last mqct_hip_matrix_build from HBM. */
+int mqct_hip_use_built_matrix(mqct_hip_t *h, const double *Ej, double mass,
+                              int theta_on);
+
+/* Per-trajectory input (DOTRAJECT initial conditions, propagator.f:2431-
+ * 2566, and the $SYSTEM integrator keywords). */
+typedef struct {
+  int s_st;            /* initial state, 1-based */
+  int at_path;         /* AT modes: index of the recorded path */
+  double l_real;       /* orbital angular momentum quantum number */
+  double E_coll;       /* collision energy, Hartree */
+  double dt;           /* TIME_STEP/sqrt(U/300) */
+  double eps;          /* EPS_ODEINT */
+  double hmin;         /* MIN_TMSTP */
+  double R_start;      /* RMAX */
+  double time_lim;     /* TIME_LIM (<=0: none) */
+  int numb_oscl_prds;
+  int numb_orb_prds;
+  int max_steps;       /* safety cap on outer steps (<=0: none) */
+} mqct_traj_t;
+
+typedef struct {
+  double t_end;
+  int n_steps;
+  int n_rhs;
+  int fail;
+  double norm_err;
+} mqct_traj_stat_t;
+
+/* Recorded classical paths for the AT modes (the AT_<j12>_<m12>_<L>_<p>
+ * files of the first AT run, propagator.f:2994-3032): n_paths paths,
+ * path p has records [path_off[p], path_off[p+1]) of 8 doubles each
+ * (t, R, P_R, Theta, P_Theta, Phi, P_Phi, V). */
+int mqct_hip_load_paths(mqct_hip_t *h, int n_paths, const int64_t *path_off,
+                        const double *records);
+
+/* Propagate n_traj trajectories.  q_out: n_traj x (2N+8) final sys_var
+ * (AT modes fill the 2N amplitudes and leave the classical slots 0);
+ * stat_out (optional): per-trajectory statistics.
+ * batched=1 selects the interval-bucketed batched right-hand side (AT
+ * modes), batched=0 the one-trajectory-per-workgroup plain FP64 form. */
+int mqct_hip_propagate(mqct_hip_t *h, int mode, int n_traj,
+                       const mqct_traj_t *traj, int batched, double *q_out,
+                       mqct_traj_stat_t *stat_out);
+int mqct_hip_propagate_timing(mqct_hip_t *h, double *ms);
+
+/* CC_RK4 propagation that also writes the SAVE_TRAJECT records of the first
+ * AT run (propagator.f:3457-3462, 4330-4333): per trajectory up to rec_cap
+ * records of 8 doubles (t, R, P_R, Theta, P_Theta, Phi, P_Phi, V) into
+ * records[n_traj][rec_cap][8], the count into rec_n[n_traj]. */
+int mqct_hip_propagate_record(mqct_hip_t *h, int n_traj,
+                              const mqct_traj_t *traj, int rec_cap,
+                              double *records, int *rec_n, double *q_out,
+                              mqct_traj_stat_t *stat_out);
+
+/* ---------------------------------------------------------------------- */
+/* MTRX_UF.DAT (Fortran sequential unformatted, 4-byte record markers):
+ * PRINT_MIJ_USER (matrix_ini.f:6640-6780) writer and READ_MIJ_USER
+ * (matrix_ini.f:5794-6640) reader.  chan_qn: per channel, the quantum
+ * numbers the state records carry after (st, chann, j12, m12): type 1:
+ * {j}; type 4: {j, ka, kc}; type 0: {j1, ka1, kc1, j2, ka2, kc2}. */
+int mqct_hip_write_mtrx_uf(const char *path, int coll_type, int n_channels,
+                           int n_states, int64_t total_size, int n_r,
+                           const int *indx_chann, const int *j12,
+                           const int *m12, const int *chan_qn,
+                           const int *ind_mat, const double *R_com,
+                           const double *mat_el);
+/* Reads the header first (pass NULL arrays to get the sizes), then the
+ * data.  hdr = {coll_type, n_channels, n_states, total_size, n_r}. */
+int mqct_hip_read_mtrx_uf(const char *path, int64_t *hdr, int *indx_chann,
+                          int *j12, int *m12, int *chan_qn, int *ind_mat,
+                          double *R_com, double *mat_el);
+
+/* ---------------------------------------------------------------------- */
+/* Learned potential: an MLP (12 features -> 4 x 256 SiLU -> 1) trained and
+ * evaluated by the library's own HIP kernels; it stands in for the PES on
+ * the type-0 grid.  No reference counterpart (the reference has only the
+ * user PES); it plugs in where USER_DEFINED_PES is called.               */
+int mqct_hip_mlp_init(mqct_hip_t *h, uint64_t seed);
+/* One Adam step on a batch of n samples (features [n][12], targets [n]),
+ * returns the batch mean squared error through loss. */
+int mqct_hip_mlp_train_step(mqct_hip_t *h, int n, const float *x_dev,
+                            const float *y_dev, float lr, float *loss);
+/* Forward pass on n samples (device pointers). */
+int mqct_hip_mlp_forward(mqct_hip_t *h, int n, const float *x_dev,
+                         float *y_dev);
+/* Gradient of the mean squared error w.r.t. all parameters (for checks),
+ * parameter order: W1,b1,W2,b2,W3,b3,W4,b4,W5,b5 (W as [out][in], fp32). */
+int mqct_hip_mlp_grad(mqct_hip_t *h, int n, const float *x_dev,
+                      const float *y_dev, float *grad_host, float *loss);
+int mqct_hip_mlp_get_params(mqct_hip_t *h, float *params_host);
+int mqct_hip_mlp_set_params(mqct_hip_t *h, const float *params_host);
+int mqct_hip_mlp_n_params(void);
+/* Target scaling used by the surrogate on the grid: V = scale*y + shift for
+ * R >= r_min, the plain potential below r_min. */
+int mqct_hip_mlp_set_scaling(mqct_hip_t *h, double scale, double shift,
+                             double r_min);
+
+/* ---------------------------------------------------------------------- */
+/* Diagnostics: multiplies 16x16 tiles with one MFMA instruction to check the
+ * lane layouts the kernels assume.  which: 0 f64 16x16x4, 1 f32 16x16x4,
+ * 2 bf16 16x16x32 (inputs given as float, rounded to bf16).  A is 16xK,
+ * B is Kx16 row-major, C out 16x16 row-major (as double). */
+int mqct_hip_probe_mfma(int which, const double *A, const double *B,
+                        double *C);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

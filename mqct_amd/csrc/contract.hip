@@ -1,0 +1,444 @@
+// contract.hip -- the quadrature contraction of the matrix build on MFMA.
+//
+// For every m-block (states with equal m12; INI_ARRAYS pairs only those,
+// matrix_ini.f:553-630) and every R point:
+//   M_ab(R) = sum_g W_g(R) * (Re psi_a(g) Re psi_b(g) + Im psi_a(g) Im psi_b(g))
+// which is INTEGRATOR_TEMP's sum over the grid (pot_comp.f:1947-1952,
+// 2535-2557, 3718-3722) with the wavefunctions precomputed.  As a GEMM:
+// rows = pairs (a, b) of an (a-tile x b-tile) block, columns = R points,
+// reduction = grid points; the A operand is formed on the fly from two
+// basis-function rows staged in LDS (a Khatri-Rao product), the B operand is
+// W(g, R).  Only the lower triangle a >= b is written (ind_mat order:
+// k = kbase[state a] + rank of b in its block).
+//
+// Three operand precisions:
+//   FP64 : v_mfma_f64_16x16x4f64, FP64 accumulation.
+//   FP32 : v_mfma_f32_16x16x4f32, FP32 partial sums over each staged chunk
+//          of grid points added into FP64 accumulators.
+//   BF16X3: v_mfma_f32_16x16x32_bf16 on split operands a = a_hi + a_lo,
+//          w = w_hi + w_lo: a_hi w_hi + a_hi w_lo + a_lo w_hi, FP32 partial
+//          sums per chunk, FP64 accumulators.
+#include "mqct_internal.h"
+
+#include <type_traits>
+
+namespace {
+
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+typedef float flt4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+constexpr int TA = 16;   // a-states per tile
+constexpr int TB = 16;   // b-states per tile (MFMA rows)
+constexpr int NR = 64;   // R points per workgroup (4 MFMA column tiles)
+constexpr int NT = NR / 16;   // (bf16 kernel; the k4 kernel takes NRT)
+constexpr int WAVES = 4; // each wave owns TA/WAVES a-states
+constexpr int AW = TA / WAVES;
+
+struct OutMap {
+  const int *pos_state;
+  const long long *kbase;  // per state
+  const int *block_of;     // per position: block start position
+  int n_r;                 // columns computed (R points)
+  int ld;                  // row stride of the output per element
+  double conv;
+};
+
+// Accumulator element i of a 16x16 MFMA output tile held by `lane`:
+// column lane%16; row 4*(lane/16)+i for the FP32-accumulating instructions
+// (f32 16x16x4, bf16 16x16x32) and lane/16+4*i for f64 16x16x4 (checked by
+// mqct_hip_probe_mfma).
+template <bool F64>
+__device__ inline int c_row(int lane, int i) {
+  return F64 ? (lane >> 4) + 4 * i : 4 * (lane >> 4) + i;
+}
+__device__ inline int c_col(int lane) { return lane & 15; }
+
+template <bool F64>
+__device__ inline void store_tile(const OutMap &om, const TileDesc &t,
+                                  int a_local, int r0, int nt, int lane,
+                                  const double (&v)[4], double *out) {
+  int a = t.a0 + a_local;
+  if (a_local >= t.na) return;
+  int sa = om.pos_state[a];
+  long long base = om.kbase[sa];
+  int bstart = om.block_of[a];
+  int col = c_col(lane);
+  int r = r0 + nt * 16 + col;
+  if (r >= om.n_r) return;
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    int bl = c_row<F64>(lane, i);
+    if (bl >= t.nb) continue;
+    int b = t.b0 + bl;
+    if (b > a) continue;  // lower triangle only
+    long long k = base + (b - bstart);
+    out[k * om.ld + r] = v[i] * om.conv;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 16x16x4 family (FP64 / FP32 operands).  KC grid points per LDS stage.
+template <typename T, int KC, int NRT>
+__global__ __launch_bounds__(256) void contract_k4_kernel(
+    const T *__restrict__ pa_re, const T *__restrict__ pa_im,
+    const T *__restrict__ pb_re, const T *__restrict__ pb_im,
+    const T *__restrict__ W, long long G, int n_r,
+    const TileDesc *__restrict__ tiles, OutMap om, double *__restrict__ out) {
+  __shared__ T sAr[TA][KC + 1], sAi[TA][KC + 1];
+  __shared__ T sBr[TB][KC + 1], sBi[TB][KC + 1];
+  constexpr int NTT = NRT / 16;
+  __shared__ T sW[NRT][KC + 1];
+  const TileDesc t = tiles[blockIdx.x];
+  const int r0 = blockIdx.y * NRT;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const bool has_im = (pa_im != nullptr);
+
+  typedef typename std::conditional<sizeof(T) == 8, dbl4, flt4>::type accv;
+  accv acc[AW][NTT];
+  dbl4 acc64[AW][NTT];
+#pragma unroll
+  for (int a = 0; a < AW; a++)
+#pragma unroll
+    for (int n = 0; n < NTT; n++) {
+      acc[a][n] = accv{0, 0, 0, 0};
+      acc64[a][n] = dbl4{0, 0, 0, 0};
+    }
+
+  for (long long g0 = 0; g0 < G; g0 += KC) {
+    // stage basis rows and weights (zero-padded at the edges)
+    for (int e = tid; e < TA * KC; e += 256) {
+      int r = e / KC, c = e % KC;
+      long long g = g0 + c;
+      bool ok = (g < G);
+      bool oka = ok && r < t.na, okb = ok && r < t.nb;
+      long long ia = (long long)(t.a0 + r) * G + g;
+      long long ib = (long long)(t.b0 + r) * G + g;
+      sAr[r][c] = oka ? pa_re[ia] : T(0);
+      sBr[r][c] = okb ? pb_re[ib] : T(0);
+      if (has_im) {
+        sAi[r][c] = oka ? pa_im[ia] : T(0);
+        sBi[r][c] = okb ? pb_im[ib] : T(0);
+      }
+    }
+    for (int e = tid; e < NRT * KC; e += 256) {
+      int r = e / KC, c = e % KC;
+      long long g = g0 + c;
+      int rr = r0 + r;
+      sW[r][c] = (g < G && rr < n_r) ? W[(long long)rr * G + g] : T(0);
+    }
+    __syncthreads();
+    const int row = lane & 15, kq = lane >> 4;
+#pragma unroll 2
+    for (int ks = 0; ks < KC; ks += 4) {
+      int k = ks + kq;
+      T br = sBr[row][k];
+      T bi = has_im ? sBi[row][k] : T(0);
+      T wv[NTT];
+#pragma unroll
+      for (int n = 0; n < NTT; n++) wv[n] = sW[n * 16 + row][k];
+#pragma unroll
+      for (int a = 0; a < AW; a++) {
+        int al = wave * AW + a;
+        T av = sAr[al][k] * br;
+        if (has_im) av += sAi[al][k] * bi;
+#pragma unroll
+        for (int n = 0; n < NTT; n++) {
+          if constexpr (sizeof(T) == 8)
+            acc[a][n] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, wv[n], acc[a][n], 0, 0, 0);
+          else
+            acc[a][n] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, wv[n], acc[a][n], 0, 0, 0);
+        }
+      }
+    }
+    if constexpr (sizeof(T) == 4) {
+#pragma unroll
+      for (int a = 0; a < AW; a++)
+#pragma unroll
+        for (int n = 0; n < NTT; n++)
+#pragma unroll
+          for (int i = 0; i < 4; i++) {
+            acc64[a][n][i] += (double)acc[a][n][i];
+            acc[a][n][i] = 0.0f;
+          }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int a = 0; a < AW; a++)
+#pragma unroll
+    for (int n = 0; n < NTT; n++) {
+      double v[4];
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+        v[i] = (sizeof(T) == 8) ? (double)acc[a][n][i] : acc64[a][n][i];
+      store_tile<sizeof(T) == 8>(om, t, wave * AW + a, r0, n, lane, v, out);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Split-BF16 x3 kernel.  KC grid points per stage (multiple of 32).
+__device__ inline void split_bf16(float x, __bf16 &hi, __bf16 &lo) {
+  hi = (__bf16)x;
+  lo = (__bf16)(x - (float)hi);
+}
+
+template <int KC>
+__global__ __launch_bounds__(256) void contract_bf16x3_kernel(
+    const float *__restrict__ pa_re, const float *__restrict__ pa_im,
+    const float *__restrict__ pb_re, const float *__restrict__ pb_im,
+    const __bf16 *__restrict__ Whi, const __bf16 *__restrict__ Wlo,
+    long long G, int n_r, const TileDesc *__restrict__ tiles, OutMap om,
+    double *__restrict__ out) {
+  __shared__ float sAr[TA][KC + 4], sAi[TA][KC + 4];
+  __shared__ float sBr[TB][KC + 4], sBi[TB][KC + 4];
+  __shared__ __attribute__((aligned(16))) __bf16 sWh[NR][KC + 8];
+  __shared__ __attribute__((aligned(16))) __bf16 sWl[NR][KC + 8];
+  const TileDesc t = tiles[blockIdx.x];
+  const int r0 = blockIdx.y * NR;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const bool has_im = (pa_im != nullptr);
+  flt4 acc[AW][NT];
+  dbl4 acc64[AW][NT];
+#pragma unroll
+  for (int a = 0; a < AW; a++)
+#pragma unroll
+    for (int n = 0; n < NT; n++) {
+      acc[a][n] = flt4{0, 0, 0, 0};
+      acc64[a][n] = dbl4{0, 0, 0, 0};
+    }
+  for (long long g0 = 0; g0 < G; g0 += KC) {
+    for (int e = tid; e < TA * KC; e += 256) {
+      int r = e / KC, c = e % KC;
+      long long g = g0 + c;
+      bool ok = (g < G);
+      bool oka = ok && r < t.na, okb = ok && r < t.nb;
+      long long ia = (long long)(t.a0 + r) * G + g;
+      long long ib = (long long)(t.b0 + r) * G + g;
+      sAr[r][c] = oka ? pa_re[ia] : 0.0f;
+      sBr[r][c] = okb ? pb_re[ib] : 0.0f;
+      if (has_im) {
+        sAi[r][c] = oka ? pa_im[ia] : 0.0f;
+        sBi[r][c] = okb ? pb_im[ib] : 0.0f;
+      }
+    }
+    for (int e = tid; e < NR * KC; e += 256) {
+      int r = e / KC, c = e % KC;
+      long long g = g0 + c;
+      int rr = r0 + r;
+      bool ok = (g < G && rr < n_r);
+      sWh[r][c] = ok ? Whi[(long long)rr * G + g] : (__bf16)0.0f;
+      sWl[r][c] = ok ? Wlo[(long long)rr * G + g] : (__bf16)0.0f;
+    }
+    __syncthreads();
+    const int row = lane & 15, kq = lane >> 4;
+    for (int ks = 0; ks < KC; ks += 32) {
+      int k0 = ks + 8 * kq;
+      bf16x8 wh[NT], wl[NT];
+#pragma unroll
+      for (int n = 0; n < NT; n++) {
+        wh[n] = *reinterpret_cast<const bf16x8 *>(&sWh[n * 16 + row][k0]);
+        wl[n] = *reinterpret_cast<const bf16x8 *>(&sWl[n * 16 + row][k0]);
+      }
+      float br[8], bi[8];
+#pragma unroll
+      for (int j = 0; j < 8; j++) {
+        br[j] = sBr[row][k0 + j];
+        bi[j] = has_im ? sBi[row][k0 + j] : 0.0f;
+      }
+#pragma unroll
+      for (int a = 0; a < AW; a++) {
+        int al = wave * AW + a;
+        bf16x8 ah, alo;
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+          float p = sAr[al][k0 + j] * br[j];
+          if (has_im) p += sAi[al][k0 + j] * bi[j];
+          __bf16 h, l;
+          split_bf16(p, h, l);
+          ah[j] = h;
+          alo[j] = l;
+        }
+#pragma unroll
+        for (int n = 0; n < NT; n++) {
+          acc[a][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, wh[n], acc[a][n], 0, 0, 0);
+          acc[a][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, wl[n], acc[a][n], 0, 0, 0);
+          acc[a][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo, wh[n], acc[a][n], 0, 0, 0);
+        }
+      }
+    }
+#pragma unroll
+    for (int a = 0; a < AW; a++)
+#pragma unroll
+      for (int n = 0; n < NT; n++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          acc64[a][n][i] += (double)acc[a][n][i];
+          acc[a][n][i] = 0.0f;
+        }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int a = 0; a < AW; a++)
+#pragma unroll
+    for (int n = 0; n < NT; n++) {
+      double v[4];
+#pragma unroll
+      for (int i = 0; i < 4; i++) v[i] = acc64[a][n][i];
+      store_tile<false>(om, t, wave * AW + a, r0, n, lane, v, out);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Single-instruction probes of the lane layouts assumed above.
+__global__ void probe_kernel(int which, const double *A, const double *B,
+                             double *C) {
+  int lane = threadIdx.x;
+  int row = lane & 15, kq = lane >> 4;
+  if (which == 0) {
+    dbl4 c = {0, 0, 0, 0};
+    c = __builtin_amdgcn_mfma_f64_16x16x4f64(A[row * 4 + kq], B[kq * 16 + row], c, 0, 0, 0);
+    for (int i = 0; i < 4; i++) C[c_row<true>(lane, i) * 16 + c_col(lane)] = c[i];
+  } else if (which == 1) {
+    flt4 c = {0, 0, 0, 0};
+    c = __builtin_amdgcn_mfma_f32_16x16x4f32((float)A[row * 4 + kq], (float)B[kq * 16 + row], c, 0, 0, 0);
+    for (int i = 0; i < 4; i++) C[c_row<false>(lane, i) * 16 + c_col(lane)] = c[i];
+  } else {
+    bf16x8 a, b;
+    for (int j = 0; j < 8; j++) {
+      a[j] = (__bf16)(float)A[row * 32 + 8 * kq + j];
+      b[j] = (__bf16)(float)B[(8 * kq + j) * 16 + row];
+    }
+    flt4 c = {0, 0, 0, 0};
+    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+    for (int i = 0; i < 4; i++) C[c_row<false>(lane, i) * 16 + c_col(lane)] = c[i];
+  }
+}
+
+}  // namespace
+
+// Tile list: all (a-tile, b-tile) blocks with b-tile <= a-tile inside each
+// m-block.
+static void make_tiles(const mqct_hip *h, std::vector<TileDesc> &tiles) {
+  tiles.clear();
+  for (size_t blk = 0; blk + 1 < h->block_start.size(); blk++) {
+    int s0 = h->block_start[blk], s1 = h->block_start[blk + 1];
+    for (int a0 = s0; a0 < s1; a0 += TA)
+      for (int b0 = s0; b0 <= a0; b0 += TB) {
+        TileDesc t;
+        t.a0 = a0;
+        t.na = (s1 - a0) < TA ? (s1 - a0) : TA;
+        t.b0 = b0;
+        t.nb = (s1 - b0) < TB ? (s1 - b0) : TB;
+        tiles.push_back(t);
+      }
+  }
+}
+
+static int ensure_tiles(mqct_hip *h, int &ntiles) {
+  std::vector<TileDesc> tiles;
+  make_tiles(h, tiles);
+  ntiles = (int)tiles.size();
+  if (tiles.empty()) return MQCT_OK;
+  if (h->d_tiles.ensure(tiles.size() * sizeof(TileDesc)))
+    return mqct_fail(MQCT_ERR_NOMEM, "tiles");
+  HIP_TRY(hipMemcpyAsync(h->d_tiles.p, tiles.data(),
+                         tiles.size() * sizeof(TileDesc),
+                         hipMemcpyHostToDevice, h->stream));
+  return MQCT_OK;
+}
+
+static OutMap out_map(mqct_hip *h, int n_cols, int ld, double conv) {
+  OutMap om;
+  om.pos_state = h->d_pos_state.as<int>();
+  om.kbase = h->d_kbase.as<long long>();
+  // block_of per position is stored after kbase in d_kbase
+  om.block_of = reinterpret_cast<const int *>(om.kbase + h->n_states);
+  om.n_r = n_cols;
+  om.ld = ld;
+  om.conv = conv;
+  return om;
+}
+
+int mqct_run_contraction(mqct_hip *h, int precision, double conv) {
+  int ntiles = 0, rc;
+  if ((rc = ensure_tiles(h, ntiles))) return rc;
+  if (ntiles == 0) return MQCT_OK;
+  OutMap om = out_map(h, h->n_r, h->n_r, conv);
+  const bool im = (h->coll_type != 1) || !h->ref_xz;
+  double *out = h->d_mat.as<double>();
+  if (precision == MQCT_PREC_FP64) {
+    dim3 grid((unsigned)ntiles, (unsigned)((h->n_r + NR - 1) / NR));
+    const double *pbr = h->ref_xz ? h->d_psi_re.as<double>() : h->d_psib_re.as<double>();
+    const double *pbi = h->ref_xz ? h->d_psi_im.as<double>() : h->d_psib_im.as<double>();
+    contract_k4_kernel<double, 32, 64><<<grid, 256, 0, h->stream>>>(
+        h->d_psi_re.as<double>(), im ? h->d_psi_im.as<double>() : nullptr,
+        pbr, im ? pbi : nullptr, h->d_W64.as<double>(), h->G, h->n_r,
+        h->d_tiles.as<TileDesc>(), om, out);
+  } else if (precision == MQCT_PREC_FP32) {
+    dim3 grid((unsigned)ntiles, (unsigned)((h->n_r + NR - 1) / NR));
+    const float *pbr = h->ref_xz ? h->d_psi32_re.as<float>() : h->d_psib32_re.as<float>();
+    const float *pbi = h->ref_xz ? h->d_psi32_im.as<float>() : h->d_psib32_im.as<float>();
+    contract_k4_kernel<float, 64, 64><<<grid, 256, 0, h->stream>>>(
+        h->d_psi32_re.as<float>(), im ? h->d_psi32_im.as<float>() : nullptr,
+        pbr, im ? pbi : nullptr, h->d_W32.as<float>(), h->G, h->n_r,
+        h->d_tiles.as<TileDesc>(), om, out);
+  } else {
+    dim3 grid((unsigned)ntiles, (unsigned)((h->n_r + NR - 1) / NR));
+    const float *pbr = h->ref_xz ? h->d_psi32_re.as<float>() : h->d_psib32_re.as<float>();
+    const float *pbi = h->ref_xz ? h->d_psi32_im.as<float>() : h->d_psib32_im.as<float>();
+    contract_bf16x3_kernel<64><<<grid, 256, 0, h->stream>>>(
+        h->d_psi32_re.as<float>(), im ? h->d_psi32_im.as<float>() : nullptr,
+        pbr, im ? pbi : nullptr, h->d_Whi.as<__bf16>(), h->d_Wlo.as<__bf16>(),
+        h->G, h->n_r, h->d_tiles.as<TileDesc>(), om, out);
+  }
+  HIP_TRY(hipGetLastError());
+  return MQCT_OK;
+}
+
+// FP64 values of every element at the two cutoff R points, for the
+// mixed-precision forms: the MIJ_ZERO_CUT decision (matrix_ini.f:1991-1997)
+// is then taken from the same FP64 sums the FP64 build forms.  Output
+// [T][2] in d_cutv; W64 rows of the two R points in d_Wcut.
+int mqct_run_cutoff_fp64(mqct_hip *h, int ir1, int ir2, double conv) {
+  int ntiles = 0, rc;
+  if ((rc = ensure_tiles(h, ntiles))) return rc;
+  if (ntiles == 0) return MQCT_OK;
+  if (h->d_Wcut.ensure((size_t)2 * h->G * 8) ||
+      h->d_cutv.ensure((size_t)h->built_T * 2 * 8))
+    return mqct_fail(MQCT_ERR_NOMEM, "cutoff columns");
+  const double *V = h->d_V.as<double>();
+  const double *om_w = h->d_omega.as<double>();
+  mqct_weight_rows(h, V, om_w, ir1, ir2, h->d_Wcut.as<double>());
+  OutMap om = out_map(h, 2, 2, conv);
+  const bool im = (h->coll_type != 1) || !h->ref_xz;
+  const double *pbr = h->ref_xz ? h->d_psi_re.as<double>() : h->d_psib_re.as<double>();
+  const double *pbi = h->ref_xz ? h->d_psi_im.as<double>() : h->d_psib_im.as<double>();
+  dim3 grid((unsigned)ntiles, 1);
+  contract_k4_kernel<double, 32, 16><<<grid, 256, 0, h->stream>>>(
+      h->d_psi_re.as<double>(), im ? h->d_psi_im.as<double>() : nullptr,
+      pbr, im ? pbi : nullptr, h->d_Wcut.as<double>(), h->G, 2,
+      h->d_tiles.as<TileDesc>(), om, h->d_cutv.as<double>());
+  HIP_TRY(hipGetLastError());
+  return MQCT_OK;
+}
+
+extern "C" int mqct_hip_probe_mfma(int which, const double *A, const double *B,
+                                   double *C) {
+  if (which < 0 || which > 2) return mqct_fail(MQCT_ERR_ARG, "probe: which");
+  int K = (which == 2) ? 32 : 4;
+  double *dA, *dB, *dC;
+  HIP_TRY(hipMalloc(&dA, 16 * K * 8));
+  HIP_TRY(hipMalloc(&dB, 16 * K * 8));
+  HIP_TRY(hipMalloc(&dC, 256 * 8));
+  HIP_TRY(hipMemcpy(dA, A, 16 * K * 8, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(dB, B, 16 * K * 8, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemset(dC, 0, 256 * 8));
+  probe_kernel<<<1, 64>>>(which, dA, dB, dC);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpy(C, dC, 256 * 8, hipMemcpyDeviceToHost));
+  hipFree(dA);
+  hipFree(dB);
+  hipFree(dC);
+  return MQCT_OK;
+}

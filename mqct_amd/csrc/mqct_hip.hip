@@ -1,0 +1,685 @@
+// mqct_hip.hip -- C ABI entry points of libmqct_hip.so for the matrix build,
+// the post-processing kernel (cutoff + spline), and MTRX_UF.DAT I/O.
+#include "mqct_internal.h"
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+
+// ---------------------------------------------------------------------------
+static thread_local std::string g_err;
+
+void mqct_set_error(const std::string &msg) { g_err = msg; }
+int mqct_fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+int DevBuf::ensure(size_t n) {
+  if (n == 0) n = 8;
+  if (p && bytes >= n) return 0;
+  release();
+  if (hipMalloc(&p, n) != hipSuccess) {
+    p = nullptr;
+    bytes = 0;
+    return 1;
+  }
+  bytes = n;
+  return 0;
+}
+
+void DevBuf::release() {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+  bytes = 0;
+}
+
+template <class T>
+static int upload(mqct_hip *h, DevBuf &b, const T *src, size_t n) {
+  if (b.ensure(n * sizeof(T))) return mqct_fail(MQCT_ERR_NOMEM, "device alloc");
+  if (n)
+    HIP_TRY(hipMemcpyAsync(b.p, src, n * sizeof(T), hipMemcpyHostToDevice,
+                           h->stream));
+  return MQCT_OK;
+}
+
+// Clebsch-Gordan through the Racah 3j sum (restates suppl_routines.f:246-310
+// in double-argument form).
+static double lf(int n) { return mq_logfact(n); }
+static double w3js(int J1, int J2, int J3, int M1, int M2, int M3) {
+  if (M1 + M2 + M3 != 0) return 0.0;
+  int IA = J1 + J2;
+  if (J3 > IA) return 0.0;
+  int IB = J1 - J2;
+  if (J3 < std::abs(IB)) return 0.0;
+  int JSUM = J3 + IA, IC = J1 - M1, ID = J2 - M2;
+  if (JSUM % 2 || IC % 2 || ID % 2) return 0.0;
+  if (std::abs(M1) > J1 || std::abs(M2) > J2 || std::abs(M3) > J3) return 0.0;
+  int IE = J3 - J2 + M1, IF = J3 - J1 - M2;
+  int ZMIN = std::max(0, std::max(-IE, -IF));
+  int IG = IA - J3, IH = J2 + M2;
+  int ZMAX = std::min(IG, std::min(IH, IC));
+  double CC = 0.0;
+  for (int Z = ZMIN; Z <= ZMAX; Z += 2) {
+    double D = std::exp(lf(Z / 2) + lf((IG - Z) / 2) + lf((IC - Z) / 2) +
+                        lf((IH - Z) / 2) + lf((IE + Z) / 2) + lf((IF + Z) / 2));
+    if (Z % 4 != 0) D = -D;
+    CC += 1.0 / D;
+  }
+  double CC1 = std::exp(lf(IG / 2) + lf((J3 + IB) / 2) + lf((J3 - IB) / 2) -
+                        lf((JSUM + 2) / 2));
+  double CC2 = std::exp(lf((J1 + M1) / 2) + lf(IC / 2) + lf(IH / 2) +
+                        lf(ID / 2) + lf((J3 - M3) / 2) + lf((J3 + M3) / 2));
+  CC *= std::sqrt(CC1 * CC2);
+  if ((IB - M3) % 4 != 0) CC = -CC;
+  return CC;
+}
+double mq_cg(int j1, int j2, int j3, int m1, int m2, int m3) {
+  int e = -j1 + j2 - m3;
+  double s = (e % 2 == 0) ? 1.0 : -1.0;
+  return w3js(2 * j1, 2 * j2, 2 * j3, 2 * m1, 2 * m2, -2 * m3) *
+         std::sqrt(2.0 * j3 + 1.0) * s;
+}
+
+// ---------------------------------------------------------------------------
+extern "C" int mqct_hip_last_error(char *buf, int len) {
+  if (!buf || len <= 0) return (int)g_err.size();
+  std::snprintf(buf, (size_t)len, "%s", g_err.c_str());
+  return (int)g_err.size();
+}
+
+extern "C" int mqct_hip_version(void) { return 10000; }
+
+void mqct_dyn_free(mqct_hip *h);
+void mqct_mlp_free(mqct_hip *h);
+
+extern "C" int mqct_hip_create(int device, mqct_hip_t **out) {
+  if (!out) return mqct_fail(MQCT_ERR_ARG, "create: null handle pointer");
+  int ndev = 0;
+  HIP_TRY(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev)
+    return mqct_fail(MQCT_ERR_ARG, "create: no such device");
+  HIP_TRY(hipSetDevice(device));
+  hipDeviceProp_t prop;
+  HIP_TRY(hipGetDeviceProperties(&prop, device));
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return mqct_fail(MQCT_ERR_ARG, std::string("create: library is built for "
+                                               "gfx950, device is ") +
+                                       prop.gcnArchName);
+  mqct_hip *h = new mqct_hip();
+  h->device = device;
+  HIP_TRY(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+  for (auto &e : h->ev) HIP_TRY(hipEventCreate(&e));
+  *out = h;
+  return MQCT_OK;
+}
+
+extern "C" int mqct_hip_destroy(mqct_hip_t *h) {
+  if (!h) return MQCT_OK;
+  (void)hipSetDevice(h->device);
+  (void)hipStreamSynchronize(h->stream);
+  mqct_dyn_free(h);
+  mqct_mlp_free(h);
+  for (auto &e : h->ev) (void)hipEventDestroy(e);
+  (void)hipStreamDestroy(h->stream);
+  delete h;
+  return MQCT_OK;
+}
+
+// ---------------------------------------------------------------------------
+extern "C" int mqct_hip_set_basis(mqct_hip_t *h, int coll_type, int n_states,
+                                  const int *j12, const int *m12,
+                                  const int *indx_chann, int n_channels,
+                                  const int *j1_ch, const int *j2_ch,
+                                  int ldvec, const double *eig1,
+                                  const double *eig2) {
+  if (!h) return mqct_fail(MQCT_ERR_ARG, "set_basis: null handle");
+  if (coll_type != 1 && coll_type != 4 && coll_type != 0)
+    return mqct_fail(MQCT_ERR_ARG, "set_basis: supported SYS_TYPE are 1, 4, 0");
+  if (n_states <= 0 || n_channels <= 0 || !j12 || !m12 || !indx_chann || !j1_ch)
+    return mqct_fail(MQCT_ERR_ARG, "set_basis: empty basis");
+  if (coll_type != 1 && (!eig1 || ldvec <= 0))
+    return mqct_fail(MQCT_ERR_ARG, "set_basis: eigenvectors required");
+  if (coll_type == 0 && (!eig2 || !j2_ch))
+    return mqct_fail(MQCT_ERR_ARG, "set_basis: second-molecule data required");
+  HIP_TRY(hipSetDevice(h->device));
+  h->coll_type = coll_type;
+  h->n_states = n_states;
+  h->n_channels = n_channels;
+  h->ldvec = ldvec;
+  h->j12.assign(j12, j12 + n_states);
+  h->m12.assign(m12, m12 + n_states);
+  h->indx_chann.assign(indx_chann, indx_chann + n_states);
+  h->j1_ch.assign(j1_ch, j1_ch + n_channels);
+  h->j2_ch.assign(n_channels, 0);
+  if (j2_ch) h->j2_ch.assign(j2_ch, j2_ch + n_channels);
+  for (int s = 0; s < n_states; s++) {
+    int c = indx_chann[s];
+    if (c < 1 || c > n_channels)
+      return mqct_fail(MQCT_ERR_ARG, "set_basis: indx_chann out of range");
+    int jj = h->j1_ch[c - 1];
+    if (coll_type != 0 && (j12[s] != jj || std::abs(m12[s]) > jj))
+      return mqct_fail(MQCT_ERR_ARG, "set_basis: j12/m12 inconsistent with channel");
+    if (coll_type != 1 && 2 * jj + 1 > ldvec)
+      return mqct_fail(MQCT_ERR_ARG, "set_basis: ldvec too small");
+    if (coll_type == 0) {
+      int j2 = h->j2_ch[c - 1];
+      if (j12[s] < std::abs(jj - j2) || j12[s] > jj + j2 || std::abs(m12[s]) > j12[s])
+        return mqct_fail(MQCT_ERR_ARG, "set_basis: j12 violates the triangle rule");
+      if (2 * j2 + 1 > ldvec)
+        return mqct_fail(MQCT_ERR_ARG, "set_basis: ldvec too small");
+    }
+  }
+  if (eig1) h->eig1.assign(eig1, eig1 + (size_t)n_channels * ldvec);
+  else h->eig1.clear();
+  if (eig2) h->eig2.assign(eig2, eig2 + (size_t)n_channels * ldvec);
+  else h->eig2.clear();
+  // m-blocks in ascending m, states ascending inside a block
+  std::map<int, std::vector<int>> blocks;
+  for (int s = 0; s < n_states; s++) blocks[m12[s]].push_back(s);
+  h->pos_state.clear();
+  h->block_start.clear();
+  std::vector<int> rank(n_states), block_of;
+  for (auto &kv : blocks) {
+    h->block_start.push_back((int)h->pos_state.size());
+    int r = 0;
+    for (int s : kv.second) {
+      rank[s] = r++;
+      h->pos_state.push_back(s);
+    }
+  }
+  h->block_start.push_back((int)h->pos_state.size());
+  h->state_pos.assign(n_states, 0);
+  for (int p = 0; p < n_states; p++) h->state_pos[h->pos_state[p]] = p;
+  block_of.resize(n_states);
+  for (size_t b = 0; b + 1 < h->block_start.size(); b++)
+    for (int p = h->block_start[b]; p < h->block_start[b + 1]; p++)
+      block_of[p] = h->block_start[b];
+  // kbase: pairs before state s in ind_mat order (st1 outer, st2 <= st1)
+  h->kbase.assign(n_states, 0);
+  int64_t acc = 0;
+  for (int s = 0; s < n_states; s++) {
+    h->kbase[s] = acc;
+    acc += rank[s] + 1;
+  }
+  h->total_pairs = acc;
+  // device tables
+  std::vector<int> chan(2 * (size_t)n_states);
+  for (int s = 0; s < n_states; s++) {
+    chan[s] = h->j1_ch[indx_chann[s] - 1];
+    chan[n_states + s] = indx_chann[s] - 1;
+  }
+  std::vector<long long> kb(n_states + (n_states + 1) / 2 + 1);
+  for (int s = 0; s < n_states; s++) kb[s] = h->kbase[s];
+  std::memcpy(&kb[n_states], block_of.data(), sizeof(int) * n_states);
+  int rc;
+  if ((rc = upload(h, h->d_j12, h->j12.data(), n_states))) return rc;
+  if ((rc = upload(h, h->d_m12, h->m12.data(), n_states))) return rc;
+  if ((rc = upload(h, h->d_pos_state, h->pos_state.data(), n_states))) return rc;
+  if ((rc = upload(h, h->d_kbase, kb.data(), kb.size()))) return rc;
+  if ((rc = upload(h, h->d_chan, chan.data(), chan.size()))) return rc;
+  if (!h->eig1.empty() && (rc = upload(h, h->d_eig1, h->eig1.data(), h->eig1.size()))) return rc;
+  if (!h->eig2.empty() && (rc = upload(h, h->d_eig2, h->eig2.data(), h->eig2.size()))) return rc;
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  h->basis_ok = true;
+  h->psi_fresh = false;
+  h->built = false;
+  return MQCT_OK;
+}
+
+extern "C" int mqct_hip_set_grid(mqct_hip_t *h, const mqct_grid_t *g) {
+  if (!h || !g) return mqct_fail(MQCT_ERR_ARG, "set_grid: null argument");
+  if (!h->basis_ok) return mqct_fail(MQCT_ERR_STATE, "set_grid: call set_basis first");
+  HIP_TRY(hipSetDevice(h->device));
+  const int t = h->coll_type;
+  auto need = [](int n, const double *x, const double *w) {
+    return n >= 1 && x && w;
+  };
+  if (!need(g->n_beta, g->xb, g->wb))
+    return mqct_fail(MQCT_ERR_ARG, "set_grid: beta grid required");
+  if (t != 1 && !need(g->n_gamma, g->xg, g->wg))
+    return mqct_fail(MQCT_ERR_ARG, "set_grid: gamma grid required");
+  if (t == 0 && !(need(g->n_alpha2, g->xaa, g->waa) &&
+                  need(g->n_beta2, g->xbb, g->wbb) &&
+                  need(g->n_gamma2, g->xgg, g->wgg)))
+    return mqct_fail(MQCT_ERR_ARG, "set_grid: type-0 grids required");
+  h->n_beta = g->n_beta;
+  h->n_gamma = (t == 1) ? 1 : g->n_gamma;
+  h->n_alpha2 = (t == 0) ? g->n_alpha2 : 1;
+  h->n_beta2 = (t == 0) ? g->n_beta2 : 1;
+  h->n_gamma2 = (t == 0) ? g->n_gamma2 : 1;
+  h->ref_xz = (t == 4) ? (g->ref_xz ? 1 : 0) : 1;
+  const double PI = MQCT_PI;
+  // angles exactly as INTEGRATOR_TEMP forms them
+  h->beta.resize(h->n_beta);
+  for (int i = 0; i < h->n_beta; i++) {
+    double b = g->xbm + g->xbl * g->xb[i];
+    if (b < 0.0) b = 0.0;
+    if (b > PI) b = PI;
+    h->beta[i] = b;
+  }
+  auto map_eq = [&](int n, const double *x, double m, double l,
+                    std::vector<double> &o) {
+    o.resize(n);
+    for (int i = 0; i < n; i++) o[i] = g->eq_gamma ? x[i] : m + l * x[i];
+  };
+  if (t != 1) map_eq(h->n_gamma, g->xg, g->xgm, g->xgl, h->gamma);
+  else h->gamma.assign(1, 0.0);
+  if (t == 0) {
+    map_eq(h->n_alpha2, g->xaa, g->xaam, g->xaal, h->alpha2);
+    h->beta2.resize(h->n_beta2);
+    for (int i = 0; i < h->n_beta2; i++) h->beta2[i] = g->xbbm + g->xbbl * g->xbb[i];
+    map_eq(h->n_gamma2, g->xgg, g->xggm, g->xggl, h->gamma2);
+  } else {
+    h->alpha2.assign(1, 0.0);
+    h->beta2.assign(1, 0.0);
+    h->gamma2.assign(1, 0.0);
+  }
+  h->G = (int64_t)h->n_beta * h->n_gamma * h->n_alpha2 * h->n_beta2 * h->n_gamma2;
+  h->omega.resize(h->G);
+  // quadrature weight per point: every factor of the integrand except the
+  // two wavefunctions and V (pot_comp.f:1950-1951, 2554-2555, 3718-3722)
+  int64_t gi = 0;
+  if (t == 1) {
+    for (int i = 0; i < h->n_beta; i++)
+      h->omega[gi++] = g->wb[i] * std::sin(h->beta[i]) * g->xbl * 2.0 * PI;
+  } else if (t == 4) {
+    for (int i = 0; i < h->n_beta; i++)
+      for (int k = 0; k < h->n_gamma; k++)
+        h->omega[gi++] = g->wb[i] * g->wg[k] * std::sin(h->beta[i]) * g->xbl *
+                         g->xgl * 2.0 * PI;
+  } else {
+    for (int i2 = 0; i2 < h->n_alpha2; i2++)
+      for (int i3 = 0; i3 < h->n_beta; i3++)
+        for (int i4 = 0; i4 < h->n_beta2; i4++)
+          for (int i5 = 0; i5 < h->n_gamma; i5++)
+            for (int i6 = 0; i6 < h->n_gamma2; i6++)
+              h->omega[gi++] = g->wb[i3] * g->wg[i5] * std::sin(h->beta[i3]) *
+                               g->waa[i2] * g->wbb[i4] * g->wgg[i6] *
+                               std::sin(h->beta2[i4]) * g->xbl * g->xgl *
+                               g->xaal * g->xbbl * g->xggl * 2 * PI;
+  }
+  int rc;
+  if ((rc = upload(h, h->d_beta, h->beta.data(), h->beta.size()))) return rc;
+  if ((rc = upload(h, h->d_gamma, h->gamma.data(), h->gamma.size()))) return rc;
+  if ((rc = upload(h, h->d_alpha2, h->alpha2.data(), h->alpha2.size()))) return rc;
+  if ((rc = upload(h, h->d_beta2, h->beta2.data(), h->beta2.size()))) return rc;
+  if ((rc = upload(h, h->d_gamma2, h->gamma2.data(), h->gamma2.size()))) return rc;
+  if ((rc = upload(h, h->d_omega, h->omega.data(), h->omega.size()))) return rc;
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  h->grid_ok = true;
+  h->psi_fresh = false;
+  h->v_ok = false;
+  h->built = false;
+  return MQCT_OK;
+}
+
+static int set_r(mqct_hip *h, int n_r, const double *R) {
+  if (n_r < 2 || !R) return mqct_fail(MQCT_ERR_ARG, "R grid needs >= 2 points");
+  for (int i = 1; i < n_r; i++)
+    if (!(R[i] > R[i - 1]))
+      return mqct_fail(MQCT_ERR_ARG, "R grid must increase");
+  h->n_r = n_r;
+  h->R.assign(R, R + n_r);
+  return upload(h, h->d_R, R, n_r);
+}
+
+extern "C" int mqct_hip_set_vgrid(mqct_hip_t *h, int n_r, const double *R_com,
+                                  const double *V) {
+  if (!h || !V) return mqct_fail(MQCT_ERR_ARG, "set_vgrid: null argument");
+  if (!h->grid_ok) return mqct_fail(MQCT_ERR_STATE, "set_vgrid: call set_grid first");
+  HIP_TRY(hipSetDevice(h->device));
+  int rc = set_r(h, n_r, R_com);
+  if (rc) return rc;
+  int64_t n = (int64_t)n_r * h->G;
+  for (int64_t i = 0; i < n; i++)
+    if (V[i] != V[i]) return mqct_fail(MQCT_ERR_ARG, "ERROR IN POTENTIAL ROUTINE (NaN in V grid)");
+  if ((rc = upload(h, h->d_V, V, (size_t)n))) return rc;
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  h->v_ok = true;
+  h->built = false;
+  return MQCT_OK;
+}
+
+extern "C" int mqct_hip_eval_pes(mqct_hip_t *h, int n_r, const double *R_com,
+                                 int pes_kind, int n_par, const double *par) {
+  if (!h || !par || n_par <= 0) return mqct_fail(MQCT_ERR_ARG, "eval_pes: bad argument");
+  if (!h->grid_ok) return mqct_fail(MQCT_ERR_STATE, "eval_pes: call set_grid first");
+  int want = h->coll_type == 1 ? MQCT_PES_LEGENDRE
+             : h->coll_type == 4 ? MQCT_PES_REALSH : MQCT_PES_SITES;
+  if (pes_kind != want)
+    return mqct_fail(MQCT_ERR_ARG, "eval_pes: potential does not match SYS_TYPE");
+  HIP_TRY(hipSetDevice(h->device));
+  int rc = set_r(h, n_r, R_com);
+  if (rc) return rc;
+  if ((rc = upload(h, h->d_par, par, n_par))) return rc;
+  if ((rc = mqct_eval_pes_dev(h, pes_kind, n_par))) return rc;
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  h->v_ok = true;
+  h->built = false;
+  return MQCT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Cutoff (matrix_ini.f:1991-1997) and clamped spline (matrix_ini.f:2005-2016
+// with suppl_routines.f:914-965).  One thread per element k; the spline's
+// forward factors y2f depend only on the R grid and come precomputed, the
+// data-dependent u sweep is kept in the output array and back-substituted in
+// place.
+__global__ void post_kernel(int64_t T, int n, const double *__restrict__ R,
+                            const double *__restrict__ sig,
+                            const double *__restrict__ pp,
+                            const double *__restrict__ y2f, double cut,
+                            int ir1, int ir2,
+                            const double *__restrict__ cutv,
+                            double *__restrict__ M,
+                            double *__restrict__ D,
+                            unsigned char *__restrict__ skip) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= T) return;
+  double *y = M + k * n, *d = D + k * n;
+  // cutv (mixed precision): FP64 values at the two cutoff points
+  double a1 = cutv ? fabs(cutv[2 * k]) : fabs(y[ir1]);
+  double a2 = cutv ? fabs(cutv[2 * k + 1]) : fabs(y[ir2]);
+  if ((a1 > a2 ? a1 : a2) < cut) {
+    for (int i = 0; i < n; i++) { y[i] = 0.0; d[i] = 0.0; }
+    skip[k] = 1;
+    return;
+  }
+  skip[k] = 0;
+  double yp1 = (y[1] - y[0]) / (R[1] - R[0]);
+  double ypn = (y[n - 1] - y[n - 2]) / (R[n - 1] - R[n - 2]);
+  double u = (3.0 / (R[1] - R[0])) * ((y[1] - y[0]) / (R[1] - R[0]) - yp1);
+  d[0] = u;
+  double yim1 = y[0], yi = y[1];
+  for (int i = 1; i < n - 1; i++) {
+    double yip1 = y[i + 1];
+    u = (6.0 * ((yip1 - yi) / (R[i + 1] - R[i]) - (yi - yim1) / (R[i] - R[i - 1])) /
+             (R[i + 1] - R[i - 1]) -
+         sig[i] * u) /
+        pp[i];
+    d[i] = u;
+    yim1 = yi;
+    yi = yip1;
+  }
+  double qn = 0.5;
+  double un = (3.0 / (R[n - 1] - R[n - 2])) *
+              (ypn - (y[n - 1] - y[n - 2]) / (R[n - 1] - R[n - 2]));
+  double next = (un - qn * u) / (qn * y2f[n - 2] + 1.0);
+  d[n - 1] = next;
+  for (int i = n - 2; i >= 0; i--) {
+    next = y2f[i] * next + d[i];
+    d[i] = next;
+  }
+}
+
+int mqct_run_post(mqct_hip *h, double cut, int ir1, int ir2,
+                  const double *cutv) {
+  int n = h->n_r;
+  std::vector<double> sig(n, 0.0), pp(n, 1.0), y2f(n, 0.0);
+  y2f[0] = -0.5;
+  for (int i = 1; i < n - 1; i++) {
+    sig[i] = (h->R[i] - h->R[i - 1]) / (h->R[i + 1] - h->R[i - 1]);
+    pp[i] = sig[i] * y2f[i - 1] + 2.0;
+    y2f[i] = (sig[i] - 1.0) / pp[i];
+  }
+  std::vector<double> tab(3 * (size_t)n);
+  std::memcpy(&tab[0], sig.data(), n * 8);
+  std::memcpy(&tab[n], pp.data(), n * 8);
+  std::memcpy(&tab[2 * n], y2f.data(), n * 8);
+  int rc = upload(h, h->d_spl, tab.data(), tab.size());
+  if (rc) return rc;
+  const double *t = h->d_spl.as<double>();
+  int64_t T = h->built_T;
+  post_kernel<<<(unsigned)((T + 255) / 256), 256, 0, h->stream>>>(
+      T, n, h->d_R.as<double>(), t, t + n, t + 2 * n, cut, ir1 - 1, ir2 - 1,
+      cutv, h->d_mat.as<double>(), h->d_der.as<double>(),
+      h->d_skip.as<unsigned char>());
+  HIP_TRY(hipGetLastError());
+  return MQCT_OK;
+}
+
+extern "C" int mqct_hip_matrix_build(mqct_hip_t *h, int precision,
+                                     int64_t total_size, const int *ind_mat,
+                                     double conv_unit_e, double mij_zero_cut,
+                                     int ir_cut1, int ir_cut2, double *mat_el,
+                                     double *mat_el_der, uint8_t *skipped) {
+  if (!h) return mqct_fail(MQCT_ERR_ARG, "matrix_build: null handle");
+  if (!h->basis_ok || !h->grid_ok || !h->v_ok)
+    return mqct_fail(MQCT_ERR_STATE, "matrix_build: basis, grid and potential must be set");
+  if (precision < MQCT_PREC_FP64 || precision > MQCT_PREC_SURROGATE)
+    return mqct_fail(MQCT_ERR_ARG, "matrix_build: unknown precision");
+  if (precision == MQCT_PREC_SURROGATE)
+    return mqct_fail(MQCT_ERR_ARG, "matrix_build: surrogate form needs mqct_hip_surrogate_vgrid first; pass MQCT_PREC_BF16X3 after it");
+  if (total_size != h->total_pairs)
+    return mqct_fail(MQCT_ERR_ARG, "ERROR: total_size does not match the basis (INI_ARRAYS pair count)");
+  if (ir_cut1 < 1 || ir_cut1 > h->n_r || ir_cut2 < 1 || ir_cut2 > h->n_r)
+    return mqct_fail(MQCT_ERR_ARG, "matrix_build: cutoff R index out of range");
+  if (ind_mat) {
+    // must be INI_ARRAYS' order: st1 ascending, st2 ascending, equal m12
+    int64_t k = 0;
+    for (int s1 = 0; s1 < h->n_states; s1++)
+      for (int s2 = 0; s2 <= s1; s2++) {
+        if (h->m12[s1] != h->m12[s2]) continue;
+        if (ind_mat[2 * k] != s1 + 1 || ind_mat[2 * k + 1] != s2 + 1)
+          return mqct_fail(MQCT_ERR_ARG, "matrix_build: ind_mat is not the INI_ARRAYS order");
+        k++;
+      }
+  }
+  HIP_TRY(hipSetDevice(h->device));
+  const int64_t T = total_size;
+  if (h->d_mat.ensure((size_t)T * h->n_r * 8) ||
+      h->d_der.ensure((size_t)T * h->n_r * 8) || h->d_skip.ensure((size_t)T))
+    return mqct_fail(MQCT_ERR_NOMEM, "matrix_build: output does not fit in HBM");
+  h->built_T = T;
+  int rc;
+  HIP_TRY(hipEventRecord(h->ev[0], h->stream));
+  if (!h->psi_fresh) {
+    if ((rc = mqct_build_psi(h))) return rc;
+  }
+  HIP_TRY(hipEventRecord(h->ev[1], h->stream));
+  if ((rc = mqct_build_weights(h, precision))) return rc;
+  HIP_TRY(hipEventRecord(h->ev[2], h->stream));
+  if ((rc = mqct_run_contraction(h, precision, conv_unit_e))) return rc;
+  const double *cutv = nullptr;
+  if (precision != MQCT_PREC_FP64) {
+    if ((rc = mqct_run_cutoff_fp64(h, ir_cut1, ir_cut2, conv_unit_e))) return rc;
+    cutv = h->d_cutv.as<double>();
+  }
+  HIP_TRY(hipEventRecord(h->ev[3], h->stream));
+  if ((rc = mqct_run_post(h, mij_zero_cut, ir_cut1, ir_cut2, cutv))) return rc;
+  HIP_TRY(hipEventRecord(h->ev[4], h->stream));
+  HIP_TRY(hipEventSynchronize(h->ev[4]));
+  float ms;
+  for (int i = 0; i < 4; i++) {
+    HIP_TRY(hipEventElapsedTime(&ms, h->ev[i], h->ev[i + 1]));
+    h->timing[i] = ms;
+  }
+  HIP_TRY(hipEventElapsedTime(&ms, h->ev[0], h->ev[4]));
+  h->timing[4] = ms;
+  h->built = true;
+  if (mat_el || mat_el_der || skipped) {
+    rc = mqct_hip_matrix_get(h, mat_el, mat_el_der);
+    if (rc) return rc;
+    if (skipped)
+      HIP_TRY(hipMemcpy(skipped, h->d_skip.p, (size_t)T, hipMemcpyDeviceToHost));
+  }
+  return MQCT_OK;
+}
+
+extern "C" int mqct_hip_matrix_get(mqct_hip_t *h, double *mat_el,
+                                   double *mat_el_der) {
+  if (!h || !h->built) return mqct_fail(MQCT_ERR_STATE, "matrix_get: nothing built");
+  HIP_TRY(hipSetDevice(h->device));
+  size_t bytes = (size_t)h->built_T * h->n_r * 8;
+  if (mat_el) HIP_TRY(hipMemcpy(mat_el, h->d_mat.p, bytes, hipMemcpyDeviceToHost));
+  if (mat_el_der) HIP_TRY(hipMemcpy(mat_el_der, h->d_der.p, bytes, hipMemcpyDeviceToHost));
+  return MQCT_OK;
+}
+
+extern "C" int mqct_hip_build_timing(mqct_hip_t *h, double *ms5) {
+  if (!h || !ms5) return mqct_fail(MQCT_ERR_ARG, "build_timing: null argument");
+  for (int i = 0; i < 5; i++) ms5[i] = h->timing[i];
+  return MQCT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// MTRX_UF.DAT: Fortran sequential unformatted records, each framed by a
+// 4-byte little-endian byte count before and after (flang/gfortran).
+// Layout of PRINT_MIJ_USER, matrix_ini.f:6658-6769.
+namespace {
+struct RecWriter {
+  FILE *f;
+  std::vector<unsigned char> buf;
+  void i32(int v) { put(&v, 4); }
+  void f64(double v) { put(&v, 8); }
+  void put(const void *p, size_t n) {
+    const unsigned char *c = (const unsigned char *)p;
+    buf.insert(buf.end(), c, c + n);
+  }
+  bool flush() {
+    uint32_t n = (uint32_t)buf.size();
+    bool ok = fwrite(&n, 4, 1, f) == 1;
+    if (!buf.empty()) ok = ok && fwrite(buf.data(), 1, buf.size(), f) == buf.size();
+    ok = ok && fwrite(&n, 4, 1, f) == 1;
+    buf.clear();
+    return ok;
+  }
+};
+struct RecReader {
+  FILE *f;
+  std::vector<unsigned char> buf;
+  size_t pos = 0;
+  bool next() {
+    uint32_t n, n2;
+    if (fread(&n, 4, 1, f) != 1) return false;
+    buf.resize(n);
+    if (n && fread(buf.data(), 1, n, f) != n) return false;
+    if (fread(&n2, 4, 1, f) != 1 || n2 != n) return false;
+    pos = 0;
+    return true;
+  }
+  bool i32(int &v) { return get(&v, 4); }
+  bool f64(double &v) { return get(&v, 8); }
+  bool get(void *p, size_t n) {
+    if (pos + n > buf.size()) return false;
+    std::memcpy(p, buf.data() + pos, n);
+    pos += n;
+    return true;
+  }
+};
+int qn_count(int coll_type) { return coll_type == 1 ? 1 : coll_type == 4 ? 3 : 6; }
+}  // namespace
+
+extern "C" int mqct_hip_write_mtrx_uf(const char *path, int coll_type,
+                                      int n_channels, int n_states,
+                                      int64_t total_size, int n_r,
+                                      const int *indx_chann, const int *j12,
+                                      const int *m12, const int *chan_qn,
+                                      const int *ind_mat, const double *R_com,
+                                      const double *mat_el) {
+  if (!path || !indx_chann || !j12 || !m12 || !chan_qn || !ind_mat || !R_com || !mat_el)
+    return mqct_fail(MQCT_ERR_ARG, "write_mtrx_uf: null argument");
+  if (coll_type != 1 && coll_type != 4 && coll_type != 0)
+    return mqct_fail(MQCT_ERR_ARG, "write_mtrx_uf: SYS_TYPE 1, 4, 0 only");
+  FILE *f = std::fopen(path, "wb");
+  if (!f) return mqct_fail(MQCT_ERR_ARG, std::string("write_mtrx_uf: cannot open ") + path);
+  RecWriter w{f, {}};
+  bool ok = true;
+  w.i32(coll_type); ok &= w.flush();
+  w.i32(n_channels); ok &= w.flush();
+  w.i32(n_states); ok &= w.flush();
+  w.i32((int)total_size); ok &= w.flush();
+  w.i32(n_r); ok &= w.flush();
+  const int nq = qn_count(coll_type);
+  for (int s = 0; s < n_states && ok; s++) {
+    int c = indx_chann[s];
+    w.i32(s + 1);
+    w.i32(c);
+    w.i32(j12[s]);
+    w.i32(m12[s]);
+    for (int q = 0; q < nq; q++) w.i32(chan_qn[(size_t)(c - 1) * nq + q]);
+    ok &= w.flush();
+  }
+  for (int64_t k = 0; k < total_size && ok; k++) {
+    w.i32((int)(k + 1));
+    w.i32(ind_mat[2 * k]);
+    w.i32(ind_mat[2 * k + 1]);
+    ok &= w.flush();
+  }
+  for (int i = 0; i < n_r && ok; i++) {
+    w.i32(i + 1);
+    w.f64(R_com[i]);
+    ok &= w.flush();
+  }
+  for (int64_t k = 0; k < total_size && ok; k++) {
+    w.i32((int)(k + 1));
+    ok &= w.flush();
+    w.put(mat_el + (size_t)k * n_r, (size_t)n_r * 8);
+    ok &= w.flush();
+  }
+  ok &= std::fclose(f) == 0;
+  if (!ok) return mqct_fail(MQCT_ERR_ARG, "write_mtrx_uf: write failed");
+  return MQCT_OK;
+}
+
+extern "C" int mqct_hip_read_mtrx_uf(const char *path, int64_t *hdr,
+                                     int *indx_chann, int *j12, int *m12,
+                                     int *chan_qn, int *ind_mat,
+                                     double *R_com, double *mat_el) {
+  if (!path || !hdr) return mqct_fail(MQCT_ERR_ARG, "read_mtrx_uf: null argument");
+  FILE *f = std::fopen(path, "rb");
+  if (!f) return mqct_fail(MQCT_ERR_ARG, std::string("read_mtrx_uf: cannot open ") + path);
+  RecReader r{f, {}};
+  int v[5];
+  for (int i = 0; i < 5; i++) {
+    if (!r.next() || !r.i32(v[i])) {
+      std::fclose(f);
+      return mqct_fail(MQCT_ERR_ARG, "read_mtrx_uf: bad header");
+    }
+  }
+  for (int i = 0; i < 5; i++) hdr[i] = v[i];
+  const int coll_type = v[0], n_states = v[2], n_r = v[4];
+  const int64_t T = v[3];
+  if (!indx_chann) {
+    std::fclose(f);
+    return MQCT_OK;
+  }
+  const int nq = qn_count(coll_type);
+  bool ok = true;
+  for (int s = 0; s < n_states && ok; s++) {
+    int st, c, a, b;
+    ok = r.next() && r.i32(st) && r.i32(c) && r.i32(a) && r.i32(b) && st == s + 1;
+    if (!ok) break;
+    indx_chann[s] = c;
+    j12[s] = a;
+    m12[s] = b;
+    for (int q = 0; q < nq && ok; q++) {
+      int x;
+      ok = r.i32(x);
+      if (chan_qn) chan_qn[(size_t)(c - 1) * nq + q] = x;
+    }
+  }
+  for (int64_t k = 0; k < T && ok; k++) {
+    int kk, a, b;
+    ok = r.next() && r.i32(kk) && r.i32(a) && r.i32(b) && kk == k + 1;
+    ind_mat[2 * k] = a;
+    ind_mat[2 * k + 1] = b;
+  }
+  for (int i = 0; i < n_r && ok; i++) {
+    int ii;
+    double x;
+    ok = r.next() && r.i32(ii) && r.f64(x) && ii == i + 1;
+    R_com[i] = x;
+  }
+  for (int64_t k = 0; k < T && ok; k++) {
+    int kk;
+    ok = r.next() && r.i32(kk) && kk == k + 1 && r.next() &&
+         r.buf.size() == (size_t)n_r * 8;
+    if (ok) std::memcpy(mat_el + (size_t)k * n_r, r.buf.data(), (size_t)n_r * 8);
+  }
+  std::fclose(f);
+  if (!ok) return mqct_fail(MQCT_ERR_ARG, "read_mtrx_uf: truncated or malformed file");
+  return MQCT_OK;
+}

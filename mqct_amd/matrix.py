@@ -1,0 +1,153 @@
+"""Matrix build through the HIP library: the host-side mirror of MATRIX_MAKE's
+compute branch (matrix_ini.f:1916-2016) for a whole basis in one call.
+
+    with Device() as dev:
+        mb = MatrixBuilder(dev, workload)
+        mat_el, mat_el_der, skipped = mb.build(precision="fp64")
+
+Arrays follow the reference: mat_el[k, i] is Mat_el(i, k) (element k, R
+point i), Hartree.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib as L
+from .basis import AngularGrid, Basis
+
+PRECISIONS = {"fp64": L.MQCT_PREC_FP64, "fp32": L.MQCT_PREC_FP32,
+              "bf16x3": L.MQCT_PREC_BF16X3}
+
+
+class Device:
+    """One library handle on one GPU."""
+
+    def __init__(self, device=0):
+        self.lib = L.load()
+        h = ctypes.c_void_p()
+        L.check(self.lib.mqct_hip_create(device, ctypes.byref(h)), "mqct_hip_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.mqct_hip_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def grid_struct(g: AngularGrid):
+    keep = []
+
+    def arr(a, n):
+        if a is None:
+            a = np.zeros(n) if n == 1 else None
+        a = L.as_f64(a if a is not None else np.zeros(n))
+        keep.append(a)
+        return L.dptr(a)
+
+    one = np.ones(1)
+    s = L.GridT()
+    s.n_beta, s.xb, s.wb, s.xbm, s.xbl = g.n_beta, arr(g.xb, g.n_beta), arr(g.wb, g.n_beta), g.xbm, g.xbl
+    s.n_gamma, s.xg, s.xgm, s.xgl = g.n_gamma, arr(g.xg, g.n_gamma), g.xgm, g.xgl
+    s.wg = arr(g.wg if g.wg is not None else one, g.n_gamma)
+    s.n_alpha2, s.xaa, s.xaam, s.xaal = g.n_alpha2, arr(g.xaa, g.n_alpha2), g.xaam, g.xaal
+    s.waa = arr(g.waa if g.waa is not None else one, g.n_alpha2)
+    s.n_beta2, s.xbb, s.xbbm, s.xbbl = g.n_beta2, arr(g.xbb, g.n_beta2), g.xbbm, g.xbbl
+    s.wbb = arr(g.wbb if g.wbb is not None else one, g.n_beta2)
+    s.n_gamma2, s.xgg, s.xggm, s.xggl = g.n_gamma2, arr(g.xgg, g.n_gamma2), g.xggm, g.xggl
+    s.wgg = arr(g.wgg if g.wgg is not None else one, g.n_gamma2)
+    s.eq_gamma, s.ref_xz = g.eq_gamma, g.ref_xz
+    return s, keep
+
+
+class MatrixBuilder:
+    def __init__(self, dev: Device, wl, vgrid=None):
+        """wl: a workloads.Workload (basis, grid, R, potential).  vgrid, if
+        given, is the host V grid in POT_STORE order (E_UNITS); otherwise the
+        library evaluates the workload's synthetic potential on the GPU."""
+        self.dev, self.lib, self.h = dev, dev.lib, dev.h
+        self.wl = wl
+        b: Basis = wl.basis
+        self._keep = []
+        j12, m12, ich = L.as_i32(b.j12), L.as_i32(b.m12), L.as_i32(b.indx_chann)
+        j1, j2 = L.as_i32(b.j1_ch), (L.as_i32(b.j2_ch) if b.j2_ch is not None else None)
+        e1 = L.as_f64(b.eig1) if b.eig1 is not None else None
+        e2 = L.as_f64(b.eig2) if b.eig2 is not None else None
+        L.check(self.lib.mqct_hip_set_basis(self.h, b.coll_type, b.n_states, L.iptr(j12), L.iptr(m12),
+                                            L.iptr(ich), b.n_channels, L.iptr(j1), L.iptr(j2),
+                                            b.ldvec, L.dptr(e1), L.dptr(e2)), "set_basis")
+        gs, keep = grid_struct(wl.grid)
+        L.check(self.lib.mqct_hip_set_grid(self.h, ctypes.byref(gs)), "set_grid")
+        R = L.as_f64(wl.R)
+        self.R = R
+        if vgrid is not None:
+            V = L.as_f64(vgrid)
+            L.check(self.lib.mqct_hip_set_vgrid(self.h, len(R), L.dptr(R), L.dptr(V)), "set_vgrid")
+        else:
+            par = L.as_f64(wl.pes_par)
+            L.check(self.lib.mqct_hip_eval_pes(self.h, len(R), L.dptr(R), wl.pes_kind, len(par),
+                                               L.dptr(par)), "eval_pes")
+        self.ind_mat = L.as_i32(b.ind_mat)
+
+    def build(self, precision="fp64", copy_out=True):
+        prec = PRECISIONS[precision] if isinstance(precision, str) else precision
+        b = self.wl.basis
+        T, n = b.total_size, len(self.R)
+        ir1, ir2 = self.wl.ir_cut
+        if copy_out:
+            M = np.zeros((T, n))
+            D = np.zeros((T, n))
+            S = np.zeros(T, np.uint8)
+            L.check(self.lib.mqct_hip_matrix_build(self.h, prec, T, L.iptr(self.ind_mat), self.wl.conv_unit_e,
+                                                   self.wl.mij_zero_cut, ir1, ir2, L.dptr(M), L.dptr(D),
+                                                   S.ctypes.data_as(L.c_u8_p)), "matrix_build")
+            return M, D, S
+        L.check(self.lib.mqct_hip_matrix_build(self.h, prec, T, L.iptr(self.ind_mat), self.wl.conv_unit_e,
+                                               self.wl.mij_zero_cut, ir1, ir2, None, None, None),
+                "matrix_build")
+        return None
+
+    def timing(self):
+        t = np.zeros(5)
+        L.check(self.lib.mqct_hip_build_timing(self.h, L.dptr(t)), "build_timing")
+        return {"psi_ms": t[0], "weights_ms": t[1], "contract_ms": t[2], "post_ms": t[3], "total_ms": t[4]}
+
+
+def write_mtrx_uf(path, wl, mat_el):
+    lib = L.load()
+    b = wl.basis
+    qn = b.chan_qn()
+    L.check(lib.mqct_hip_write_mtrx_uf(path.encode(), b.coll_type, b.n_channels, b.n_states, b.total_size,
+                                       len(wl.R), L.iptr(L.as_i32(b.indx_chann)), L.iptr(L.as_i32(b.j12)),
+                                       L.iptr(L.as_i32(b.m12)), L.iptr(qn), L.iptr(L.as_i32(b.ind_mat)),
+                                       L.dptr(L.as_f64(wl.R)), L.dptr(L.as_f64(mat_el))), "write_mtrx_uf")
+
+
+def read_mtrx_uf(path):
+    lib = L.load()
+    hdr = np.zeros(5, np.int64)
+    L.check(lib.mqct_hip_read_mtrx_uf(path.encode(), hdr.ctypes.data_as(L.c_i64_p), None, None, None, None,
+                                      None, None, None), "read_mtrx_uf header")
+    coll, nch, ns, T, nr = (int(x) for x in hdr)
+    nq = {1: 1, 4: 3, 0: 6}[coll]
+    ich, j12, m12 = (np.zeros(ns, np.int32) for _ in range(3))
+    qn = np.zeros((nch, nq), np.int32)
+    ind = np.zeros((T, 2), np.int32)
+    R = np.zeros(nr)
+    M = np.zeros((T, nr))
+    L.check(lib.mqct_hip_read_mtrx_uf(path.encode(), hdr.ctypes.data_as(L.c_i64_p), L.iptr(ich), L.iptr(j12),
+                                      L.iptr(m12), L.iptr(qn), L.iptr(ind), L.dptr(R), L.dptr(M)),
+            "read_mtrx_uf")
+    return {"coll_type": coll, "n_channels": nch, "n_states": ns, "total_size": T, "n_r": nr,
+            "indx_chann": ich, "j12": j12, "m12": m12, "chan_qn": qn, "ind_mat": ind, "R": R, "mat_el": M}

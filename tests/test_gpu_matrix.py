@@ -1,0 +1,148 @@
+"""GPU parity of the matrix build (the HIP library through its C ABI) against
+the CPU oracle, on seeded inputs.
+
+Bar: FP64 elements within 1e-10 relative, except where the quadrature sum
+cancels: an element whose terms sum in absolute value to A may differ by
+1e-13*A (a few hundred roundings of the largest term -- any summation order
+does that), so the test is |gpu - oracle| <= max(1e-10 |oracle|, 1e-13 A).
+State tables, pair list and the cutoff mask are bit-exact.  Mixed-precision
+forms are held to their worst-case rounding bound against A: FP32 4e-6 A,
+split-BF16x3 6e-5 A; per-element errors are also reported relative to the
+element and to its largest value over R.
+"""
+import numpy as np
+import pytest
+
+from mqct_amd import workloads as W
+from mqct_amd.matrix import MatrixBuilder
+import oracle.oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_err(g, o, floor_frac=1e-12):
+    atol = floor_frac * np.abs(o).max()
+    return np.max(np.abs(g - o) / np.maximum(np.abs(o), atol))
+
+
+def fp64_ok(g, o, A):
+    """max over elements of |g-o| / max(1e-10|o|, 1e-13 A); <= 1 passes."""
+    return np.max(np.abs(g - o) / np.maximum(1e-10 * np.abs(o), 1e-13 * A))
+
+
+def abs_sums(orc, wl, ks=None, irs=None):
+    ks = range(wl.basis.total_size) if ks is None else ks
+    irs = range(len(wl.R)) if irs is None else irs
+    return orc.abs_sums(wl.basis.ind_mat, ks, irs) * wl.conv_unit_e
+
+
+def probe(dev, which, K, rng):
+    from mqct_amd import _lib as L
+    A = rng.standard_normal((16, K))
+    B = rng.standard_normal((K, 16))
+    if which == 2:
+        A = A.astype(np.float32).astype(np.float64)
+        B = B.astype(np.float32).astype(np.float64)
+        # bf16-exact values so the product is exact up to fp32 summation
+        A = np.round(A * 8) / 8
+        B = np.round(B * 8) / 8
+    C = np.zeros((16, 16))
+    L.check(dev.lib.mqct_hip_probe_mfma(which, L.dptr(np.ascontiguousarray(A)), L.dptr(np.ascontiguousarray(B)),
+                                        L.dptr(C)), "probe")
+    return A @ B, C
+
+
+@pytest.mark.parametrize("which,K,tol", [(0, 4, 1e-14), (1, 4, 1e-5), (2, 32, 1e-5)])
+def test_mfma_lane_layout(dev, which, K, tol):
+    ref, C = probe(dev, which, K, np.random.default_rng(which))
+    assert np.max(np.abs(C - ref)) <= tol * np.max(np.abs(ref))
+
+
+def build_both(dev, wl, precision="fp64"):
+    mb = MatrixBuilder(dev, wl)
+    M, D, S = mb.build(precision)
+    orc = O.oracle_for_workload(wl)
+    Mo, Do, So = orc.matrix_make(wl.basis.ind_mat, wl.conv_unit_e, wl.mij_zero_cut, *wl.ir_cut)
+    return (M, D, S), (Mo, Do, So), orc
+
+
+def test_diatom_atom_fp64_full_size(dev):
+    """121 states, 506 pairs, 64 GL points, 200 R: every element."""
+    wl = W.diatom_atom()
+    (M, D, S), (Mo, Do, So), orc = build_both(dev, wl)
+    assert np.array_equal(S, So)
+    A = abs_sums(orc, wl)
+    assert fp64_ok(M, Mo, A) <= 1.0
+    assert rel_err(D, Do, 1e-10) <= 1e-8
+
+
+def test_asym_top_atom_fp64_small(dev):
+    wl = W.asym_top_atom(jmax=3, n_beta=12, n_gamma=24, n_r=20)
+    (M, D, S), (Mo, Do, So), orc = build_both(dev, wl)
+    assert np.array_equal(S, So)
+    A = abs_sums(orc, wl)
+    assert fp64_ok(M, Mo, A) <= 1.0
+    assert rel_err(D, Do, 1e-10) <= 1e-8
+
+
+def test_asym_top_atom_refxz_no(dev):
+    """REF_XZ=NO integrand (pot_comp.f:2544-2553)."""
+    wl = W.asym_top_atom(jmax=2, n_beta=8, n_gamma=16, n_r=12, ref_xz=0)
+    (M, D, S), (Mo, Do, So), orc = build_both(dev, wl)
+    assert np.array_equal(S, So)
+    assert fp64_ok(M, Mo, abs_sums(orc, wl)) <= 1.0
+
+
+def test_two_asym_tops_fp64_small(dev):
+    wl = W.two_asym_tops(jmax1=1, jmax2=1, n_r=8, n_ang_each=(2, 3, 3, 2, 3))
+    (M, D, S), (Mo, Do, So), orc = build_both(dev, wl)
+    assert np.array_equal(S, So)
+    assert fp64_ok(M, Mo, abs_sums(orc, wl)) <= 1.0
+
+
+def test_asym_top_atom_full_size_sampled(dev):
+    """Full size (1771 states, 86 779 pairs, 24x48 grid, 300 R) on the GPU;
+    the oracle checks a seeded sample of 150 elements at 5 R points each."""
+    wl = W.asym_top_atom()
+    mb = MatrixBuilder(dev, wl)
+    M, D, S = mb.build("fp64")
+    orc = O.oracle_for_workload(wl)
+    rng = np.random.default_rng(7)
+    ks = rng.choice(wl.basis.total_size, 150, replace=False)
+    irs = [0, 57, 149, 233, 299]
+    A = abs_sums(orc, wl, ks, irs)
+    worst = 0.0
+    for a, k in enumerate(ks):
+        if S[k]:
+            continue
+        for b, ir in enumerate(irs):
+            o = orc.element(wl.basis.ind_mat, int(k), ir) * wl.conv_unit_e
+            worst = max(worst, abs(M[k, ir] - o) / max(1e-10 * abs(o), 1e-13 * A[a, b]))
+    assert worst <= 1.0
+
+
+MIXED_TOL = {"fp32": 4e-6, "bf16x3": 6e-5}
+
+
+def check_mixed(dev, wl, prec):
+    (M, D, S), (Mo, Do, So), orc = build_both(dev, wl, prec)
+    assert np.array_equal(S, So)  # the mask comes from FP64 sums
+    live = ~So.astype(bool)
+    A = abs_sums(orc, wl)
+    back = np.max((np.abs(M - Mo) / np.maximum(A, 1e-300))[live])
+    colmax = np.abs(Mo).max(axis=1, keepdims=True)
+    rel_col = np.max((np.abs(M - Mo) / np.maximum(colmax, 1e-300))[live])
+    rel_glob = np.max(np.abs(M - Mo)) / np.abs(Mo).max()
+    print(f"{wl.name} {prec}: |err|/A max {back:.3e}, |err|/colmax max {rel_col:.3e}, "
+          f"|err|/max|M| {rel_glob:.3e}")
+    assert back <= MIXED_TOL[prec]
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+def test_mixed_precision_diatom_atom(dev, prec):
+    check_mixed(dev, W.diatom_atom(), prec)
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+def test_mixed_precision_asym_top(dev, prec):
+    check_mixed(dev, W.asym_top_atom(jmax=3, n_beta=12, n_gamma=24, n_r=20), prec)
