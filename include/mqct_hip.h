@@ -37,7 +37,7 @@ enum {
   MQCT_PREC_FP64 = 0,      /* FP64 MFMA, FP64 accumulation */
   MQCT_PREC_FP32 = 1,      /* FP32 MFMA, FP32 chunk sums added in FP64 */
   MQCT_PREC_BF16X3 = 2,    /* split-BF16 hi*hi + hi*lo + lo*hi on MFMA */
-  MQCT_PREC_SURROGATE = 3  /* potential from the MLP surrogate, then BF16X3 */
+  MQCT_PREC_SURROGATE = 3  /* reserved: fill V with mqct_hip_surrogate_vgrid */
 };
 
 /* synthetic potentials evaluated on the GPU (parameter layouts in
@@ -216,28 +216,37 @@ int mqct_hip_read_mtrx_uf(const char *path, int64_t *hdr, int *indx_chann,
 
 /* ---------------------------------------------------------------------- */
 /* Learned potential: an MLP (12 features -> 4 x 256 SiLU -> 1) trained and
- * evaluated by the library's own HIP kernels; it stands in for the PES on
- * the type-0 grid.  No reference counterpart (the reference has only the
- * user PES); it plugs in where USER_DEFINED_PES is called.               */
+ * evaluated by the library's own HIP kernels (BF16 weights, FP32
+ * accumulation, Adam); it stands in for USER_DEFINED_PES on the SYS_TYPE 0
+ * grid.  No reference counterpart: it plugs in where the reference calls
+ * the user PES (pes_sys_type.f:88-97).  Geometries are (R, beta1, gamma1,
+ * alpha2, beta2, gamma2); the network predicts y with V = scale*sinh(y) +
+ * shift (cm^-1); below r_min the plain potential is kept. */
 int mqct_hip_mlp_init(mqct_hip_t *h, uint64_t seed);
-/* One Adam step on a batch of n samples (features [n][12], targets [n]),
- * returns the batch mean squared error through loss. */
-int mqct_hip_mlp_train_step(mqct_hip_t *h, int n, const float *x_dev,
-                            const float *y_dev, float lr, float *loss);
-/* Forward pass on n samples (device pointers). */
-int mqct_hip_mlp_forward(mqct_hip_t *h, int n, const float *x_dev,
-                         float *y_dev);
-/* Gradient of the mean squared error w.r.t. all parameters (for checks),
- * parameter order: W1,b1,W2,b2,W3,b3,W4,b4,W5,b5 (W as [out][in], fp32). */
-int mqct_hip_mlp_grad(mqct_hip_t *h, int n, const float *x_dev,
-                      const float *y_dev, float *grad_host, float *loss);
-int mqct_hip_mlp_get_params(mqct_hip_t *h, float *params_host);
-int mqct_hip_mlp_set_params(mqct_hip_t *h, const float *params_host);
 int mqct_hip_mlp_n_params(void);
-/* Target scaling used by the surrogate on the grid: V = scale*y + shift for
- * R >= r_min, the plain potential below r_min. */
+int mqct_hip_mlp_get_params(mqct_hip_t *h, float *params);
+int mqct_hip_mlp_set_params(mqct_hip_t *h, const float *params);
 int mqct_hip_mlp_set_scaling(mqct_hip_t *h, double scale, double shift,
                              double r_min);
+/* Training set kept on the device: geo[n][6], targets y[n]. */
+int mqct_hip_mlp_set_data(mqct_hip_t *h, int n, const double *geo,
+                          const float *y);
+/* One Adam step on samples [start, start+n) of the training set; loss =
+ * batch mean squared error before the step. */
+int mqct_hip_mlp_train_step(mqct_hip_t *h, int start, int n, float lr,
+                            float *loss);
+/* Mean-squared-error gradient on samples [start, start+n), parameter order
+ * W1,b1,W2,b2,W3,b3,W4,b4,W5,b5 with W as [out][in]. */
+int mqct_hip_mlp_grad(mqct_hip_t *h, int start, int n, float *grad,
+                      float *loss);
+int mqct_hip_mlp_predict(mqct_hip_t *h, int n, const double *geo, float *y);
+/* Fill the V grid of the current SYS_TYPE 0 grid from the network (then
+ * build with MQCT_PREC_BF16X3 or any other precision). */
+int mqct_hip_surrogate_vgrid(mqct_hip_t *h, int n_r, const double *R_com);
+/* The synthetic potential at n arbitrary geometries (training targets). */
+int mqct_hip_pes_points(mqct_hip_t *h, int pes_kind, int n_par,
+                        const double *par, int n, const double *geo,
+                        double *V);
 
 /* ---------------------------------------------------------------------- */
 /* Diagnostics: multiplies 16x16 tiles with one MFMA instruction to check the

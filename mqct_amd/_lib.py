@@ -97,16 +97,19 @@ SIGNATURES = {
     "mqct_hip_read_mtrx_uf": (ctypes.c_int, [ctypes.c_char_p, c_i64_p, c_int_p, c_int_p, c_int_p, c_int_p,
                                              c_int_p, c_dbl_p, c_dbl_p]),
     "mqct_hip_mlp_init": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64]),
-    "mqct_hip_mlp_train_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
-                                               ctypes.c_float, c_flt_p]),
-    "mqct_hip_mlp_forward": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
-    "mqct_hip_mlp_grad": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
-                                         c_flt_p, c_flt_p]),
+    "mqct_hip_mlp_n_params": (ctypes.c_int, []),
     "mqct_hip_mlp_get_params": (ctypes.c_int, [ctypes.c_void_p, c_flt_p]),
     "mqct_hip_mlp_set_params": (ctypes.c_int, [ctypes.c_void_p, c_flt_p]),
-    "mqct_hip_mlp_n_params": (ctypes.c_int, []),
     "mqct_hip_mlp_set_scaling": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
                                                 ctypes.c_double]),
+    "mqct_hip_mlp_set_data": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, c_dbl_p, c_flt_p]),
+    "mqct_hip_mlp_train_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                                               c_flt_p]),
+    "mqct_hip_mlp_grad": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, c_flt_p, c_flt_p]),
+    "mqct_hip_mlp_predict": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, c_dbl_p, c_flt_p]),
+    "mqct_hip_surrogate_vgrid": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, c_dbl_p]),
+    "mqct_hip_pes_points": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, c_dbl_p, ctypes.c_int,
+                                           c_dbl_p, c_dbl_p]),
     "mqct_hip_probe_mfma": (ctypes.c_int, [ctypes.c_int, c_dbl_p, c_dbl_p, c_dbl_p]),
 }
 

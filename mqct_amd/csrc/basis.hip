@@ -291,6 +291,18 @@ __global__ void pes_kernel(int coll_type, int pes_kind,
   V[idx] = v;
 }
 
+__global__ void pes_points_kernel(int pes_kind, const double *__restrict__ par, int n,
+                                  const double *__restrict__ geo, double *__restrict__ V) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double *p = geo + 6 * (long long)i;
+  double v;
+  if (pes_kind == MQCT_PES_LEGENDRE) v = pes_legendre(par, p[0], p[1]);
+  else if (pes_kind == MQCT_PES_REALSH) v = pes_realsh(par, p[0], p[1], p[2]);
+  else v = pes_sites(par, p[0], p[1], p[2], p[3], p[4], p[5]);
+  V[i] = v;
+}
+
 // W = omega * V in the requested operand precisions.
 __global__ void weight_kernel(int64_t G, int n_r, const double *__restrict__ V,
                               const double *__restrict__ omega,
@@ -501,4 +513,24 @@ void mqct_weight_rows(mqct_hip *h, const double *V, const double *omega,
                       int ir1, int ir2, double *out) {
   weight_rows_kernel<<<nblk(2 * h->G, 256), 256, 0, h->stream>>>(
       h->G, V, omega, ir1 - 1, ir2 - 1, out);
+}
+
+extern "C" int mqct_hip_pes_points(mqct_hip_t *h, int pes_kind, int n_par, const double *par,
+                                   int n, const double *geo, double *V) {
+  if (!h || !par || n_par <= 0 || n <= 0 || !geo || !V)
+    return mqct_fail(MQCT_ERR_ARG, "pes_points: bad argument");
+  if (pes_kind < MQCT_PES_LEGENDRE || pes_kind > MQCT_PES_SITES)
+    return mqct_fail(MQCT_ERR_ARG, "pes_points: unknown potential");
+  HIP_TRY(hipSetDevice(h->device));
+  DevBuf dp, dg, dv;
+  if (dp.ensure((size_t)n_par * 8) || dg.ensure((size_t)n * 48) || dv.ensure((size_t)n * 8))
+    return mqct_fail(MQCT_ERR_NOMEM, "pes_points");
+  HIP_TRY(hipMemcpy(dp.p, par, (size_t)n_par * 8, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(dg.p, geo, (size_t)n * 48, hipMemcpyHostToDevice));
+  pes_points_kernel<<<nblk(n, 256), 256, 0, h->stream>>>(pes_kind, dp.as<double>(), n, dg.as<double>(),
+                                                         dv.as<double>());
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  HIP_TRY(hipMemcpy(V, dv.p, (size_t)n * 8, hipMemcpyDeviceToHost));
+  return MQCT_OK;
 }

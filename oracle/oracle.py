@@ -389,7 +389,8 @@ class DynOracle:
         d.j12, d.m12 = ip(k(j12, i32)), ip(k(m12, i32))
         d.Ej = dp(k(Ej_hartree, f64))
         d.n_r = len(R)
-        d.R = dp(k(R, f64))
+        self.R = k(R, f64)
+        d.R = dp(self.R)
         d.mat, d.mat_der = dp(k(mat, f64)), dp(k(mat_der, f64))
         d.mass = mass
         d.theta_on = theta_on
@@ -422,6 +423,7 @@ class DynOracle:
 
     def dotraject(self, mode, s_st, l_real, E_coll, dt, eps=1e-8, hmin=1e-6, R_start=None,
                   numb_oscl_prds=1, numb_orb_prds=1, max_steps=0, rec_cap=0):
+        R_start = float(self.R[-1]) if R_start is None else R_start  # RMAX = last R point
         p = TrajParam(mode=mode, s_st=s_st, l_real=l_real, E_coll=E_coll, dt=dt, eps=eps, h1=dt, hmin=hmin,
                       R_start=R_start, time_lim=-1.0, numb_oscl_prds=numb_oscl_prds,
                       numb_orb_prds=numb_orb_prds, max_steps=max_steps)

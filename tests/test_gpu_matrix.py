@@ -27,7 +27,8 @@ def rel_err(g, o, floor_frac=1e-12):
 
 def fp64_ok(g, o, A):
     """max over elements of |g-o| / max(1e-10|o|, 1e-13 A); <= 1 passes."""
-    return np.max(np.abs(g - o) / np.maximum(1e-10 * np.abs(o), 1e-13 * A))
+    den = np.maximum(np.maximum(1e-10 * np.abs(o), 1e-13 * A), 1e-300)
+    return np.max(np.abs(g - o) / den)
 
 
 def abs_sums(orc, wl, ks=None, irs=None):
