@@ -187,6 +187,17 @@ int mqct_hip_propagate(mqct_hip_t *h, int mode, int n_traj,
                        mqct_traj_stat_t *stat_out);
 int mqct_hip_propagate_timing(mqct_hip_t *h, double *ms);
 
+/* Kernel timing of the batched form (a diagnostic with no counterpart in
+ * the reference).  on=1: the following batched propagations record HIP
+ * events on their stream around every launch of the interval GEMM (inside
+ * the captured graph) and accumulate; on=0 stops.  Each call resets the
+ * accumulators.  kernel_stats returns the GEMM launches, their summed
+ * duration (ms) and the algorithmic FLOPs they did: 16*S2 per trajectory
+ * right-hand side with R inside the grid, S2 = sum over m-blocks of nb^2. */
+int mqct_hip_batched_profile(mqct_hip_t *h, int on);
+int mqct_hip_batched_kernel_stats(mqct_hip_t *h, int64_t *launches,
+                                  double *gemm_ms, double *gemm_flops);
+
 /* CC_RK4 propagation that also writes the SAVE_TRAJECT records of the first
  * AT run (propagator.f:3457-3462, 4330-4333): per trajectory up to rec_cap
  * records of 8 doubles (t, R, P_R, Theta, P_Theta, Phi, P_Phi, V) into

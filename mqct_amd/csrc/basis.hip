@@ -187,6 +187,8 @@ __device__ double pes_legendre(const double *__restrict__ par, double R,
     }
     v += (par[1 + l] * exp(-par[1 + L + l] * R) - par[1 + 2 * L + l] / r6) * pl;
   }
+  const double *w = par + 1 + 3 * L;  // isotropic core Aw exp(-bw (R - Rw))
+  v += w[0] * exp(-w[1] * (R - w[2]));
   return v;
 }
 
@@ -213,6 +215,7 @@ __device__ double pes_realsh(const double *__restrict__ par, double R,
     v += rad * ang;
     p += 3 + 2 * l + 1;
   }
+  v += p[0] * exp(-p[1] * (R - p[2]));  // isotropic core
   return v;
 }
 

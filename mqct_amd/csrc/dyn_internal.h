@@ -15,7 +15,18 @@ struct DynState {
   int n_paths = 0;
   DevBuf d_scratch, d_trajp, d_q, d_stat, d_counter;
   // batched form
-  DevBuf d_bat;  // see batched.hip
+  DevBuf d_bat;      // trajectory states, buckets, controls
+  DevBuf d_KM;       // dense per-knot block matrices [n_r][2][S2]
+  long long bat_S2 = 0;
+  bool km_valid = false;  // d_KM holds the current matrix
+  // batched-form kernel timing
+  bool prof_on = false;
+  long long prof_launches = 0;
+  double prof_ms = 0.0, prof_flops = 0.0;
+  std::vector<hipEvent_t> prof_ev;
+  ~DynState() {
+    for (auto e : prof_ev) (void)hipEventDestroy(e);
+  }
   double last_ms = 0.0;
 };
 

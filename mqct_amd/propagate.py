@@ -152,3 +152,20 @@ def billing_scale(E_ch_cm, chann_ini, U_cm):
 def m_average_weight(m, jmin_ind, jmax_ind):
     """propagator.f:1268-1274: 2/(1+delta_m0) / ((jmax+jmin+1)(jmax-jmin+1))."""
     return 2.0 / (1.0 + (1.0 if m == 0 else 0.0)) / ((jmax_ind + jmin_ind + 1.0) * (jmax_ind - jmin_ind + 1.0))
+
+
+def at_first_step(dev, wl_small, b, E_cm, s_st=1, time_step=40.0, rec_cap=8000):
+    """The first AT run: CC-MQCT trajectories on a small basis that record
+    the classical path (SAVE_TRAJECT; propagator.f:3457-3462).  Returns the
+    path records and the integer L of every trajectory."""
+    from .matrix import MatrixBuilder
+    mb = MatrixBuilder(dev, wl_small)
+    mb.build("fp64", copy_out=False)
+    prop = Propagator(dev, wl_small.basis, wl_small.R, mass_amu=wl_small.mass_amu, from_build=True)
+    k = np.sqrt(prop.mass * 2.0 * np.asarray(E_cm) * CM_TO_HARTREE)
+    l = np.floor(np.asarray(b) * k)
+    rows = []
+    for i in range(len(b)):
+        rows += prop.trajectories([l[i]], float(E_cm[i]), s_st, time_step=time_step)
+    q, st, paths = prop.run_record(rows, rec_cap)
+    return paths, l, st

@@ -3,12 +3,12 @@
   diatom_atom      rigid diatom + atom (SYS_TYPE=1), B=1.93 cm^-1, mu=3.6 amu,
                    j=0..10 (11 channels, 121 states, 506 pairs), 64-point
                    Gauss-Legendre in theta, 200 R in 3..20 bohr,
-                   V = sum_{l<=6} (A e^{-bR} - C/R^6) P_l(cos theta).
+                   V = sum_{l<=6} (A e^{-bR} - C/R^6) P_l(cos theta) + core.
   asym_top_atom    asymmetric top + atom (SYS_TYPE=4), A/B/C = 27.9/14.5/9.3,
                    j=0..10 (121 channels, 1771 states, 86 779 pairs),
                    beta 24 Gauss-Legendre x gamma 48 equidistant, 300 R in
                    3..20 bohr, V = real spherical harmonics l<=8 with the
-                   same radial form, coefficients ~ 2^-l.
+                   same radial form, coefficients ~ 2^-l, + core.
   two_asym_tops    two water-like tops (SYS_TYPE=0), constants as above for
                    both, mu=9.0 amu, 3+3-site exp-6 + point charges,
                    16^5 angular grid, 128 R in 4..25 bohr.  The basis is
@@ -28,16 +28,32 @@ SEEDS = {"diatom_atom": 20240501, "asym_top_atom": 20240502, "two_asym_tops": 20
 ROT_A, ROT_B, ROT_C = 27.9, 14.5, 9.3
 
 
-def pes_legendre_params(seed, lmax=6):
+# Isotropic repulsive core added to the Legendre and real-harmonic
+# potentials: Aw exp(-bw (R - Rw)), 5000 cm^-1 at 3.5 bohr, below 0.25 cm^-1
+# beyond 6 bohr.  The radial form A exp(-bR) - C/R^6 with A <= 1e5, b >= 1.5
+# stays below 1.1e3 cm^-1 at R = 3 bohr and falls to -inf as R -> 0, so
+# without a core the trajectories of the 200..2000 cm^-1 workloads with small
+# impact parameters leave the R grid (R < R_1) into the spline's cubic
+# extrapolation.  With it the turning point of a 2000 cm^-1 head-on
+# collision lies near 3.7 bohr, inside the grid.
+CORE = (5000.0, 4.0, 3.5)
+
+
+def pes_legendre_params(seed, lmax=6, core=CORE):
+    """[L, A_0..A_{L-1}, b_0.., C_0.., Aw, bw, Rw] for
+    V = sum_l (A_l e^{-b_l R} - C_l / R^6) P_l(cos theta) + Aw e^{-bw (R - Rw)}."""
     rng = np.random.Generator(np.random.PCG64(seed))
     L = lmax + 1
     A = rng.uniform(1e3, 1e5, L)
     b = rng.uniform(1.5, 2.5, L)
     C = rng.uniform(1e4, 1e6, L)
-    return np.concatenate([[L], A, b, C]).astype(np.float64)
+    return np.concatenate([[L], A, b, C, core]).astype(np.float64)
 
 
-def pes_realsh_params(seed, lmax=8):
+def pes_realsh_params(seed, lmax=8, core=CORE):
+    """[lmax, (A_l, b_l, C_l, c_{l,-l..l}) for l = 0..lmax, Aw, bw, Rw] for
+    V = sum_l (A_l e^{-b_l R} - C_l / R^6) sum_m c_lm S_lm(beta, gamma)
+        + Aw e^{-bw (R - Rw)}, S_lm the real orthonormal spherical harmonics."""
     rng = np.random.Generator(np.random.PCG64(seed))
     out = [float(lmax)]
     for l in range(lmax + 1):
@@ -46,6 +62,7 @@ def pes_realsh_params(seed, lmax=8):
         C = rng.uniform(1e4, 1e6)
         c = rng.uniform(-1.0, 1.0, 2 * l + 1) * 2.0 ** (-l)
         out += [A, b, C, *c]
+    out += list(core)
     return np.array(out, np.float64)
 
 
@@ -127,3 +144,13 @@ def two_asym_tops(jmax1=2, jmax2=2, n_ang=16, n_r=128, seed=None, keep=None,
     R = B.r_grid(n_r, 4.0, 25.0)
     par = pes_sites_params(SEEDS["two_asym_tops"] if seed is None else seed)
     return Workload("two_asym_tops", b, g, R, 2, par, 9.0)
+
+
+def at_batch(n_traj=4096, b_max=18.0, e_min=200.0, e_max=2000.0, seed=20240504):
+    """The AT-MQCT propagation workload: trajectories of the asymmetric
+    top + atom system with impact parameters uniform in [0, b_max] bohr and
+    collision energies uniform in [e_min, e_max] cm^-1 (seeded)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    b = rng.uniform(0.0, b_max, n_traj)
+    E = rng.uniform(e_min, e_max, n_traj)
+    return b, E

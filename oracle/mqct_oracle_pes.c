@@ -45,6 +45,8 @@ double or_pes_legendre(const double *par, double R, double theta) {
   double r6 = R * R * R * R * R * R;
   for (int l = 0; l < L; l++)
     v += (A[l] * exp(-b[l] * R) - C[l] / r6) * legendre_p(l, x);
+  const double *w = par + 1 + 3 * L; /* isotropic core Aw exp(-bw (R - Rw)) */
+  v += w[0] * exp(-w[1] * (R - w[2]));
   return v;
 }
 
@@ -71,6 +73,7 @@ double or_pes_realsh(const double *par, double R, double beta, double gamma) {
     v += rad * ang;
     p += 3 + 2 * l + 1;
   }
+  v += p[0] * exp(-p[1] * (R - p[2])); /* isotropic core */
   return v;
 }
 

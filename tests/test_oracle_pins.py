@@ -182,7 +182,7 @@ def test_state_counts_of_the_workloads():
 
 # -- matrix elements ----------------------------------------------------------
 def test_type1_matrix_is_gaunt_closed_form():
-    """V = sum_l V_l(R) P_l(cos theta): <j m|V|j' m> = sum_l V_l (-1)^m
+    """V = sum_l V_l(R) P_l(cos theta) (the core in V_0): <j m|V|j' m> = sum_l V_l (-1)^m
     sqrt((2j+1)(2j'+1)) (j l j';0 0 0)(j l j';-m 0 m); the 64-point Gauss
     grid is exact for these degrees."""
     wl = W.diatom_atom(n_r=5)
@@ -190,9 +190,11 @@ def test_type1_matrix_is_gaunt_closed_form():
     orc = O.oracle_for_workload(wl)
     par = wl.pes_par
     Lp = int(par[0])
-    A, bb, C = par[1:1 + Lp], par[1 + Lp:1 + 2 * Lp], par[1 + 2 * Lp:]
+    A, bb, C = par[1:1 + Lp], par[1 + Lp:1 + 2 * Lp], par[1 + 2 * Lp:1 + 3 * Lp]
+    Aw, bw, Rw = par[1 + 3 * Lp:]
     for ir, R in enumerate(wl.R):
         Vl = A * np.exp(-bb * R) - C / R ** 6
+        Vl[0] += Aw * np.exp(-bw * (R - Rw))  # isotropic core
         for k in range(0, b.total_size, 7):
             s1, s2 = b.ind_mat[k] - 1
             j1, j2, m = int(b.j12[s1]), int(b.j12[s2]), int(b.m12[s1])
