@@ -67,6 +67,11 @@ def main():
     ap.add_argument("--n-traj", type=int, default=4096)
     ap.add_argument("--plain-sample", type=int, default=512)
     ap.add_argument("--xsec-sample", type=int, default=256)
+    ap.add_argument("--hmin", type=float, default=20.0,
+                    help="MIN_TMSTP (a.u.): the step floor of odeint's error control")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="time the steps with per-launch HIP events instead of replaying the captured graph "
+                         "(for runs under a kernel tracer)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-xsec", action="store_true")
@@ -103,24 +108,18 @@ def main():
     prop.load_paths(paths)
     rows = []
     for i in range(n):
-        rows += prop.trajectories([l[i]], float(E[i]), 1, time_step=10.0, eps=1e-8, hmin=1e-4, at_paths=[i])
+        rows += prop.trajectories([l[i]], float(E[i]), 1, time_step=10.0, eps=1e-8, hmin=args.hmin,
+                                  at_paths=[i])
 
-    # prime: knot matrices, kernels, graph with event nodes (not a step)
-    prof_ok = True
-    L.check(dev.lib.mqct_hip_batched_profile(dev.h, 1), "batched_profile")
-    try:
-        prop.run(L.MQCT_PROP_AT_ODEINT, rows[:8], batched=True)
-    except L.MqctError as e:
-        prof_ok = False
-        log("kernel event timing unavailable:", e)
-        L.check(dev.lib.mqct_hip_batched_profile(dev.h, 0), "batched_profile")
-        prop.run(L.MQCT_PROP_AT_ODEINT, rows[:8], batched=True)
+    # prime: knot matrices, kernels, graph (not a step)
+    prop.run(L.MQCT_PROP_AT_ODEINT, rows[:8], batched=True)
     for _ in range(args.warmup):
         prop.run(L.MQCT_PROP_AT_ODEINT, rows, batched=True)
 
-    # -- timed steps -----------------------------------------------------------
-    if prof_ok:
-        L.check(dev.lib.mqct_hip_batched_profile(dev.h, 1), "batched_profile")  # resets
+    # -- timed steps (graph-captured iterations) -------------------------------
+    import ctypes
+    if args.no_graph:
+        L.check(dev.lib.mqct_hip_batched_profile(dev.h, 1), "batched_profile")
     t0 = time.perf_counter()
     for _ in range(args.steps):
         q, st = prop.run(L.MQCT_PROP_AT_ODEINT, rows, batched=True)
@@ -128,24 +127,33 @@ def main():
     value = n * args.steps / el
     log("batched AT: %d traj x %d steps in %.2f s -> %.1f traj/s; rhs mean %.0f max %d; norm err max %.2e" %
         (n, args.steps, el, value, st["n_rhs"].mean(), st["n_rhs"].max(), st["norm_err"].max()))
+
+    # -- one more step with HIP events around every GEMM launch ---------------
     roof = None
-    if prof_ok:
-        import ctypes
-        la, ms, fl = ctypes.c_int64(), ctypes.c_double(), ctypes.c_double()
-        L.check(dev.lib.mqct_hip_batched_kernel_stats(dev.h, ctypes.byref(la), ctypes.byref(ms),
-                                                      ctypes.byref(fl)), "kernel_stats")
-        L.check(dev.lib.mqct_hip_batched_profile(dev.h, 0), "batched_profile")
-        if la.value > 0 and ms.value > 0:
-            avg_ms = ms.value / la.value
-            ach = fl.value / la.value / (avg_ms * 1e-3) / 1e12
-            roof = {"bound": "mfma", "achieved": round(ach, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                    "frac": round(ach / PEAK_FP64_TFLOPS, 4), "traffic": traffic_from_profiles(),
-                    "kernel": "gemm_kernel (interval GEMM, FP64 MFMA 16x16x4)",
-                    "launches": la.value, "avg_launch_us": round(avg_ms * 1e3, 2),
-                    "flops_per_launch": fl.value / la.value,
-                    "gemm_share_of_step": round(ms.value / (el * 1e3), 3)}
-            log("gemm: %d launches, avg %.1f us, %.2f TFLOP/s (%.1f%% of FP64 peak), %.0f%% of the step" %
-                (la.value, avg_ms * 1e3, ach, 100 * ach / PEAK_FP64_TFLOPS, 100 * ms.value / (el * 1e3)))
+    if args.no_graph:
+        q2, pel = q, el
+    else:
+        L.check(dev.lib.mqct_hip_batched_profile(dev.h, 1), "batched_profile")
+        t0 = time.perf_counter()
+        q2, _ = prop.run(L.MQCT_PROP_AT_ODEINT, rows, batched=True)
+        pel = time.perf_counter() - t0
+    la, ms, fl = ctypes.c_int64(), ctypes.c_double(), ctypes.c_double()
+    L.check(dev.lib.mqct_hip_batched_kernel_stats(dev.h, ctypes.byref(la), ctypes.byref(ms), ctypes.byref(fl)),
+            "kernel_stats")
+    L.check(dev.lib.mqct_hip_batched_profile(dev.h, 0), "batched_profile")
+    same = bool(np.array_equal(q2, q))
+    if la.value > 0 and ms.value > 0:
+        avg_ms = ms.value / la.value
+        ach = fl.value / la.value / (avg_ms * 1e-3) / 1e12
+        roof = {"bound": "mfma", "achieved": round(ach, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(ach / PEAK_FP64_TFLOPS, 4), "traffic": traffic_from_profiles(),
+                "kernel": "gemm_kernel (interval GEMM, FP64 MFMA 16x16x4)",
+                "launches": la.value, "avg_launch_us": round(avg_ms * 1e3, 2),
+                "flops_per_launch": fl.value / la.value,
+                "gemm_share_of_step": round(ms.value / (pel * 1e3), 3),
+                "timed_step_ms": round(pel * 1e3, 1), "same_result_as_graph_steps": same}
+        log("gemm: %d launches, avg %.1f us, %.2f TFLOP/s (%.1f%% of FP64 peak), %.0f%% of a %.2f s event-timed step"
+            % (la.value, avg_ms * 1e3, ach, 100 * ach / PEAK_FP64_TFLOPS, 100 * ms.value / (pel * 1e3), pel))
 
     # -- plain FP64 per-trajectory kernel on a sample ------------------------
     plain = None

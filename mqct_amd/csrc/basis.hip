@@ -455,6 +455,29 @@ int mqct_build_psi(mqct_hip *h) {
   return MQCT_OK;
 }
 
+long long mqct_pes_par_len(int pes_kind, int n_par, const double *par) {
+  if (n_par < 1) return -1;
+  long long need;
+  if (pes_kind == MQCT_PES_LEGENDRE) {
+    double L = par[0];
+    if (!(L >= 1 && L <= 64)) return -1;
+    need = 1 + 3 * (long long)L + 3;  // A, b, C per l; core triple
+  } else if (pes_kind == MQCT_PES_REALSH) {
+    double lmax = par[0];
+    if (!(lmax >= 0 && lmax <= 32)) return -1;
+    need = 1 + 3;
+    for (int l = 0; l <= (int)lmax; l++) need += 3 + 2 * l + 1;
+  } else if (pes_kind == MQCT_PES_SITES) {
+    if (n_par < 3) return -1;
+    double n1 = par[1], n2 = par[2];
+    if (!(n1 >= 1 && n1 <= 64 && n2 >= 1 && n2 <= 64)) return -1;
+    need = 3 + 6 * ((long long)n1 + (long long)n2);
+  } else {
+    return -1;
+  }
+  return need <= n_par ? need : -1;
+}
+
 int mqct_eval_pes_dev(mqct_hip *h, int pes_kind, int n_par) {
   (void)n_par;
   int64_t n = (int64_t)h->n_r * h->G;
@@ -524,6 +547,8 @@ extern "C" int mqct_hip_pes_points(mqct_hip_t *h, int pes_kind, int n_par, const
     return mqct_fail(MQCT_ERR_ARG, "pes_points: bad argument");
   if (pes_kind < MQCT_PES_LEGENDRE || pes_kind > MQCT_PES_SITES)
     return mqct_fail(MQCT_ERR_ARG, "pes_points: unknown potential");
+  if (mqct_pes_par_len(pes_kind, n_par, par) < 0)
+    return mqct_fail(MQCT_ERR_ARG, "pes_points: parameter array too short for its header");
   HIP_TRY(hipSetDevice(h->device));
   DevBuf dp, dg, dv;
   if (dp.ensure((size_t)n_par * 8) || dg.ensure((size_t)n * 48) || dv.ensure((size_t)n * 8))

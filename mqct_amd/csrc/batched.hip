@@ -177,60 +177,109 @@ __global__ __launch_bounds__(BS) void prep_kernel(BDev d) {
   }
 }
 
-// One workgroup per (knot, block): W[t][side][rows of b] = [Y | Y2] [wY v ; wY2 v]
-// for all bucket entries, 8 entries (16 real columns) per MFMA column tile.
+// Interval GEMM.  One workgroup per (knot r, m-block b, 64-row group):
+//   W[t][side][rows] = [Y_r | Y2_r] [w_Y v_t ; w_Y2 v_t]
+// for every bucket entry (t, side) of knot r, 64 entries (128 real columns:
+// Re and Im of each v) per column super-tile.  Each wave owns a 16-row tile
+// and keeps 8 column tiles of f64 16x16x4 MFMA accumulators; the weighted
+// vectors are staged through LDS in slices of GK rows of K = 2 nb, the
+// matrix fragments come straight from HBM/L2 (the blocks are symmetric, so
+// lanes of one k read 16 consecutive rows).
+constexpr int GE = 64;            // entries per column super-tile
+constexpr int GC = 2 * GE / 16;   // MFMA column tiles
+constexpr int GK = 16;            // K rows per LDS slice
+constexpr int GLD = 2 * GE + 16;  // LDS row stride in doubles (half-waves hit disjoint banks)
+
 __global__ __launch_bounds__(BS) void gemm_kernel(BDev d) {
   const int r = blockIdx.x, b = blockIdx.y;
   const int ne = d.cnt[r];
   if (ne == 0) return;
   const int p0 = d.blk_start[b], nb = d.blk_start[b + 1] - p0;
+  const int row0 = blockIdx.z * 64;
+  if (row0 >= nb) return;
   const int K = 2 * nb;
   const double *Y = d.KM + ((long long)r * 2) * d.S2 + d.blk_off[b];
   const double *Y2 = d.KM + ((long long)r * 2 + 1) * d.S2 + d.blk_off[b];
-  extern __shared__ double sV[];  // [K][16]
+  __shared__ double Bs[GK][GLD];
+  __shared__ int s_t[GE], s_side[GE];
+  __shared__ double s_wy[GE], s_wy2[GE];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int rt0 = row0 + wave * 16;
+  const bool wave_rows = rt0 < nb;
+  const int rr = rt0 + (lane & 15), kq = lane >> 4;
   const int2 *bk = d.bucket + (long long)r * d.n_traj;
-  const int row = lane & 15, kq = lane >> 4;
-  const int nrt = (nb + 15) / 16;
-  for (int e0 = 0; e0 < ne; e0 += 8) {
-    // stage the weighted vectors: column 2e = Re, 2e+1 = Im of entry e
-    for (int idx = tid; idx < K * 16; idx += BS) {
-      int k = idx / 16, col = idx % 16, e = e0 + col / 2;
-      double val = 0.0;
-      if (e < ne) {
-        int2 ent = bk[e];
+  for (int e0 = 0; e0 < ne; e0 += GE) {
+    const int nE = min(GE, ne - e0);
+    const int nct = (2 * nE + 15) / 16;
+    __syncthreads();
+    if (tid < GE) {
+      int t = -1, side = 0;
+      double wy = 0.0, wy2 = 0.0;
+      if (tid < nE) {
+        int2 ent = bk[e0 + tid];
         const Ctl &c = d.ctl[ent.x];
-        int li = k < nb ? k : k - nb;
-        double w = (k < nb) ? (ent.y == 0 ? c.aa : c.bb) : (ent.y == 0 ? c.c1 : c.c2);
-        double2 v = d.V[(long long)ent.x * d.N + p0 + li];
-        val = w * ((col & 1) ? v.y : v.x);
+        t = ent.x;
+        side = ent.y;
+        wy = side ? c.bb : c.aa;
+        wy2 = side ? c.c2 : c.c1;
       }
-      sV[k * 16 + col] = val;
+      s_t[tid] = t;
+      s_side[tid] = side;
+      s_wy[tid] = wy;
+      s_wy2[tid] = wy2;
     }
-    __syncthreads();
-    for (int rt = wave; rt < nrt; rt += BS / 64) {
-      dbl4 acc = {0, 0, 0, 0};
-      const int rr = rt * 16 + row;
-      for (int k0 = 0; k0 < K; k0 += 4) {
-        int k = k0 + kq;
+    dbl4 acc[GC];
+#pragma unroll
+    for (int ct = 0; ct < GC; ct++) acc[ct] = dbl4{0.0, 0.0, 0.0, 0.0};
+    for (int k0 = 0; k0 < K; k0 += GK) {
+      __syncthreads();
+      for (int idx = tid; idx < GK * GE; idx += BS) {
+        const int kk = idx % GK, e = idx / GK, k = k0 + kk;
+        double vx = 0.0, vy = 0.0;
+        if (e < nE && k < K) {
+          const bool lo = k < nb;
+          const double w = lo ? s_wy[e] : s_wy2[e];
+          const double2 x = d.V[(long long)s_t[e] * d.N + p0 + (lo ? k : k - nb)];
+          vx = w * x.x;
+          vy = w * x.y;
+        }
+        Bs[kk][2 * e] = vx;
+        Bs[kk][2 * e + 1] = vy;
+      }
+      double af[GK / 4];
+#pragma unroll
+      for (int s4 = 0; s4 < GK / 4; s4++) {
+        const int k = k0 + 4 * s4 + kq;
         double a = 0.0;
-        // the block matrices are symmetric: read column rr of row k (lanes of
-        // equal k read consecutive addresses)
         if (rr < nb && k < K) a = (k < nb) ? Y[(long long)k * nb + rr] : Y2[(long long)(k - nb) * nb + rr];
-        double bv = (k < K) ? sV[k * 16 + row] : 0.0;
-        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bv, acc, 0, 0, 0);
+        af[s4] = a;
       }
-      // f64 16x16x4 output: row lane/16 + 4*i, column lane%16
-      for (int i = 0; i < 4; i++) {
-        int orow = rt * 16 + (lane >> 4) + 4 * i;
-        int col = lane & 15, e = e0 + col / 2;
-        if (orow >= nb || e >= ne) continue;
-        int2 ent = bk[e];
-        double2 *w = d.W + ((long long)ent.x * 2 + ent.y) * d.N + p0 + orow;
-        if (col & 1) w->y = acc[i]; else w->x = acc[i];
+      __syncthreads();
+      if (wave_rows) {
+#pragma unroll
+        for (int s4 = 0; s4 < GK / 4; s4++) {
+          const double *brow = &Bs[4 * s4 + kq][lane & 15];
+#pragma unroll
+          for (int ct = 0; ct < GC; ct++)
+            if (ct < nct) acc[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[s4], brow[16 * ct], acc[ct], 0, 0, 0);
+        }
       }
     }
-    __syncthreads();
+    // f64 16x16x4 output: row lane/16 + 4i, column lane%16 of each tile
+    if (wave_rows) {
+#pragma unroll
+      for (int ct = 0; ct < GC; ct++) {
+        if (ct >= nct) break;
+        const int col = 16 * ct + (lane & 15), e = col >> 1;
+        if (e >= nE) continue;
+        double *wb = reinterpret_cast<double *>(d.W + ((long long)s_t[e] * 2 + s_side[e]) * d.N + p0) + (col & 1);
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          const int orow = rt0 + (lane >> 4) + 4 * i;
+          if (orow < nb) wb[2 * orow] = acc[ct][i];
+        }
+      }
+    }
   }
 }
 
@@ -459,6 +508,7 @@ int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *t
   DynState *ds = h->dyn;
   const int N = ds->N, n_r = ds->n_r;
   const long long T = ds->T;
+  (void)hipGetLastError();  // a failed earlier call must not be reported here
   // block order of the states (m12 ascending, states ascending)
   std::vector<int> pos_state, state_pos(N), blk_start, pos_blk(N);
   std::vector<long long> blk_off;
@@ -484,7 +534,7 @@ int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *t
   const int nblk = (int)blk_start.size() - 1;
   int maxnb = 0;
   for (int b = 0; b < nblk; b++) maxnb = std::max(maxnb, blk_start[b + 1] - blk_start[b]);
-  if ((size_t)2 * maxnb * 16 * 8 > 64 * 1024)
+  if (maxnb > 64 * 1024)
     return mqct_fail(MQCT_ERR_ARG, "propagate: an m-block is too large for the batched form");
   // per position: energies, Coriolis neighbours (state-1 / state+1) and factors
   std::vector<double> E(N), sq1(N), sq2(N);
@@ -571,37 +621,58 @@ int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *t
   init_kernel<<<n_traj, BS, 0, h->stream>>>(d, d_tr.as<mqct_traj_t>(), d_sp.as<int>());
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(h->stream));
-  const size_t lds = (size_t)2 * maxnb * 16 * 8;
+  const int rgroups = (maxnb + 63) / 64;
   // capture ITER iterations in one graph
   const int ITER = 16;
-  hipGraph_t graph;
-  hipGraphExec_t exec;
-  HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
   const bool prof = ds->prof_on;
-  if (prof && ds->prof_ev.empty()) {
-    ds->prof_ev.resize(2 * ITER);
-    for (auto &e : ds->prof_ev) HIP_TRY(hipEventCreate(&e));
-  }
-  for (int it = 0; it < ITER; it++) {
-    prep_kernel<<<n_traj, BS, 0, h->stream>>>(d);
-    if (prof) HIP_TRY(hipEventRecord(ds->prof_ev[2 * it], h->stream));
-    gemm_kernel<<<dim3(n_r, nblk), BS, lds, h->stream>>>(d);
-    if (prof) HIP_TRY(hipEventRecord(ds->prof_ev[2 * it + 1], h->stream));
-    finish_kernel<<<n_traj, BS, 0, h->stream>>>(d);
-  }
-  HIP_TRY(hipStreamEndCapture(h->stream, &graph));
-  HIP_TRY(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-  HIP_TRY(hipEventRecord(h->ev[5], h->stream));
   int active = n_traj;
   // every trajectory ends after at most MAXSTP accepted steps and a bounded
   // number of step-size reductions; the cap only guards the host loop
-  const int max_replays = 1 << 17;
-  int rep = 0;
-  for (; rep < max_replays && active > 0; rep++) {
-    HIP_TRY(hipGraphLaunch(exec, h->stream));
-    HIP_TRY(hipMemcpyAsync(&active, d.active, 4, hipMemcpyDeviceToHost, h->stream));
-    HIP_TRY(hipStreamSynchronize(h->stream));
-    if (prof) {
+  const int max_rounds = 1 << 17;
+  HIP_TRY(hipEventRecord(h->ev[5], h->stream));
+  if (!prof) {
+    // ITER iterations (prep, GEMM, finish) captured once, replayed until
+    // every trajectory has ended
+    struct GraphGuard {
+      hipGraph_t g = nullptr;
+      hipGraphExec_t e = nullptr;
+      ~GraphGuard() {
+        if (e) (void)hipGraphExecDestroy(e);
+        if (g) (void)hipGraphDestroy(g);
+      }
+    } gg;
+    HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    for (int it = 0; it < ITER; it++) {
+      prep_kernel<<<n_traj, BS, 0, h->stream>>>(d);
+      gemm_kernel<<<dim3(n_r, nblk, rgroups), BS, 0, h->stream>>>(d);
+      finish_kernel<<<n_traj, BS, 0, h->stream>>>(d);
+    }
+    hipError_t cap = hipStreamEndCapture(h->stream, &gg.g);
+    HIP_TRY(cap);
+    HIP_TRY(hipGraphInstantiate(&gg.e, gg.g, nullptr, nullptr, 0));
+    for (int rep = 0; rep < max_rounds && active > 0; rep++) {
+      HIP_TRY(hipGraphLaunch(gg.e, h->stream));
+      HIP_TRY(hipMemcpyAsync(&active, d.active, 4, hipMemcpyDeviceToHost, h->stream));
+      HIP_TRY(hipStreamSynchronize(h->stream));
+    }
+  } else {
+    // the same iterations launched one by one with HIP events on this
+    // stream around every GEMM launch (kernel timing)
+    if (ds->prof_ev.empty()) {
+      ds->prof_ev.resize(2 * ITER);
+      for (auto &e : ds->prof_ev) HIP_TRY(hipEventCreate(&e));
+    }
+    for (int rep = 0; rep < max_rounds && active > 0; rep++) {
+      for (int it = 0; it < ITER; it++) {
+        prep_kernel<<<n_traj, BS, 0, h->stream>>>(d);
+        HIP_TRY(hipEventRecord(ds->prof_ev[2 * it], h->stream));
+        gemm_kernel<<<dim3(n_r, nblk, rgroups), BS, 0, h->stream>>>(d);
+        HIP_TRY(hipEventRecord(ds->prof_ev[2 * it + 1], h->stream));
+        finish_kernel<<<n_traj, BS, 0, h->stream>>>(d);
+      }
+      HIP_TRY(hipGetLastError());
+      HIP_TRY(hipMemcpyAsync(&active, d.active, 4, hipMemcpyDeviceToHost, h->stream));
+      HIP_TRY(hipStreamSynchronize(h->stream));
       for (int it = 0; it < ITER; it++) {
         float ems = 0.f;
         HIP_TRY(hipEventElapsedTime(&ems, ds->prof_ev[2 * it], ds->prof_ev[2 * it + 1]));
@@ -615,8 +686,6 @@ int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *t
   float ms = 0.f;
   HIP_TRY(hipEventElapsedTime(&ms, h->ev[5], h->ev[6]));
   ds->last_ms = ms;
-  (void)hipGraphExecDestroy(exec);
-  (void)hipGraphDestroy(graph);
   if (active > 0) return mqct_fail(MQCT_ERR_HIP, "batched: trajectories did not finish");
   if (prof) {
     unsigned long long nbk = 0;

@@ -350,6 +350,8 @@ extern "C" int mqct_hip_eval_pes(mqct_hip_t *h, int n_r, const double *R_com,
              : h->coll_type == 4 ? MQCT_PES_REALSH : MQCT_PES_SITES;
   if (pes_kind != want)
     return mqct_fail(MQCT_ERR_ARG, "eval_pes: potential does not match SYS_TYPE");
+  if (mqct_pes_par_len(pes_kind, n_par, par) < 0)
+    return mqct_fail(MQCT_ERR_ARG, "eval_pes: parameter array too short for its header");
   HIP_TRY(hipSetDevice(h->device));
   int rc = set_r(h, n_r, R_com);
   if (rc) return rc;

@@ -187,3 +187,7 @@ int mqct_run_cutoff_fp64(mqct_hip *h, int ir1, int ir2, double conv);
 void mqct_weight_rows(mqct_hip *h, const double *V, const double *omega,
                       int ir1, int ir2, double *out);
 int mqct_eval_pes_dev(mqct_hip *h, int pes_kind, int n_par);
+// Number of doubles the synthetic potential pes_kind reads from par (its
+// header values decide the rest), or -1 if the header itself is invalid or
+// n_par is too short for it.
+long long mqct_pes_par_len(int pes_kind, int n_par, const double *par);

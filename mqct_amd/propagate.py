@@ -26,6 +26,20 @@ def traj_array(rows):
     return arr
 
 
+def l_grid(mass_au, U_cm, b_max, jmin, jmax, dl=1):
+    """L values of PROPAGATE (propagator.f:467-494) and TRAJ_ORB
+    (L = L_min + (id-1)*DL): k = sqrt(2 mu U), J_max = int(b_max k) - j_max,
+    L from the smallest |J - j| to J_max + j_max.  Returns (L values,
+    (J_min, J_max), k)."""
+    k = np.sqrt(mass_au * 2.0 * U_cm * CM_TO_HARTREE)
+    J_max = int(b_max * k) - jmax
+    J_min = 0
+    L_max = J_max + jmax
+    L_min = min(abs(J - jj) for J in range(J_min, J_max + 1) for jj in range(jmin, jmax + 1))
+    n = (L_max - L_min) // dl + 1
+    return np.array([L_min + i * dl for i in range(n)]), (J_min, J_max), k
+
+
 class Propagator:
     def __init__(self, dev, basis, R, mat_el=None, mat_el_der=None, mass_amu=1.0, theta_on=0,
                  from_build=False):
@@ -50,21 +64,13 @@ class Propagator:
 
     # -- trajectory sets -----------------------------------------------------
     def l_grid(self, U_cm, b_max, chann_ini, dl=1):
-        """L values of PROPAGATE (propagator.f:467-494) and TRAJ_ORB
-        (L = L_min + (id-1)*DL) for initial channel chann_ini (1-based)."""
-        E = U_cm * CM_TO_HARTREE
-        k = np.sqrt(self.mass * 2.0 * E)
+        """L values for initial channel chann_ini (1-based); see l_grid()."""
         j = int(self.b.j12[self.b.chann_indx[chann_ini - 1] - 1]) if self.b.coll_type != 0 else None
         jmin = jmax = j if j is not None else None
         if self.b.coll_type == 0:
             states = np.nonzero(self.b.indx_chann == chann_ini)[0]
             jmin, jmax = int(self.b.j12[states].min()), int(self.b.j12[states].max())
-        J_max = int(b_max * k) - jmax
-        J_min = 0
-        L_max = J_max + jmax
-        L_min = min(abs(J - jj) for J in range(J_min, J_max + 1) for jj in range(jmin, jmax + 1))
-        n = (L_max - L_min) // dl + 1
-        return np.array([L_min + i * dl for i in range(n)]), (J_min, J_max), k
+        return l_grid(self.mass, U_cm, b_max, jmin, jmax, dl)
 
     def trajectories(self, l_values, U_cm, s_st, time_step=10.0, eps=1e-8, hmin=1e-6,
                      R_start=None, numb_oscl_prds=1, numb_orb_prds=1, at_paths=None, max_steps=200000):

@@ -10,6 +10,12 @@ Fixtures written to tests/golden/fixtures.json:
   mf_he  : EXAMPLES_IO/SYS_TYPE_4/MF_He_CS_MQCT -- constants and channel
            list of MF_He_CS_MQCT.inp, channel energies printed in
            USER_INPUT_CHECK.out, state table and MIJ index list of MTRX.DAT.
+  co_he  : EXAMPLES_IO/SYS_TYPE_1/CO_He_dynamics -- the $BASIS/$SYSTEM
+           constants of CO_He_dynamics.inp, the opacity table P_n(L) of
+           OPACITY_FUNCTIONS_L.out (input of the cross-section assembly),
+           and its outputs: sigma(U), E_coll and sigma(E_coll) of
+           CROSS_SECTIONS.out and the J-resolved table of
+           PARTIAL_CROSS_SECTIONS_J.out.
 """
 import json
 import os
@@ -79,6 +85,36 @@ def mf_he():
             "pairs": pairs, "n_r": len(R)}
 
 
+def _table(lines, start):
+    rows = []
+    for ln in lines[start:]:
+        p = ln.split()
+        if not p or not re.match(r"^-?\d", p[0]):
+            break
+        rows.append([float(x) for x in p])
+    return rows
+
+
+def co_he():
+    d = "/root/reference/EXAMPLES_IO/SYS_TYPE_1/CO_He_dynamics"
+    inp = open(os.path.join(d, "CO_He_dynamics.inp")).read()
+    num = lambda key: float(re.search(r"\b%s = ([\d.]+)" % key, inp).group(1))
+    op = open(os.path.join(d, "OPACITY_FUNCTIONS_L.out")).read().splitlines()
+    i = next(k for k, ln in enumerate(op) if ln.startswith("B_IMPACT"))
+    opac = _table(op, i + 1)
+    cs = open(os.path.join(d, "CROSS_SECTIONS.out")).read().splitlines()
+    i = next(k for k, ln in enumerate(cs) if "sigma(U)" in ln)
+    sig = _table(cs, i + 1)
+    pj = open(os.path.join(d, "PARTIAL_CROSS_SECTIONS_J.out")).read().splitlines()
+    i = next(k for k, ln in enumerate(pj) if ln.startswith("J_TOTAL"))
+    part = _table(pj, i + 1)
+    return {"BE": num("BE"), "JMAX": int(num("JMAX")), "MASS_RED": num("MASS_RED"), "U": num("U_ENERGY"),
+            "B_IMPCT": num("B_IMPCT"), "RMAX": num("RMAX"),
+            "opacity": opac,            # [b, L, P_1..P_21]
+            "sigma": sig,               # [ilv, flv, sigma(U) A^2, E_coll cm^-1, sigma(E_coll) A^2]
+            "partial_J": part}          # [J, sigma_J per channel]
+
+
 if __name__ == "__main__":
-    json.dump({"h2o_he": h2o_he(), "mf_he": mf_he()}, open(OUT, "w"), indent=1)
+    json.dump({"h2o_he": h2o_he(), "mf_he": mf_he(), "co_he": co_he()}, open(OUT, "w"), indent=1)
     print("wrote", OUT)
