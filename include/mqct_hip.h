@@ -180,8 +180,10 @@ int mqct_hip_load_paths(mqct_hip_t *h, int n_paths, const int64_t *path_off,
 /* Propagate n_traj trajectories.  q_out: n_traj x (2N+8) final sys_var
  * (AT modes fill the 2N amplitudes and leave the classical slots 0);
  * stat_out (optional): per-trajectory statistics.
- * batched=1 selects the interval-bucketed batched right-hand side (AT
- * modes), batched=0 the one-trajectory-per-workgroup plain FP64 form. */
+ * batched=0: the one-trajectory-per-workgroup plain FP64 form;
+ * batched=1: the interval-bucketed batched form (AT modes) with FP64
+ * spline coefficients; batched=2: the same with the coefficients stored
+ * in FP32 (amplitudes, products and time stepping stay FP64). */
 int mqct_hip_propagate(mqct_hip_t *h, int mode, int n_traj,
                        const mqct_traj_t *traj, int batched, double *q_out,
                        mqct_traj_stat_t *stat_out);
@@ -192,11 +194,15 @@ int mqct_hip_propagate_timing(mqct_hip_t *h, double *ms);
  * events on their stream around every launch of the interval GEMM (inside
  * the captured graph) and accumulate; on=0 stops.  Each call resets the
  * accumulators.  kernel_stats returns the GEMM launches, their summed
- * duration (ms) and the algorithmic FLOPs they did: 16*S2 per trajectory
- * right-hand side with R inside the grid, S2 = sum over m-blocks of nb^2. */
+ * duration (ms), the algorithmic FLOPs they did (16*S2 per trajectory
+ * right-hand side with R inside the grid, S2 = sum over m-blocks of nb^2)
+ * and their algorithmic bytes (the knot matrices of every non-empty bucket,
+ * 2*S2 values each, plus 64*N bytes of amplitudes in and products out per
+ * right-hand side). */
 int mqct_hip_batched_profile(mqct_hip_t *h, int on);
 int mqct_hip_batched_kernel_stats(mqct_hip_t *h, int64_t *launches,
-                                  double *gemm_ms, double *gemm_flops);
+                                  double *gemm_ms, double *gemm_flops,
+                                  double *gemm_bytes);
 
 /* CC_RK4 propagation that also writes the SAVE_TRAJECT records of the first
  * AT run (propagator.f:3457-3462, 4330-4333): per trajectory up to rec_cap

@@ -310,19 +310,23 @@ __global__ void pes_points_kernel(int pes_kind, const double *__restrict__ par, 
 __global__ void weight_kernel(int64_t G, int n_r, const double *__restrict__ V,
                               const double *__restrict__ omega,
                               double *__restrict__ W64, float *__restrict__ W32,
-                              __hip_bfloat16 *__restrict__ Whi,
-                              __hip_bfloat16 *__restrict__ Wlo) {
+                              __hip_bfloat16 *__restrict__ W0,
+                              __hip_bfloat16 *__restrict__ W1,
+                              __hip_bfloat16 *__restrict__ W2) {
   int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (int64_t)n_r * G) return;
   double w = omega[idx % G] * V[idx];
   if (W64) W64[idx] = w;
-  float wf = (float)w;
-  if (W32) W32[idx] = wf;
-  if (Whi) {
-    __hip_bfloat16 hi = __float2bfloat16(wf);
-    float rem = wf - __bfloat162float(hi);
-    Whi[idx] = hi;
-    Wlo[idx] = __float2bfloat16(rem);
+  if (W32) W32[idx] = (float)w;
+  if (W0) {
+    // three BF16 terms, w ~ w0 + w1 + w2 to about 24 significant bits
+    const __hip_bfloat16 b0 = __float2bfloat16((float)w);
+    const double r1 = w - (double)__bfloat162float(b0);
+    const __hip_bfloat16 b1 = __float2bfloat16((float)r1);
+    const double r2 = r1 - (double)__bfloat162float(b1);
+    W0[idx] = b0;
+    W1[idx] = b1;
+    W2[idx] = __float2bfloat16((float)r2);
   }
 }
 
@@ -496,7 +500,7 @@ int mqct_build_weights(mqct_hip *h, int precision) {
   int64_t n = (int64_t)h->n_r * h->G;
   double *W64 = nullptr;
   float *W32 = nullptr;
-  __hip_bfloat16 *Whi = nullptr, *Wlo = nullptr;
+  __hip_bfloat16 *Whi = nullptr, *Wlo = nullptr, *Wlo2 = nullptr;
   if (precision == MQCT_PREC_FP64) {
     if (h->d_W64.ensure(n * 8)) return mqct_fail(MQCT_ERR_NOMEM, "W fp64");
     W64 = h->d_W64.as<double>();
@@ -504,14 +508,15 @@ int mqct_build_weights(mqct_hip *h, int precision) {
     if (h->d_W32.ensure(n * 4)) return mqct_fail(MQCT_ERR_NOMEM, "W fp32");
     W32 = h->d_W32.as<float>();
   } else {
-    if (h->d_Whi.ensure(n * 2) || h->d_Wlo.ensure(n * 2))
+    if (h->d_Whi.ensure(n * 2) || h->d_Wlo.ensure(n * 2) || h->d_Wlo2.ensure(n * 2))
       return mqct_fail(MQCT_ERR_NOMEM, "W bf16");
     Whi = h->d_Whi.as<__hip_bfloat16>();
     Wlo = h->d_Wlo.as<__hip_bfloat16>();
+    Wlo2 = h->d_Wlo2.as<__hip_bfloat16>();
   }
   weight_kernel<<<nblk(n, 256), 256, 0, h->stream>>>(
       h->G, h->n_r, h->d_V.as<double>(), h->d_omega.as<double>(), W64, W32,
-      Whi, Wlo);
+      Whi, Wlo, Wlo2);
   HIP_TRY(hipGetLastError());
   if (precision != MQCT_PREC_FP64) {
     int64_t np = (int64_t)h->n_states * h->G;

@@ -2,26 +2,30 @@
 // propagation (DERIVS_BK_adia, mqct_eqtns.f:720-879, integrated by odeint /
 // rkqs / rkck or rk4 as in propagator.f:3873-3908).
 //
-// All active trajectories advance together, one right-hand-side evaluation
-// each per iteration, every trajectory at its own stage of its own step:
-//   prep   : stage vector of each trajectory, its time, the path values
-//            (R, Theta, P_Phi) and the spline knots klo/khi of R; the
-//            rotated amplitudes v = exp(-iEt) y_stage (block order); the
-//            trajectory is appended to the buckets of its two knots.
-//   gemm   : for every knot r and m-block b, one dense product over the
-//            whole bucket: W = [Y_r | Y2_r] [w_Y v ; w_Y2 v] on FP64 MFMA,
-//            where Y_r (Mat_el at knot r) and Y2_r (its spline second
-//            derivatives) are the block's dense matrices and w_Y, w_Y2 the
-//            trajectory's spline weights for that knot (splint_bk,
-//            mqct_eqtns.f:575-607).  Each knot's matrices are read once per
-//            iteration for all trajectories that need them.
-//   finish : da/dt = -i exp(iEt) (W_lo + W_hi) + state-diagonal Coriolis
-//            (mqct_eqtns.f:845-877) and the integrator's state machine
-//            (Cash-Karp stages, error control and step-size rules of rkqs,
-//            step loop and end condition of odeint; or classical rk4).
-// The three launches of an iteration are captured once in a HIP graph and
-// replayed; the host reads the number of running trajectories between
-// replays.
+// All running trajectories advance together, one right-hand-side
+// evaluation each per iteration, every trajectory at its own stage of its
+// own step.  An iteration is two launches:
+//   gemm : for every knot r and m-block b, one dense product over the
+//          knot's bucket: W = [Y_r | Y2_r] [w_Y v ; w_Y2 v] on FP64 MFMA,
+//          where Y_r (Mat_el at knot r) and Y2_r (its spline second
+//          derivatives) are the block's dense matrices and w_Y, w_Y2 the
+//          trajectory's spline weights for that knot (splint_bk,
+//          mqct_eqtns.f:575-607).  Each knot's matrices are read once per
+//          iteration for all trajectories that need them.
+//   step : per trajectory, completes the stage, k = -i exp(iEt) (W_lo +
+//          W_hi) + state-diagonal Coriolis (mqct_eqtns.f:845-877), runs the
+//          integrator's state machine (Cash-Karp stages, error control and
+//          step-size rules of rkqs, step loop and end condition of odeint;
+//          or classical rk4), then prepares the next stage: its time, the
+//          path values (R, Theta, P_Phi) at that time, the knots and spline
+//          weights of R, the stage vector, its rotated amplitudes v =
+//          exp(-iEt) y_stage (block order) and its Coriolis part, and
+//          appends the trajectory to the buckets of its two knots.
+// Buckets are double-buffered by iteration parity.  The iterations are
+// captured once in a HIP graph and replayed; the host reads the number of
+// running trajectories between replays.  The knot matrices are kept in
+// FP64 or, as the FP32 coefficient form, in FP32 (converted to FP64 in the
+// MFMA operand).
 #include "dyn_internal.h"
 
 #include <algorithm>
@@ -33,24 +37,22 @@ namespace {
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 constexpr int BS = 256;
-constexpr int NSLOT = 8;  // y, k1..k6, ys
+constexpr int NSLOT = 8;       // y, k1..k6, yscal
 constexpr int MAXSTP = 10000;  // odeint (suppl_routines.f:1362)
 
 struct Ctl {
   double x, x1, x2, h, hnext, hmin, eps;
-  double tau, aa, bb, c1, c2, dphidt;  // current stage evaluation
+  double tau, aa, bb, c1, c2;  // stage being evaluated: time, spline weights
   double norm_err;
-  int stage, nstp, status, zero;       // status: 0 run, 1 done
-  int klo, path, nrhs;
-  int rk4;                             // 1: the stages are rk4's (AT rk4, or odeint's last step)
-  int last;                            // rk4 stepping: the current step ends the path
-  int pad;
+  int stage, nstp, status, zero;  // status: 0 running, 1 done
+  int klo, path, nrhs, rk4;       // rk4: the stages are rk4's (AT rk4, or odeint's last step)
+  int last, rec;                  // last: rk4 step that ends the path; rec: path-record hint
 };
 
 struct BDev {
   int N, nblk, n_r, n_traj, mode;
-  const double *R;
-  const double *KM;           // [n_r][2][S2] dense block matrices Y, Y2
+  const double *R;            // knots; R[-1] holds the reduced mass
+  const void *KM;             // [n_r][2][S2] dense block matrices Y, Y2 (double or float)
   long long S2;
   const int *blk_start;       // [nblk+1] positions
   const long long *blk_off;   // [nblk] offset of the block in S2
@@ -63,28 +65,12 @@ struct BDev {
   double2 *V;                 // [n_traj][N]
   double2 *W;                 // [n_traj][2][N]
   Ctl *ctl;
-  int *cnt;                   // [n_r]
-  int2 *bucket;               // [n_r][n_traj] (t, side)
+  int *cnt;                   // [2][n_r]
+  int2 *bucket;               // [2][n_r][n_traj] (t, side)
   int *active;                // running trajectories
   unsigned long long *nbucketed;  // right-hand sides with R inside the grid
-  double *scal;               // [n_traj][2N] yscal
+  unsigned long long *ntouched;   // (GEMM launch, knot) pairs with a non-empty bucket
 };
-
-__device__ double path_at(const BDev &d, int path, int col, double t) {
-  long long o0 = d.poff[path], o1 = d.poff[path + 1];
-  int n = (int)(o1 - o0);
-  const double *rec = d.prec + o0 * 8;
-  const double *der = d.pder + o0 * 7;
-  int klo = 0, khi = n - 1;
-  while (khi - klo > 1) {
-    int k = (khi + klo) / 2;
-    if (rec[k * 8] > t) khi = k; else klo = k;
-  }
-  double h = rec[khi * 8] - rec[klo * 8];
-  double a = (rec[khi * 8] - t) / h, b = (t - rec[klo * 8]) / h;
-  return a * rec[klo * 8 + 1 + col] + b * rec[khi * 8 + 1 + col] +
-         ((a * a * a - a) * der[klo * 7 + col] + (b * b * b - b) * der[khi * 7 + col]) * (h * h) / 6.0;
-}
 
 // Cash-Karp tableau (suppl_routines.f:1306-1321); A4 = REAL*4 .6
 __constant__ double CK_A[6] = {0.0, .2, .3, (double)0.6f, 1.0, .875};
@@ -95,193 +81,6 @@ __constant__ double CK_B[6][5] = {
     {.3, -.9, 1.2, 0, 0},
     {-11.0 / 54.0, 2.5, -70.0 / 27.0, 35.0 / 27.0, 0},
     {1631.0 / 55296.0, 175.0 / 512.0, 575.0 / 13824.0, 44275.0 / 110592.0, 253.0 / 4096.0}};
-
-__global__ __launch_bounds__(BS) void prep_kernel(BDev d) {
-  const int t = blockIdx.x, tid = threadIdx.x, N = d.N, n2 = 2 * N;
-  Ctl c = d.ctl[t];
-  if (c.status != 0) return;
-  double *S = d.S + (long long)t * NSLOT * n2;
-  const double *y = S;
-  double *ys = S + 7 * n2;
-  const int s = c.stage;
-  double tau;
-  if (!c.rk4) {
-    tau = c.x + CK_A[s] * c.h;
-    for (int i = tid; i < n2; i += BS) {
-      double v = y[i];
-      if (s > 0) {
-        double acc = 0.0;
-        for (int q = 0; q < s; q++) acc += CK_B[s][q] * S[(1 + q) * n2 + i];
-        // rkck forms y + h*(sum b k) (and y + B21*h*k1 for the first stage)
-        v = (s == 1) ? y[i] + CK_B[1][0] * c.h * S[n2 + i] : y[i] + c.h * acc;
-      }
-      ys[i] = v;
-    }
-  } else {  // rk4: x, x+h/2, x+h/2, x+h
-    double hh = c.h * 0.5;
-    tau = (s == 0) ? c.x : (s == 3 ? c.x + c.h : c.x + hh);
-    for (int i = tid; i < n2; i += BS) {
-      double v = y[i];
-      if (s == 1) v = y[i] + hh * S[n2 + i];
-      else if (s == 2) v = y[i] + hh * S[2 * n2 + i];
-      else if (s == 3) v = y[i] + c.h * S[3 * n2 + i];
-      ys[i] = v;
-    }
-  }
-  __syncthreads();
-  __shared__ double sh[8];
-  if (tid == 0) {
-    double R = path_at(d, c.path, 0, tau);
-    double tet = path_at(d, c.path, 2, tau);
-    double pphi = path_at(d, c.path, 5, tau);
-    double st = sin(tet);
-    double mu = d.R[-1];  // stored just before the R grid
-    sh[0] = R;
-    sh[1] = pphi / R / R / mu / (st * st);
-  }
-  __syncthreads();
-  const double R = sh[0];
-  if (tid == 0) {
-    const int n = d.n_r;
-    c.tau = tau;
-    c.dphidt = sh[1];
-    c.zero = R > d.R[n - 1];
-    int klo = 0, khi = n - 1;
-    while (khi - klo > 1) {
-      int k = (khi + klo) / 2;
-      if (d.R[k] > R) khi = k; else klo = k;
-    }
-    double h = d.R[khi] - d.R[klo];
-    double aa = (d.R[khi] - R) / h, bb = (R - d.R[klo]) / h;
-    double cc5 = (h * h) / 6.0;
-    c.klo = klo;
-    c.aa = aa;
-    c.bb = bb;
-    c.c1 = (aa * aa * aa - aa) * cc5;
-    c.c2 = (bb * bb * bb - bb) * cc5;
-    if (!c.zero) {
-      atomicAdd(d.nbucketed, 1ULL);
-      int p0 = atomicAdd(&d.cnt[klo], 1);
-      d.bucket[(long long)klo * d.n_traj + p0] = make_int2(t, 0);
-      int p1 = atomicAdd(&d.cnt[khi], 1);
-      d.bucket[(long long)khi * d.n_traj + p1] = make_int2(t, 1);
-    }
-    d.ctl[t] = c;
-  }
-  // rotated amplitudes
-  double2 *v = d.V + (long long)t * N;
-  for (int i = tid; i < N; i += BS) {
-    double cs = cos(tau * d.E[i]), sn = sin(tau * d.E[i]);
-    double xr = ys[i], xi = ys[i + N];
-    v[i] = make_double2(xr * cs + xi * sn, xi * cs - xr * sn);
-  }
-}
-
-// Interval GEMM.  One workgroup per (knot r, m-block b, 64-row group):
-//   W[t][side][rows] = [Y_r | Y2_r] [w_Y v_t ; w_Y2 v_t]
-// for every bucket entry (t, side) of knot r, 64 entries (128 real columns:
-// Re and Im of each v) per column super-tile.  Each wave owns a 16-row tile
-// and keeps 8 column tiles of f64 16x16x4 MFMA accumulators; the weighted
-// vectors are staged through LDS in slices of GK rows of K = 2 nb, the
-// matrix fragments come straight from HBM/L2 (the blocks are symmetric, so
-// lanes of one k read 16 consecutive rows).
-constexpr int GE = 64;            // entries per column super-tile
-constexpr int GC = 2 * GE / 16;   // MFMA column tiles
-constexpr int GK = 16;            // K rows per LDS slice
-constexpr int GLD = 2 * GE + 16;  // LDS row stride in doubles (half-waves hit disjoint banks)
-
-__global__ __launch_bounds__(BS) void gemm_kernel(BDev d) {
-  const int r = blockIdx.x, b = blockIdx.y;
-  const int ne = d.cnt[r];
-  if (ne == 0) return;
-  const int p0 = d.blk_start[b], nb = d.blk_start[b + 1] - p0;
-  const int row0 = blockIdx.z * 64;
-  if (row0 >= nb) return;
-  const int K = 2 * nb;
-  const double *Y = d.KM + ((long long)r * 2) * d.S2 + d.blk_off[b];
-  const double *Y2 = d.KM + ((long long)r * 2 + 1) * d.S2 + d.blk_off[b];
-  __shared__ double Bs[GK][GLD];
-  __shared__ int s_t[GE], s_side[GE];
-  __shared__ double s_wy[GE], s_wy2[GE];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int rt0 = row0 + wave * 16;
-  const bool wave_rows = rt0 < nb;
-  const int rr = rt0 + (lane & 15), kq = lane >> 4;
-  const int2 *bk = d.bucket + (long long)r * d.n_traj;
-  for (int e0 = 0; e0 < ne; e0 += GE) {
-    const int nE = min(GE, ne - e0);
-    const int nct = (2 * nE + 15) / 16;
-    __syncthreads();
-    if (tid < GE) {
-      int t = -1, side = 0;
-      double wy = 0.0, wy2 = 0.0;
-      if (tid < nE) {
-        int2 ent = bk[e0 + tid];
-        const Ctl &c = d.ctl[ent.x];
-        t = ent.x;
-        side = ent.y;
-        wy = side ? c.bb : c.aa;
-        wy2 = side ? c.c2 : c.c1;
-      }
-      s_t[tid] = t;
-      s_side[tid] = side;
-      s_wy[tid] = wy;
-      s_wy2[tid] = wy2;
-    }
-    dbl4 acc[GC];
-#pragma unroll
-    for (int ct = 0; ct < GC; ct++) acc[ct] = dbl4{0.0, 0.0, 0.0, 0.0};
-    for (int k0 = 0; k0 < K; k0 += GK) {
-      __syncthreads();
-      for (int idx = tid; idx < GK * GE; idx += BS) {
-        const int kk = idx % GK, e = idx / GK, k = k0 + kk;
-        double vx = 0.0, vy = 0.0;
-        if (e < nE && k < K) {
-          const bool lo = k < nb;
-          const double w = lo ? s_wy[e] : s_wy2[e];
-          const double2 x = d.V[(long long)s_t[e] * d.N + p0 + (lo ? k : k - nb)];
-          vx = w * x.x;
-          vy = w * x.y;
-        }
-        Bs[kk][2 * e] = vx;
-        Bs[kk][2 * e + 1] = vy;
-      }
-      double af[GK / 4];
-#pragma unroll
-      for (int s4 = 0; s4 < GK / 4; s4++) {
-        const int k = k0 + 4 * s4 + kq;
-        double a = 0.0;
-        if (rr < nb && k < K) a = (k < nb) ? Y[(long long)k * nb + rr] : Y2[(long long)(k - nb) * nb + rr];
-        af[s4] = a;
-      }
-      __syncthreads();
-      if (wave_rows) {
-#pragma unroll
-        for (int s4 = 0; s4 < GK / 4; s4++) {
-          const double *brow = &Bs[4 * s4 + kq][lane & 15];
-#pragma unroll
-          for (int ct = 0; ct < GC; ct++)
-            if (ct < nct) acc[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[s4], brow[16 * ct], acc[ct], 0, 0, 0);
-        }
-      }
-    }
-    // f64 16x16x4 output: row lane/16 + 4i, column lane%16 of each tile
-    if (wave_rows) {
-#pragma unroll
-      for (int ct = 0; ct < GC; ct++) {
-        if (ct >= nct) break;
-        const int col = 16 * ct + (lane & 15), e = col >> 1;
-        if (e >= nE) continue;
-        double *wb = reinterpret_cast<double *>(d.W + ((long long)s_t[e] * 2 + s_side[e]) * d.N + p0) + (col & 1);
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-          const int orow = rt0 + (lane >> 4) + 4 * i;
-          if (orow < nb) wb[2 * orow] = acc[ct][i];
-        }
-      }
-    }
-  }
-}
 
 __device__ inline double block_sum(double x, double *red) {
   for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
@@ -305,38 +104,306 @@ __device__ inline double block_max(double x, double *red) {
   return s;
 }
 
-__global__ __launch_bounds__(BS) void finish_kernel(BDev d) {
+// Interval of a sorted table containing x, as splint's bisection finds it
+// (suppl_routines.f:968-1002): the largest lo <= n-2 with xa[lo] <= x (0 if
+// x < xa[0]).  Walks from a hint, which gives the same interval.
+__device__ inline int interval(const double *xa, int stride, int n, double x, int hint) {
+  int lo = hint < 0 ? 0 : (hint > n - 2 ? n - 2 : hint);
+  while (lo > 0 && xa[(long long)lo * stride] > x) lo--;
+  while (lo < n - 2 && xa[(long long)(lo + 1) * stride] <= x) lo++;
+  return lo;
+}
+
+// Prepare stage c.stage (whole block; c uniform on entry): the stage vector
+// into sys (LDS, 2N), v = exp(-i E tau) y_stage into V, the Coriolis part of
+// k into its slot; thread 0 evaluates the path at tau, finds the knots and
+// spline weights, records them in its c and appends the trajectory to the
+// buckets of parity `par`.
+__device__ void prepare_stage(const BDev &d, int t, Ctl &c, double *sys, double *shm, int par) {
+  const int tid = threadIdx.x, N = d.N, n2 = 2 * N;
+  double *S = d.S + (long long)t * NSLOT * n2;
+  const double *y = S;
+  const int s = c.stage;
+  double tau;
+  if (!c.rk4) {
+    tau = c.x + CK_A[s] * c.h;
+    for (int i = tid; i < n2; i += BS) {
+      double v = y[i];
+      if (s == 1) {
+        v = y[i] + CK_B[1][0] * c.h * S[n2 + i];
+      } else if (s > 1) {
+        double acc = 0.0;
+        for (int q = 0; q < s; q++) acc += CK_B[s][q] * S[(1 + q) * n2 + i];
+        v = y[i] + c.h * acc;
+      }
+      sys[i] = v;
+    }
+  } else {  // rk4: x, x+h/2, x+h/2, x+h
+    const double hh = c.h * 0.5;
+    tau = (s == 0) ? c.x : (s == 3 ? c.x + c.h : c.x + hh);
+    for (int i = tid; i < n2; i += BS) {
+      double v = y[i];
+      if (s == 1) v = y[i] + hh * S[n2 + i];
+      else if (s == 2) v = y[i] + hh * S[2 * n2 + i];
+      else if (s == 3) v = y[i] + c.h * S[3 * n2 + i];
+      sys[i] = v;
+    }
+  }
+  if (tid == 0) {
+    // recorded path at tau: R, Theta, P_Phi (DERIVS_BK_adia, mqct_eqtns.f:748-759)
+    const long long o0 = d.poff[c.path];
+    const int n = (int)(d.poff[c.path + 1] - o0);
+    const double *rec = d.prec + o0 * 8;
+    const double *der = d.pder + o0 * 7;
+    const int lo = interval(rec, 8, n, tau, c.rec);
+    c.rec = lo;
+    const int hi = lo + 1;
+    const double hr = rec[hi * 8] - rec[lo * 8];
+    const double a = (rec[hi * 8] - tau) / hr, b = (tau - rec[lo * 8]) / hr;
+    const double ca = a * a * a - a, cb = b * b * b - b;
+    double pv[3];
+    const int col[3] = {0, 2, 5};
+    for (int q = 0; q < 3; q++)
+      pv[q] = a * rec[lo * 8 + 1 + col[q]] + b * rec[hi * 8 + 1 + col[q]] +
+              (ca * der[lo * 7 + col[q]] + cb * der[hi * 7 + col[q]]) * (hr * hr) / 6.0;
+    const double R = pv[0], st = sin(pv[1]);
+    shm[0] = pv[2] / R / R / d.R[-1] / (st * st);  // dPhi/dt
+    // knots of R (splint_bk, mqct_eqtns.f:575-607); R > R_n: no coupling
+    const int nr = d.n_r;
+    c.tau = tau;
+    c.zero = R > d.R[nr - 1];
+    const int klo = interval(d.R, 1, nr, R, c.klo), khi = klo + 1;
+    const double hk = d.R[khi] - d.R[klo];
+    const double aa = (d.R[khi] - R) / hk, bb = (R - d.R[klo]) / hk;
+    const double cc5 = (hk * hk) / 6.0;
+    c.klo = klo;
+    c.aa = aa;
+    c.bb = bb;
+    c.c1 = (aa * aa * aa - aa) * cc5;
+    c.c2 = (bb * bb * bb - bb) * cc5;
+    if (!c.zero) {
+      atomicAdd(d.nbucketed, 1ULL);
+      int *cnt = d.cnt + par * nr;
+      int2 *bk = d.bucket + (long long)par * nr * d.n_traj;
+      const int p0 = atomicAdd(&cnt[klo], 1);
+      bk[(long long)klo * d.n_traj + p0] = make_int2(t, 0);
+      const int p1 = atomicAdd(&cnt[khi], 1);
+      bk[(long long)khi * d.n_traj + p1] = make_int2(t, 1);
+    }
+  }
+  __syncthreads();
+  const double dphidt = shm[0];
+  double2 *v = d.V + (long long)t * N;
+  double *kp = S + (1 + s) * n2;
+  for (int i = tid; i < N; i += BS) {
+    const double cs = cos(tau * d.E[i]), sn = sin(tau * d.E[i]);
+    const double xr = sys[i], xi = sys[i + N];
+    v[i] = make_double2(xr * cs + xi * sn, xi * cs - xr * sn);
+    double re = 0.0, im = 0.0;
+    const int lo = d.nb_lo[i], hi = d.nb_hi[i];
+    if (lo >= 0) {
+      re += dphidt / 2.0 * d.sq1[i] * sys[lo + N];
+      im -= dphidt / 2.0 * d.sq1[i] * sys[lo];
+    }
+    if (hi >= 0) {
+      re += dphidt / 2.0 * d.sq2[i] * sys[hi + N];
+      im -= dphidt / 2.0 * d.sq2[i] * sys[hi];
+    }
+    kp[i] = re;
+    kp[i + N] = im;
+  }
+}
+
+// Interval GEMM.  One workgroup per (knot r, m-block b, 128-row group):
+//   W[t][side][rows] = [Y_r | Y2_r] [w_Y v_t ; w_Y2 v_t]
+// for every bucket entry (t, side) of knot r, 64 entries (128 real columns:
+// Re and Im of each v) per column super-tile.  Each wave owns two 16-row
+// tiles and keeps 8 column tiles of f64 16x16x4 MFMA accumulators for each.
+// K = 2 nb runs in slices of GK rows: the weighted vectors of a slice are
+// staged through a double-buffered LDS tile, and the next slice's matrix
+// fragments and vector values are loaded into registers while the current
+// slice's MFMAs run.  The matrix fragments come straight from HBM/L2 (the
+// blocks are symmetric, so lanes of one k read 16 consecutive rows).
+// Block (0,0,0) also clears the other parity's bucket counts for the step
+// kernel that follows.
+constexpr int GE = 64;             // entries per column super-tile
+constexpr int GC = 2 * GE / 16;    // MFMA column tiles
+constexpr int GK = 16;             // K rows per slice
+constexpr int GLD = 2 * GE + 16;   // LDS row stride in doubles (half-waves hit disjoint banks)
+constexpr int GRT = 2;             // row tiles per wave
+constexpr int GROWS = 4 * GRT * 16;  // rows per workgroup
+constexpr int GSTG = GK * GE / BS;   // staged (k, entry) values per thread per slice
+
+template <typename KT>
+__global__ __launch_bounds__(BS) void gemm_kernel(BDev d, int par) {
+  const int r = blockIdx.x, b = blockIdx.y;
+  if (r == 0 && b == 0 && blockIdx.z == 0)
+    for (int i = threadIdx.x; i < d.n_r; i += BS) d.cnt[(par ^ 1) * d.n_r + i] = 0;
+  const int ne = d.cnt[par * d.n_r + r];
+  if (ne == 0) return;
+  const int p0 = d.blk_start[b], nb = d.blk_start[b + 1] - p0;
+  const int row0 = blockIdx.z * GROWS;
+  if (row0 >= nb) return;
+  if (b == 0 && blockIdx.z == 0 && threadIdx.x == 0) atomicAdd(d.ntouched, 1ULL);
+  const int K = 2 * nb;
+  const int nsl = (K + GK - 1) / GK;
+  const KT *Y = static_cast<const KT *>(d.KM) + ((long long)r * 2) * d.S2 + d.blk_off[b];
+  const KT *Y2 = static_cast<const KT *>(d.KM) + ((long long)r * 2 + 1) * d.S2 + d.blk_off[b];
+  __shared__ double Bs[2][GK][GLD];
+  __shared__ int s_t[GE], s_side[GE];
+  __shared__ double s_wy[GE], s_wy2[GE];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int kq = lane >> 4;
+  int rr[GRT];
+  bool tile_on[GRT];
+#pragma unroll
+  for (int q = 0; q < GRT; q++) {
+    const int rt0 = row0 + (wave * GRT + q) * 16;
+    tile_on[q] = rt0 < nb;
+    rr[q] = rt0 + (lane & 15);
+  }
+  const int2 *bk = d.bucket + ((long long)par * d.n_r + r) * d.n_traj;
+  // one slice's matrix fragments (lane: rows rr[q], k = k0 + 4 s4 + kq)
+  auto load_a = [&](int k0, double (&af)[GRT][GK / 4]) {
+#pragma unroll
+    for (int q = 0; q < GRT; q++)
+#pragma unroll
+      for (int s4 = 0; s4 < GK / 4; s4++) {
+        const int k = k0 + 4 * s4 + kq;
+        double a = 0.0;
+        if (rr[q] < nb && k < K)
+          a = (k < nb) ? (double)Y[(long long)k * nb + rr[q]] : (double)Y2[(long long)(k - nb) * nb + rr[q]];
+        af[q][s4] = a;
+      }
+  };
+  for (int e0 = 0; e0 < ne; e0 += GE) {
+    const int nE = min(GE, ne - e0);
+    const int nct = (2 * nE + 15) / 16;
+    __syncthreads();
+    if (tid < GE) {
+      int t = 0, side = 0;
+      double wy = 0.0, wy2 = 0.0;
+      if (tid < nE) {
+        const int2 ent = bk[e0 + tid];
+        const Ctl &c = d.ctl[ent.x];
+        t = ent.x;
+        side = ent.y;
+        wy = side ? c.bb : c.aa;
+        wy2 = side ? c.c2 : c.c1;
+      }
+      s_t[tid] = t;
+      s_side[tid] = side;
+      s_wy[tid] = wy;
+      s_wy2[tid] = wy2;
+    }
+    __syncthreads();
+    // one slice's weighted vector values: thread -> (k row kk, entry e),
+    // kk fastest so a group of threads reads 16 consecutive amplitudes
+    auto load_b = [&](int k0, double2 (&vb)[GSTG]) {
+#pragma unroll
+      for (int j = 0; j < GSTG; j++) {
+        const int idx = tid + j * BS, kk = idx % GK, e = idx / GK, k = k0 + kk;
+        double2 v = make_double2(0.0, 0.0);
+        if (e < nE && k < K) {
+          const bool lo = k < nb;
+          const double w = lo ? s_wy[e] : s_wy2[e];
+          const double2 x = d.V[(long long)s_t[e] * d.N + p0 + (lo ? k : k - nb)];
+          v = make_double2(w * x.x, w * x.y);
+        }
+        vb[j] = v;
+      }
+    };
+    auto store_b = [&](int buf, const double2 (&vb)[GSTG]) {
+#pragma unroll
+      for (int j = 0; j < GSTG; j++) {
+        const int idx = tid + j * BS, kk = idx % GK, e = idx / GK;
+        Bs[buf][kk][2 * e] = vb[j].x;
+        Bs[buf][kk][2 * e + 1] = vb[j].y;
+      }
+    };
+    dbl4 acc[GRT][GC];
+#pragma unroll
+    for (int q = 0; q < GRT; q++)
+#pragma unroll
+      for (int ct = 0; ct < GC; ct++) acc[q][ct] = dbl4{0.0, 0.0, 0.0, 0.0};
+    double af[GRT][GK / 4], an[GRT][GK / 4];
+    double2 vb[GSTG];
+    load_a(0, af);
+    load_b(0, vb);
+    store_b(0, vb);
+    __syncthreads();
+    for (int sl = 0; sl < nsl; sl++) {
+      const int cur = sl & 1;
+      const bool more = sl + 1 < nsl;
+      if (more) {
+        load_a((sl + 1) * GK, an);
+        load_b((sl + 1) * GK, vb);
+      }
+#pragma unroll
+      for (int s4 = 0; s4 < GK / 4; s4++) {
+        const double *brow = &Bs[cur][4 * s4 + kq][lane & 15];
+        double bv[GC];
+#pragma unroll
+        for (int ct = 0; ct < GC; ct++) bv[ct] = brow[16 * ct];
+#pragma unroll
+        for (int q = 0; q < GRT; q++) {
+          if (!tile_on[q]) continue;
+#pragma unroll
+          for (int ct = 0; ct < GC; ct++)
+            if (ct < nct) acc[q][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[q][s4], bv[ct], acc[q][ct], 0, 0, 0);
+        }
+      }
+      if (more) {
+        store_b(cur ^ 1, vb);
+#pragma unroll
+        for (int q = 0; q < GRT; q++)
+#pragma unroll
+          for (int s4 = 0; s4 < GK / 4; s4++) af[q][s4] = an[q][s4];
+      }
+      __syncthreads();
+    }
+    // f64 16x16x4 output: row lane/16 + 4i, column lane%16 of each tile
+#pragma unroll
+    for (int q = 0; q < GRT; q++) {
+      if (!tile_on[q]) continue;
+      const int rt0 = row0 + (wave * GRT + q) * 16;
+#pragma unroll
+      for (int ct = 0; ct < GC; ct++) {
+        if (ct >= nct) break;
+        const int col = 16 * ct + (lane & 15), e = col >> 1;
+        if (e >= nE) continue;
+        double *wb = reinterpret_cast<double *>(d.W + ((long long)s_t[e] * 2 + s_side[e]) * d.N + p0) + (col & 1);
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          const int orow = rt0 + (lane >> 4) + 4 * i;
+          if (orow < nb) wb[2 * orow] = acc[q][ct][i];
+        }
+      }
+    }
+  }
+}
+
+// Completes the stage the GEMM evaluated, advances the integrator and
+// prepares the next stage (into the buckets of parity `par`).
+__global__ __launch_bounds__(BS) void step_kernel(BDev d, int par) {
+  extern __shared__ double sys[];  // 2N: stage vector of the next stage
+  __shared__ double red[8], shm[4];
   const int t = blockIdx.x, tid = threadIdx.x, N = d.N, n2 = 2 * N;
-  __shared__ double red[8];
-  if (t == 0)
-    for (int i = tid; i < d.n_r; i += BS) d.cnt[i] = 0;  // the gemm has consumed the buckets
   Ctl c = d.ctl[t];
   if (c.status != 0) return;
   double *S = d.S + (long long)t * NSLOT * n2;
-  double *y = S, *ys = S + 7 * n2;
+  double *y = S;
   const int s = c.stage;
-  double *k = S + (1 + s) * n2;  // k1 .. k6 (rk4 uses k1..k4)
-  const double2 *W0 = d.W + ((long long)t * 2) * N, *W1 = W0 + N;
-  for (int i = tid; i < N; i += BS) {
-    double wx = 0.0, wy = 0.0;
-    if (!c.zero) {
-      wx = W0[i].x + W1[i].x;
-      wy = W0[i].y + W1[i].y;
+  double *k = S + (1 + s) * n2;  // k1 .. k6 (rk4 uses k1..k4); holds the Coriolis part
+  if (!c.zero) {
+    const double2 *W0 = d.W + ((long long)t * 2) * N, *W1 = W0 + N;
+    for (int i = tid; i < N; i += BS) {
+      const double wx = W0[i].x + W1[i].x, wy = W0[i].y + W1[i].y;
+      const double cs = cos(c.tau * d.E[i]), sn = sin(c.tau * d.E[i]);
+      // -i exp(i E tau) w
+      k[i] = k[i] + (sn * wx + cs * wy);
+      k[i + N] = k[i + N] - (cs * wx - sn * wy);
     }
-    double cs = cos(c.tau * d.E[i]), sn = sin(c.tau * d.E[i]);
-    double re = sn * wx + cs * wy;
-    double im = -(cs * wx - sn * wy);
-    int lo = d.nb_lo[i], hi = d.nb_hi[i];
-    if (lo >= 0) {
-      re += c.dphidt / 2.0 * d.sq1[i] * ys[lo + N];
-      im -= c.dphidt / 2.0 * d.sq1[i] * ys[lo];
-    }
-    if (hi >= 0) {
-      re += c.dphidt / 2.0 * d.sq2[i] * ys[hi + N];
-      im -= c.dphidt / 2.0 * d.sq2[i] * ys[hi];
-    }
-    k[i] = re;
-    k[i + N] = im;
   }
   __syncthreads();
   c.nrhs++;
@@ -345,7 +412,7 @@ __global__ __launch_bounds__(BS) void finish_kernel(BDev d) {
     if (s < 3) {
       c.stage = s + 1;
     } else {
-      double h6 = c.h / 6.0;
+      const double h6 = c.h / 6.0;
       const double *k1 = S + n2, *k2 = S + 2 * n2, *k3 = S + 3 * n2, *k4 = S + 4 * n2;
       for (int i = tid; i < n2; i += BS) y[i] = y[i] + h6 * (k1[i] + k4[i] + 2.0 * (k2[i] + k3[i]));
       c.nstp++;
@@ -355,7 +422,7 @@ __global__ __launch_bounds__(BS) void finish_kernel(BDev d) {
         c.x = c.x2 + copysign(TINY, c.x2 - c.x1);
         c.status = 1;
       } else {
-        // DOTRAJECT's AT rk4 loop (propagator.f:3889-3908): a step that
+        // DOTRAJECT's AT rk4 loop (propagator.f:3862-3880): a step that
         // would reach the path end is shortened to end there and is the last
         c.x += c.h;
         if (c.last) c.status = 1;
@@ -375,7 +442,7 @@ __global__ __launch_bounds__(BS) void finish_kernel(BDev d) {
       c.stage = 1;
     } else {
       // odeint: yscal from dydx at the start of the step, then clamp h
-      double *ysc = d.scal + (long long)t * n2;
+      double *ysc = S + 7 * n2;
       for (int i = tid; i < n2; i += BS) ysc[i] = fabs(y[i]) + fabs(c.h * k[i]) + TINY;
       if ((c.x + c.h - c.x2) * (c.x + c.h - c.x1) > 0.0) c.h = c.x2 - c.x;
       c.stage = 1;
@@ -387,18 +454,18 @@ __global__ __launch_bounds__(BS) void finish_kernel(BDev d) {
     const double DC1 = C1 - 2825.0 / 27648.0, DC3 = C3 - 18575.0 / 48384.0,
                  DC4 = C4 - 13525.0 / 55296.0, DC5 = -277.0 / 14336.0, DC6 = C6 - .25;
     const double *k1 = S + n2, *k3 = S + 3 * n2, *k4 = S + 4 * n2, *k5 = S + 5 * n2, *k6 = S + 6 * n2;
-    const double *ysc = d.scal + (long long)t * n2;
+    const double *ysc = S + 7 * n2;
     double em = 0.0;
     for (int i = tid; i < n2; i += BS) {
-      double yerr = c.h * (DC1 * k1[i] + DC3 * k3[i] + DC4 * k4[i] + DC5 * k5[i] + DC6 * k6[i]);
+      const double yerr = c.h * (DC1 * k1[i] + DC3 * k3[i] + DC4 * k4[i] + DC5 * k5[i] + DC6 * k6[i]);
       em = fmax(em, fabs(yerr / ysc[i]));
     }
     em = block_max(em, red) / c.eps;
     // rkqs (suppl_routines.f:1229-1283): a step at or below hmin is taken as is
-    bool accept = (c.h <= c.hmin) || !(em > 1.0);
+    const bool accept = (c.h <= c.hmin) || !(em > 1.0);
     if (!accept) {
-      double htemp = 0.9 * c.h * pow(em, -.25);
-      double a = fabs(htemp), b = 0.1 * fabs(c.h);
+      const double htemp = 0.9 * c.h * pow(em, -.25);
+      const double a = fabs(htemp), b = 0.1 * fabs(c.h);
       c.h = copysign(a > b ? a : b, c.h);
       c.stage = 1;  // retry with the same k1
     } else {
@@ -417,75 +484,84 @@ __global__ __launch_bounds__(BS) void finish_kernel(BDev d) {
     }
   }
   __syncthreads();
-  if (tid == 0) {
-    if (c.status != 0) atomicSub(d.active, 1);
-    d.ctl[t] = c;
+  if (c.status != 0) {
+    if (tid == 0) {
+      atomicSub(d.active, 1);
+      d.ctl[t] = c;
+    }
+    return;
   }
+  prepare_stage(d, t, c, sys, shm, par);
+  if (tid == 0) d.ctl[t] = c;
 }
 
 // dense block matrices of every knot from the R-major pair arrays
+template <typename KT>
 __global__ void knot_matrices_kernel(long long T, int n_r, const double *__restrict__ Mt,
                                      const double *__restrict__ Y2t, const int *__restrict__ pi,
                                      const int *__restrict__ pj, const int *__restrict__ state_pos,
                                      const int *__restrict__ pos_blk, const int *__restrict__ blk_start,
                                      const long long *__restrict__ blk_off, long long S2,
-                                     double *__restrict__ KM) {
-  long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+                                     KT *__restrict__ KM) {
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= T * n_r) return;
-  int r = (int)(idx / T);
-  long long k = idx % T;
-  int a = state_pos[pi[k]], b = state_pos[pj[k]];
-  int blk = pos_blk[a];
-  int nb = blk_start[blk + 1] - blk_start[blk];
-  int la = a - blk_start[blk], lb = b - blk_start[blk];
-  double m = Mt[(long long)r * T + k], m2 = Y2t[(long long)r * T + k];
-  double *Yr = KM + ((long long)r * 2) * S2 + blk_off[blk];
-  double *Y2r = KM + ((long long)r * 2 + 1) * S2 + blk_off[blk];
+  const int r = (int)(idx / T);
+  const long long k = idx % T;
+  const int a = state_pos[pi[k]], b = state_pos[pj[k]];
+  const int blk = pos_blk[a];
+  const int nb = blk_start[blk + 1] - blk_start[blk];
+  const int la = a - blk_start[blk], lb = b - blk_start[blk];
+  const KT m = (KT)Mt[(long long)r * T + k], m2 = (KT)Y2t[(long long)r * T + k];
+  KT *Yr = KM + ((long long)r * 2) * S2 + blk_off[blk];
+  KT *Y2r = KM + ((long long)r * 2 + 1) * S2 + blk_off[blk];
   Yr[(long long)la * nb + lb] = m;
   Yr[(long long)lb * nb + la] = m;
   Y2r[(long long)la * nb + lb] = m2;
   Y2r[(long long)lb * nb + la] = m2;
 }
 
-__global__ void init_kernel(BDev d, const mqct_traj_t *__restrict__ tr,
-                            const int *__restrict__ state_pos) {
+__global__ __launch_bounds__(BS) void init_kernel(BDev d, const mqct_traj_t *__restrict__ tr,
+                                                  const int *__restrict__ state_pos) {
+  extern __shared__ double sys[];
+  __shared__ double shm[4];
   const int t = blockIdx.x, tid = threadIdx.x, n2 = 2 * d.N;
   double *S = d.S + (long long)t * NSLOT * n2;
+  const mqct_traj_t p = tr[t];
   for (int i = tid; i < n2; i += BS) S[i] = 0.0;
   __syncthreads();
-  if (tid == 0) {
-    const mqct_traj_t p = tr[t];
-    S[state_pos[p.s_st - 1]] = 1.0;
-    Ctl c{};
-    long long o1 = d.poff[p.at_path + 1];
-    c.x = 0.0;
-    c.x1 = 0.0;
-    c.x2 = d.prec[(o1 - 1) * 8];
-    c.rk4 = d.mode == MQCT_PROP_AT_RK4;
-    c.h = p.dt;
-    if (c.rk4 && c.x + c.h >= c.x2) { c.h = c.x2 - c.x; c.last = 1; }
-    c.hnext = c.h;
-    c.hmin = p.hmin;
-    c.eps = p.eps;
-    c.path = p.at_path;
-    c.status = 0;
-    c.stage = 0;
-    d.ctl[t] = c;
+  if (tid == 0) S[state_pos[p.s_st - 1]] = 1.0;
+  __syncthreads();
+  Ctl c{};
+  const long long o1 = d.poff[p.at_path + 1];
+  c.x = 0.0;
+  c.x1 = 0.0;
+  c.x2 = d.prec[(o1 - 1) * 8];
+  c.rk4 = d.mode == MQCT_PROP_AT_RK4;
+  c.h = p.dt;
+  if (c.rk4 && c.x + c.h >= c.x2) {
+    c.h = c.x2 - c.x;
+    c.last = 1;
   }
+  c.hnext = c.h;
+  c.hmin = p.hmin;
+  c.eps = p.eps;
+  c.path = p.at_path;
+  prepare_stage(d, t, c, sys, shm, 0);
+  if (tid == 0) d.ctl[t] = c;
 }
 
-__global__ void out_kernel(BDev d, const int *__restrict__ pos_state, double *__restrict__ q,
-                           mqct_traj_stat_t *__restrict__ st) {
+__global__ __launch_bounds__(BS) void out_kernel(BDev d, const int *__restrict__ pos_state,
+                                                 double *__restrict__ q, mqct_traj_stat_t *__restrict__ st) {
+  __shared__ double red[8];
   const int t = blockIdx.x, tid = threadIdx.x, N = d.N, n2 = 2 * N;
   const double *y = d.S + (long long)t * NSLOT * n2;
   double *o = q + (long long)t * (n2 + 8);
   for (int p = tid; p < N; p += BS) {
-    int s = pos_state[p];
+    const int s = pos_state[p];
     o[s] = y[p];
     o[s + N] = y[p + N];
   }
   for (int i = tid; i < 8; i += BS) o[n2 + i] = 0.0;
-  __shared__ double red[8];
   double nrm = 0.0;
   for (int i = tid; i < N; i += BS) nrm += y[i] * y[i] + y[i + N] * y[i + N];
   nrm = block_sum(nrm, red);
@@ -504,7 +580,7 @@ __global__ void out_kernel(BDev d, const int *__restrict__ pos_state, double *__
 }  // namespace
 
 int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *traj,
-                         double *q_out, mqct_traj_stat_t *stat_out) {
+                         double *q_out, mqct_traj_stat_t *stat_out, bool knots_fp32) {
   DynState *ds = h->dyn;
   const int N = ds->N, n_r = ds->n_r;
   const long long T = ds->T;
@@ -512,49 +588,45 @@ int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *t
   // block order of the states (m12 ascending, states ascending)
   std::vector<int> pos_state, state_pos(N), blk_start, pos_blk(N);
   std::vector<long long> blk_off;
+  long long S2 = 0;
   {
     std::vector<std::pair<int, int>> key(N);
     for (int s = 0; s < N; s++) key[s] = {ds->m12[s], s};
     std::sort(key.begin(), key.end());
-    long long off = 0;
     for (int p = 0; p < N; p++) {
       pos_state.push_back(key[p].second);
       if (p == 0 || key[p].first != key[p - 1].first) blk_start.push_back(p);
     }
     blk_start.push_back(N);
     for (size_t b = 0; b + 1 < blk_start.size(); b++) {
-      long long nb = blk_start[b + 1] - blk_start[b];
-      blk_off.push_back(off);
-      off += nb * nb;
+      const long long nb = blk_start[b + 1] - blk_start[b];
+      blk_off.push_back(S2);
+      S2 += nb * nb;
       for (int p = blk_start[b]; p < blk_start[b + 1]; p++) pos_blk[p] = (int)b;
     }
-    ds->bat_S2 = off;
   }
   for (int p = 0; p < N; p++) state_pos[pos_state[p]] = p;
   const int nblk = (int)blk_start.size() - 1;
   int maxnb = 0;
   for (int b = 0; b < nblk; b++) maxnb = std::max(maxnb, blk_start[b + 1] - blk_start[b]);
-  if (maxnb > 64 * 1024)
-    return mqct_fail(MQCT_ERR_ARG, "propagate: an m-block is too large for the batched form");
+  const size_t lds_step = (size_t)2 * N * 8;
+  if (lds_step > 64 * 1024)
+    return mqct_fail(MQCT_ERR_ARG,
+                     "propagate: the batched form keeps a stage vector of 2N doubles in LDS; N too large");
   // per position: energies, Coriolis neighbours (state-1 / state+1) and factors
   std::vector<double> E(N), sq1(N), sq2(N);
   std::vector<int> nlo(N), nhi(N);
   for (int p = 0; p < N; p++) {
-    int s = pos_state[p];
+    const int s = pos_state[p];
     E[p] = ds->E[s];
-    double j = ds->j12[s], m = ds->m12[s];
-    double m1 = m + 1.0, m0 = m - 1.0;
+    const double j = ds->j12[s], m = ds->m12[s];
+    const double m1 = m + 1.0, m0 = m - 1.0;
     sq1[p] = std::sqrt(j * (j + 1.0) - m0 * (m0 + 1.0));
     sq2[p] = std::sqrt(j * (j + 1.0) - m1 * (m1 - 1.0));
     nlo[p] = s > 0 ? state_pos[s - 1] : -1;
     nhi[p] = s < N - 1 ? state_pos[s + 1] : -1;
   }
-  const long long S2 = ds->bat_S2;
   const int n2 = 2 * N;
-  // device buffers
-  DevBuf &B = ds->d_bat;
-  size_t need_km = (size_t)n_r * 2 * S2 * 8;
-  if (ds->d_KM.ensure(need_km)) return mqct_fail(MQCT_ERR_NOMEM, "batched: knot matrices");
   DevBuf d_ps, d_sp, d_bs, d_bo, d_pb, d_E, d_nlo, d_nhi, d_sq1, d_sq2, d_R, d_tr;
   auto up = [&](DevBuf &b, const void *src, size_t bytes) -> int {
     if (b.ensure(bytes)) return 1;
@@ -569,24 +641,37 @@ int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *t
       up(d_nhi, nhi.data(), N * 4) || up(d_sq1, sq1.data(), N * 8) || up(d_sq2, sq2.data(), N * 8) ||
       up(d_R, Rmu.data(), (n_r + 1) * 8) || up(d_tr, traj, (size_t)n_traj * sizeof(mqct_traj_t)))
     return mqct_fail(MQCT_ERR_NOMEM, "batched: tables");
-  if (!ds->km_valid) {
+  // knot matrices: made once per matrix and storage precision
+  const size_t need_km = (size_t)n_r * 2 * S2 * (knots_fp32 ? 4 : 8);
+  if (!ds->km_valid || ds->km_fp32 != knots_fp32) {
+    if (ds->d_KM.ensure(need_km)) return mqct_fail(MQCT_ERR_NOMEM, "batched: knot matrices");
     HIP_TRY(hipMemsetAsync(ds->d_KM.p, 0, need_km, h->stream));
-    knot_matrices_kernel<<<(unsigned)((T * n_r + 255) / 256), 256, 0, h->stream>>>(
-        T, n_r, ds->d_Mt.as<double>(), ds->d_Y2t.as<double>(), ds->d_pi.as<int>(), ds->d_pj.as<int>(),
-        d_sp.as<int>(), d_pb.as<int>(), d_bs.as<int>(), d_bo.as<long long>(), S2, ds->d_KM.as<double>());
+    const unsigned g = (unsigned)((T * n_r + 255) / 256);
+    if (knots_fp32)
+      knot_matrices_kernel<float><<<g, 256, 0, h->stream>>>(
+          T, n_r, ds->d_Mt.as<double>(), ds->d_Y2t.as<double>(), ds->d_pi.as<int>(), ds->d_pj.as<int>(),
+          d_sp.as<int>(), d_pb.as<int>(), d_bs.as<int>(), d_bo.as<long long>(), S2, ds->d_KM.as<float>());
+    else
+      knot_matrices_kernel<double><<<g, 256, 0, h->stream>>>(
+          T, n_r, ds->d_Mt.as<double>(), ds->d_Y2t.as<double>(), ds->d_pi.as<int>(), ds->d_pj.as<int>(),
+          d_sp.as<int>(), d_pb.as<int>(), d_bs.as<int>(), d_bo.as<long long>(), S2, ds->d_KM.as<double>());
     HIP_TRY(hipGetLastError());
     ds->km_valid = true;
+    ds->km_fp32 = knots_fp32;
   }
-  size_t off_S = 0, off_V = off_S + (size_t)n_traj * NSLOT * n2 * 8;
-  size_t off_W = off_V + (size_t)n_traj * N * 16;
-  size_t off_C = off_W + (size_t)n_traj * 2 * N * 16;
-  size_t off_cnt = off_C + (size_t)n_traj * sizeof(Ctl);
-  size_t off_bk = off_cnt + ((size_t)n_r * 4 + 255) / 256 * 256;
-  size_t off_act = off_bk + (size_t)n_r * n_traj * 8;
-  size_t off_scal = off_act + 256;
-  size_t total = off_scal + (size_t)n_traj * n2 * 8;
-  if (B.ensure(total)) return mqct_fail(MQCT_ERR_NOMEM, "batched: trajectory state");
-  char *base = B.as<char>();
+  ds->bat_S2 = S2;
+  // trajectory state: slots, rotated amplitudes, GEMM outputs, controls,
+  // double-buffered buckets, counters
+  auto al = [](size_t x) { return (x + 255) / 256 * 256; };
+  const size_t off_V = al((size_t)n_traj * NSLOT * n2 * 8);
+  const size_t off_W = off_V + al((size_t)n_traj * N * 16);
+  const size_t off_C = off_W + al((size_t)n_traj * 2 * N * 16);
+  const size_t off_cnt = off_C + al((size_t)n_traj * sizeof(Ctl));
+  const size_t off_bk = off_cnt + al((size_t)2 * n_r * 4);
+  const size_t off_act = off_bk + al((size_t)2 * n_r * n_traj * 8);
+  const size_t total = off_act + 256;
+  if (ds->d_bat.ensure(total)) return mqct_fail(MQCT_ERR_NOMEM, "batched: trajectory state");
+  char *base = ds->d_bat.as<char>();
   BDev d;
   d.N = N;
   d.nblk = nblk;
@@ -594,7 +679,7 @@ int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *t
   d.n_traj = n_traj;
   d.mode = mode;
   d.R = d_R.as<double>() + 1;
-  d.KM = ds->d_KM.as<double>();
+  d.KM = ds->d_KM.p;
   d.S2 = S2;
   d.blk_start = d_bs.as<int>();
   d.blk_off = d_bo.as<long long>();
@@ -606,7 +691,7 @@ int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *t
   d.prec = ds->d_prec.as<double>();
   d.pder = ds->d_pder.as<double>();
   d.poff = ds->d_poff.as<long long>();
-  d.S = reinterpret_cast<double *>(base + off_S);
+  d.S = reinterpret_cast<double *>(base);
   d.V = reinterpret_cast<double2 *>(base + off_V);
   d.W = reinterpret_cast<double2 *>(base + off_W);
   d.ctl = reinterpret_cast<Ctl *>(base + off_C);
@@ -614,16 +699,19 @@ int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *t
   d.bucket = reinterpret_cast<int2 *>(base + off_bk);
   d.active = reinterpret_cast<int *>(base + off_act);
   d.nbucketed = reinterpret_cast<unsigned long long *>(base + off_act + 8);
-  d.scal = reinterpret_cast<double *>(base + off_scal);
-  HIP_TRY(hipMemsetAsync(d.cnt, 0, (size_t)n_r * 4, h->stream));
-  HIP_TRY(hipMemsetAsync(d.active, 0, 16, h->stream));
+  d.ntouched = reinterpret_cast<unsigned long long *>(base + off_act + 16);
+  HIP_TRY(hipMemsetAsync(d.cnt, 0, (size_t)2 * n_r * 4, h->stream));
+  HIP_TRY(hipMemsetAsync(d.active, 0, 32, h->stream));
   HIP_TRY(hipMemcpyAsync(d.active, &n_traj, 4, hipMemcpyHostToDevice, h->stream));
-  init_kernel<<<n_traj, BS, 0, h->stream>>>(d, d_tr.as<mqct_traj_t>(), d_sp.as<int>());
+  init_kernel<<<n_traj, BS, lds_step, h->stream>>>(d, d_tr.as<mqct_traj_t>(), d_sp.as<int>());
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(h->stream));
-  const int rgroups = (maxnb + 63) / 64;
-  // capture ITER iterations in one graph
-  const int ITER = 16;
+  const dim3 ggrid(n_r, nblk, (maxnb + GROWS - 1) / GROWS);
+  auto gemm = [&](int par) {
+    if (knots_fp32) gemm_kernel<float><<<ggrid, BS, 0, h->stream>>>(d, par);
+    else gemm_kernel<double><<<ggrid, BS, 0, h->stream>>>(d, par);
+  };
+  const int ITER = 16;  // even: the bucket parity pattern repeats per replay
   const bool prof = ds->prof_on;
   int active = n_traj;
   // every trajectory ends after at most MAXSTP accepted steps and a bounded
@@ -631,8 +719,8 @@ int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *t
   const int max_rounds = 1 << 17;
   HIP_TRY(hipEventRecord(h->ev[5], h->stream));
   if (!prof) {
-    // ITER iterations (prep, GEMM, finish) captured once, replayed until
-    // every trajectory has ended
+    // ITER iterations (GEMM, step) captured once, replayed until every
+    // trajectory has ended
     struct GraphGuard {
       hipGraph_t g = nullptr;
       hipGraphExec_t e = nullptr;
@@ -643,11 +731,10 @@ int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *t
     } gg;
     HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
     for (int it = 0; it < ITER; it++) {
-      prep_kernel<<<n_traj, BS, 0, h->stream>>>(d);
-      gemm_kernel<<<dim3(n_r, nblk, rgroups), BS, 0, h->stream>>>(d);
-      finish_kernel<<<n_traj, BS, 0, h->stream>>>(d);
+      gemm(it & 1);
+      step_kernel<<<n_traj, BS, lds_step, h->stream>>>(d, (it & 1) ^ 1);
     }
-    hipError_t cap = hipStreamEndCapture(h->stream, &gg.g);
+    const hipError_t cap = hipStreamEndCapture(h->stream, &gg.g);
     HIP_TRY(cap);
     HIP_TRY(hipGraphInstantiate(&gg.e, gg.g, nullptr, nullptr, 0));
     for (int rep = 0; rep < max_rounds && active > 0; rep++) {
@@ -664,11 +751,10 @@ int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *t
     }
     for (int rep = 0; rep < max_rounds && active > 0; rep++) {
       for (int it = 0; it < ITER; it++) {
-        prep_kernel<<<n_traj, BS, 0, h->stream>>>(d);
         HIP_TRY(hipEventRecord(ds->prof_ev[2 * it], h->stream));
-        gemm_kernel<<<dim3(n_r, nblk, rgroups), BS, 0, h->stream>>>(d);
+        gemm(it & 1);
         HIP_TRY(hipEventRecord(ds->prof_ev[2 * it + 1], h->stream));
-        finish_kernel<<<n_traj, BS, 0, h->stream>>>(d);
+        step_kernel<<<n_traj, BS, lds_step, h->stream>>>(d, (it & 1) ^ 1);
       }
       HIP_TRY(hipGetLastError());
       HIP_TRY(hipMemcpyAsync(&active, d.active, 4, hipMemcpyDeviceToHost, h->stream));
@@ -688,11 +774,17 @@ int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *t
   ds->last_ms = ms;
   if (active > 0) return mqct_fail(MQCT_ERR_HIP, "batched: trajectories did not finish");
   if (prof) {
-    unsigned long long nbk = 0;
-    HIP_TRY(hipMemcpy(&nbk, d.nbucketed, 8, hipMemcpyDeviceToHost));
-    ds->prof_flops += 16.0 * (double)S2 * (double)nbk;
+    unsigned long long cnts[2] = {0, 0};
+    HIP_TRY(hipMemcpy(cnts, d.nbucketed, 16, hipMemcpyDeviceToHost));
+    // per bucketed right-hand side: 2 sides x 2 matrices x nb x nb x (Re, Im)
+    // multiply-adds summed over the blocks = 16 S2 flops; it reads its
+    // rotated amplitudes for both entries and writes both partial products
+    // (4 x 16 N bytes); each touched knot's matrices are read once
+    ds->prof_flops += 16.0 * (double)S2 * (double)cnts[0];
+    ds->prof_bytes += 64.0 * N * (double)cnts[0] + 2.0 * (double)S2 * (knots_fp32 ? 4 : 8) * (double)cnts[1];
   }
-  if (ds->d_q.ensure((size_t)n_traj * (n2 + 8) * 8) || ds->d_stat.ensure((size_t)n_traj * sizeof(mqct_traj_stat_t)))
+  if (ds->d_q.ensure((size_t)n_traj * (n2 + 8) * 8) ||
+      ds->d_stat.ensure((size_t)n_traj * sizeof(mqct_traj_stat_t)))
     return mqct_fail(MQCT_ERR_NOMEM, "batched: output");
   out_kernel<<<n_traj, BS, 0, h->stream>>>(d, d_ps.as<int>(), ds->d_q.as<double>(),
                                            ds->d_stat.as<mqct_traj_stat_t>());
@@ -711,15 +803,17 @@ extern "C" int mqct_hip_batched_profile(mqct_hip_t *h, int on) {
   ds->prof_launches = 0;
   ds->prof_ms = 0.0;
   ds->prof_flops = 0.0;
+  ds->prof_bytes = 0.0;
   return MQCT_OK;
 }
 
 extern "C" int mqct_hip_batched_kernel_stats(mqct_hip_t *h, int64_t *launches, double *gemm_ms,
-                                             double *gemm_flops) {
-  if (!h || !h->dyn || !launches || !gemm_ms || !gemm_flops)
+                                             double *gemm_flops, double *gemm_bytes) {
+  if (!h || !h->dyn || !launches || !gemm_ms || !gemm_flops || !gemm_bytes)
     return mqct_fail(MQCT_ERR_ARG, "batched_kernel_stats: bad argument");
   *launches = h->dyn->prof_launches;
   *gemm_ms = h->dyn->prof_ms;
   *gemm_flops = h->dyn->prof_flops;
+  *gemm_bytes = h->dyn->prof_bytes;
   return MQCT_OK;
 }

@@ -15,8 +15,8 @@
 //   FP64 : v_mfma_f64_16x16x4f64, FP64 accumulation.
 //   FP32 : v_mfma_f32_16x16x4f32, FP32 partial sums over each staged chunk
 //          of grid points added into FP64 accumulators.
-//   BF16X3: v_mfma_f32_16x16x32_bf16 on split operands a = a_hi + a_lo,
-//          w = w_hi + w_lo: a_hi w_hi + a_hi w_lo + a_lo w_hi, FP32 partial
+//   BF16X3: v_mfma_f32_16x16x32_bf16 on operands split into three BF16
+//          terms each, the six cross products of order <= 2, FP32 partial
 //          sums per chunk, FP64 accumulators.
 #include "mqct_internal.h"
 
@@ -177,23 +177,31 @@ __global__ __launch_bounds__(256) void contract_k4_kernel(
 }
 
 // ---------------------------------------------------------------------------
-// Split-BF16 x3 kernel.  KC grid points per stage (multiple of 32).
-__device__ inline void split_bf16(float x, __bf16 &hi, __bf16 &lo) {
-  hi = (__bf16)x;
-  lo = (__bf16)(x - (float)hi);
+// Three-term split-BF16 kernel.  Each operand is carried as three BF16
+// terms, x = x0 + x1 + x2 (about 24 significant bits): the weights come
+// split from HBM, the pair product p = Re_a Re_b + Im_a Im_b (FP32) is split
+// on the fly.  The six cross products of order <= 2,
+//   p2 w0 + p1 w1 + p0 w2 + p1 w0 + p0 w1 + p0 w0,
+// run on v_mfma_f32_16x16x32_bf16 into FP32 partial sums over each staged
+// chunk of KC grid points, added into FP64 accumulators; the omitted terms
+// are below 2^-24 relative.
+__device__ inline void split3_bf16(float x, __bf16 &b0, __bf16 &b1, __bf16 &b2) {
+  b0 = (__bf16)x;
+  const float r1 = x - (float)b0;
+  b1 = (__bf16)r1;
+  b2 = (__bf16)(r1 - (float)b1);
 }
 
 template <int KC>
 __global__ __launch_bounds__(256) void contract_bf16x3_kernel(
     const float *__restrict__ pa_re, const float *__restrict__ pa_im,
     const float *__restrict__ pb_re, const float *__restrict__ pb_im,
-    const __bf16 *__restrict__ Whi, const __bf16 *__restrict__ Wlo,
+    const __bf16 *__restrict__ W0, const __bf16 *__restrict__ W1, const __bf16 *__restrict__ W2,
     long long G, int n_r, const TileDesc *__restrict__ tiles, OutMap om,
     double *__restrict__ out) {
   __shared__ float sAr[TA][KC + 4], sAi[TA][KC + 4];
   __shared__ float sBr[TB][KC + 4], sBi[TB][KC + 4];
-  __shared__ __attribute__((aligned(16))) __bf16 sWh[NR][KC + 8];
-  __shared__ __attribute__((aligned(16))) __bf16 sWl[NR][KC + 8];
+  __shared__ __attribute__((aligned(16))) __bf16 sW[3][NR][KC + 8];
   const TileDesc t = tiles[blockIdx.x];
   const int r0 = blockIdx.y * NR;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -207,6 +215,7 @@ __global__ __launch_bounds__(256) void contract_bf16x3_kernel(
       acc[a][n] = flt4{0, 0, 0, 0};
       acc64[a][n] = dbl4{0, 0, 0, 0};
     }
+  const __bf16 *Wp[3] = {W0, W1, W2};
   for (long long g0 = 0; g0 < G; g0 += KC) {
     for (int e = tid; e < TA * KC; e += 256) {
       int r = e / KC, c = e % KC;
@@ -227,19 +236,13 @@ __global__ __launch_bounds__(256) void contract_bf16x3_kernel(
       long long g = g0 + c;
       int rr = r0 + r;
       bool ok = (g < G && rr < n_r);
-      sWh[r][c] = ok ? Whi[(long long)rr * G + g] : (__bf16)0.0f;
-      sWl[r][c] = ok ? Wlo[(long long)rr * G + g] : (__bf16)0.0f;
+#pragma unroll
+      for (int q = 0; q < 3; q++) sW[q][r][c] = ok ? Wp[q][(long long)rr * G + g] : (__bf16)0.0f;
     }
     __syncthreads();
     const int row = lane & 15, kq = lane >> 4;
     for (int ks = 0; ks < KC; ks += 32) {
       int k0 = ks + 8 * kq;
-      bf16x8 wh[NT], wl[NT];
-#pragma unroll
-      for (int n = 0; n < NT; n++) {
-        wh[n] = *reinterpret_cast<const bf16x8 *>(&sWh[n * 16 + row][k0]);
-        wl[n] = *reinterpret_cast<const bf16x8 *>(&sWl[n * 16 + row][k0]);
-      }
       float br[8], bi[8];
 #pragma unroll
       for (int j = 0; j < 8; j++) {
@@ -249,21 +252,28 @@ __global__ __launch_bounds__(256) void contract_bf16x3_kernel(
 #pragma unroll
       for (int a = 0; a < AW; a++) {
         int al = wave * AW + a;
-        bf16x8 ah, alo;
+        bf16x8 p0, p1, p2;
 #pragma unroll
         for (int j = 0; j < 8; j++) {
           float p = sAr[al][k0 + j] * br[j];
           if (has_im) p += sAi[al][k0 + j] * bi[j];
-          __bf16 h, l;
-          split_bf16(p, h, l);
-          ah[j] = h;
-          alo[j] = l;
+          __bf16 h0, h1, h2;
+          split3_bf16(p, h0, h1, h2);
+          p0[j] = h0;
+          p1[j] = h1;
+          p2[j] = h2;
         }
 #pragma unroll
         for (int n = 0; n < NT; n++) {
-          acc[a][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, wh[n], acc[a][n], 0, 0, 0);
-          acc[a][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, wl[n], acc[a][n], 0, 0, 0);
-          acc[a][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo, wh[n], acc[a][n], 0, 0, 0);
+          const bf16x8 w0 = *reinterpret_cast<const bf16x8 *>(&sW[0][n * 16 + row][k0]);
+          const bf16x8 w1 = *reinterpret_cast<const bf16x8 *>(&sW[1][n * 16 + row][k0]);
+          const bf16x8 w2 = *reinterpret_cast<const bf16x8 *>(&sW[2][n * 16 + row][k0]);
+          acc[a][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p2, w0, acc[a][n], 0, 0, 0);
+          acc[a][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p1, w1, acc[a][n], 0, 0, 0);
+          acc[a][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p0, w2, acc[a][n], 0, 0, 0);
+          acc[a][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p1, w0, acc[a][n], 0, 0, 0);
+          acc[a][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p0, w1, acc[a][n], 0, 0, 0);
+          acc[a][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p0, w0, acc[a][n], 0, 0, 0);
         }
       }
     }
@@ -390,7 +400,7 @@ int mqct_run_contraction(mqct_hip *h, int precision, double conv) {
     contract_bf16x3_kernel<64><<<grid, 256, 0, h->stream>>>(
         h->d_psi32_re.as<float>(), im ? h->d_psi32_im.as<float>() : nullptr,
         pbr, im ? pbi : nullptr, h->d_Whi.as<__bf16>(), h->d_Wlo.as<__bf16>(),
-        h->G, h->n_r, h->d_tiles.as<TileDesc>(), om, out);
+        h->d_Wlo2.as<__bf16>(), h->G, h->n_r, h->d_tiles.as<TileDesc>(), om, out);
   }
   HIP_TRY(hipGetLastError());
   return MQCT_OK;

@@ -19,10 +19,11 @@ struct DynState {
   DevBuf d_KM;       // dense per-knot block matrices [n_r][2][S2]
   long long bat_S2 = 0;
   bool km_valid = false;  // d_KM holds the current matrix
+  bool km_fp32 = false;   // ... stored in FP32
   // batched-form kernel timing
   bool prof_on = false;
   long long prof_launches = 0;
-  double prof_ms = 0.0, prof_flops = 0.0;
+  double prof_ms = 0.0, prof_flops = 0.0, prof_bytes = 0.0;
   std::vector<hipEvent_t> prof_ev;
   ~DynState() {
     for (auto e : prof_ev) (void)hipEventDestroy(e);
@@ -36,4 +37,4 @@ int mqct_dyn_run_plain(mqct_hip *h, int mode, int n_traj,
                        double *rec_out = nullptr, int *rec_n_out = nullptr);
 int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj,
                          const mqct_traj_t *traj, double *q_out,
-                         mqct_traj_stat_t *stat_out);
+                         mqct_traj_stat_t *stat_out, bool knots_fp32);

@@ -83,7 +83,7 @@ struct mqct_hip {
   DevBuf d_R, d_V, d_par;
   DevBuf d_psi_re, d_psi_im, d_psib_re, d_psib_im;  // FP64 [pos][G]
   DevBuf d_psi32_re, d_psi32_im, d_psib32_re, d_psib32_im;
-  DevBuf d_W64, d_W32, d_Whi, d_Wlo;  // [n_r][G]
+  DevBuf d_W64, d_W32, d_Whi, d_Wlo, d_Wlo2;  // [n_r][G]; BF16 terms of W
   DevBuf d_tab1, d_tab2;             // type 0 per-level one-molecule tables
   DevBuf d_tiles;
   DevBuf d_mat, d_der, d_skip, d_spl;  // outputs [T][n_r]

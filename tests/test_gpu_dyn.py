@@ -173,3 +173,29 @@ def test_at_batched_matches_plain_asym_top(dev, mode):
         # the same trajectories (if any) exhaust MAXSTP in both forms
         assert np.array_equal(s0["n_steps"] >= 10000, s1["n_steps"] >= 10000)
     assert np.all(s1["norm_err"] < 1e-6)
+
+
+def test_at_batched_fp32_coefficients(dev):
+    """The batched form with the knot matrices stored in FP32 (batched=2):
+    a fixed perturbation of the coupling of ~6e-8 relative, so amplitudes
+    stay within 1e-6 of the FP64-coefficient form and the norm is
+    conserved as well as there."""
+    from mqct_amd.matrix import MatrixBuilder
+    from mqct_amd.propagate import at_first_step
+    wl = W.asym_top_atom(jmax=3, n_beta=12, n_gamma=24, n_r=120)
+    bb, E = W.at_batch(32, seed=9)
+    paths, l, _ = at_first_step(dev, W.asym_top_atom(jmax=1, n_beta=12, n_gamma=24, n_r=120), bb, E,
+                                time_step=20.0, rec_cap=20000)
+    MatrixBuilder(dev, wl).build("fp64", copy_out=False)
+    prop = Propagator(dev, wl.basis, wl.R, mass_amu=wl.mass_amu, from_build=True)
+    prop.load_paths(paths)
+    rows = []
+    for i in range(len(bb)):
+        rows += prop.trajectories([l[i]], float(E[i]), 1, time_step=10.0, eps=1e-8, hmin=20.0, at_paths=[i])
+    q64, s64 = prop.run(L.MQCT_PROP_AT_RK4, rows, batched=1)
+    q32, s32 = prop.run(L.MQCT_PROP_AT_RK4, rows, batched=2)
+    N = wl.basis.n_states
+    d = np.max(np.abs(q32[:, :2 * N] - q64[:, :2 * N]))
+    assert 0.0 < d <= 1e-6, d
+    assert np.array_equal(s32["n_steps"], s64["n_steps"])
+    assert np.all(s32["norm_err"] < 1e-6)
