@@ -81,6 +81,13 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-xsec", action="store_true")
     ap.add_argument("--no-build", action="store_true")
+    ap.add_argument("--no-c3", action="store_true",
+                    help="skip the two-asymmetric-tops forms (builds, surrogate, 1024 trajectories)")
+    ap.add_argument("--c3-traj", type=int, default=1024)
+    ap.add_argument("--sur-samples", type=int, default=4_000_000)
+    ap.add_argument("--sur-steps", type=int, default=20000)
+    ap.add_argument("--sur-rmin", type=float, default=5.0,
+                    help="the surrogate covers R >= this (bohr); the plain PES fills the grid below")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -149,6 +156,9 @@ def main():
     L.check(dev.lib.mqct_hip_batched_kernel_stats(dev.h, ctypes.byref(la), ctypes.byref(ms), ctypes.byref(fl),
                                                   ctypes.byref(by)), "kernel_stats")
     L.check(dev.lib.mqct_hip_batched_profile(dev.h, 0), "batched_profile")
+    ge, gk, grt = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    L.check(dev.lib.mqct_hip_batched_tile(dev.h, ctypes.byref(ge), ctypes.byref(gk), ctypes.byref(grt)), "tile")
+    tile = {"entries": ge.value, "k_rows": gk.value, "row_tiles_per_wave": grt.value}
     roof = None
     if la.value > 0 and ms.value > 0:
         sec = ms.value * 1e-3 / la.value
@@ -164,7 +174,7 @@ def main():
         roof.update({"traffic": traffic_from_profiles(args.knots),
                      "kernel": "gemm_kernel<%s> (interval GEMM, FP64 MFMA 16x16x4)" %
                                ("float" if form == 2 else "double"),
-                     "launches": la.value, "avg_launch_us": round(sec * 1e6, 2),
+                     "tile": tile, "launches": la.value, "avg_launch_us": round(sec * 1e6, 2),
                      "flops_per_launch": fl.value / la.value, "bytes_per_launch": by.value / la.value,
                      "fp64_mfma_tflops": round(tflops, 3), "mfma_frac": round(f_mfma, 4),
                      "hbm_gbs": round(gbs, 1), "hbm_frac": round(f_hbm, 4),
@@ -233,6 +243,12 @@ def main():
             "%s %s %.1f" % (w, p, build[w][p]["total_ms"]) for w in build for p in build[w]))
     build = {w: {p: {k: round(float(v), 3) for k, v in t.items()} for p, t in d.items()} for w, d in build.items()}
 
+    # -- two asymmetric tops: precision and surrogate forms through 1024 AT
+    #    trajectories -------------------------------------------------------
+    c3 = None
+    if not args.no_c3:
+        c3 = two_tops_section(dev, args)
+
     # -- CPU baseline: the oracle on one core, one trajectory of the batch ----
     cpu = None
     if not args.no_cpu:
@@ -256,9 +272,104 @@ def main():
         "speedup_vs_plain": round(value / plain["traj_per_s"], 2) if plain else None,
         "matrix_build_ms": build,
         "xsec_rel_err_of_builds_vs_fp64": xsec,
+        "two_asym_tops": c3,
         "first_step_paths_truncated": trunc,
     }
     print(json.dumps(out), flush=True)
+
+
+def elem_err(M, M64):
+    """Matrix-element error against the FP64 build, relative to the largest
+    |M| at the same R (elements that cancel to almost nothing carry only
+    absolute accuracy): max and 99.9th percentile over elements and R."""
+    scale = np.maximum(np.abs(M64).max(axis=0, keepdims=True), 1e-300)
+    e = np.abs(M - M64) / scale
+    return {"max": float(e.max()), "p999": float(np.percentile(e, 99.9))}
+
+
+def two_tops_section(dev, args):
+    """The two-asymmetric-tops system (j1, j2 <= 2: 1225 states, 115 070
+    pairs, 16^5 grid, 128 R): FP64, FP32 and split-BF16 builds with the
+    plain PES, and a split-BF16 build on the surrogate PES grid; each matrix
+    drives the same batch of AT trajectories (paths from a first step on
+    j1, j2 <= 1; rk4 with TIME_STEP = 10, the reference's default
+    propagator: the strong couplings near the inner wall of this potential
+    need steps below the odeint step floor that the asymmetric-top workload
+    uses), and the cross sections are compared with the FP64 build's."""
+    from mqct_amd import _lib as L
+    from mqct_amd import workloads as W
+    from mqct_amd.matrix import MatrixBuilder
+    from mqct_amd.propagate import Propagator, at_first_step
+    from mqct_amd.surrogate import train_surrogate
+    n = args.c3_traj
+    wl = W.two_asym_tops()
+    bb, E = W.two_tops_at_batch(n)
+    t0 = time.time()
+    paths, l, st0 = at_first_step(dev, W.two_asym_tops(jmax1=1, jmax2=1), bb, E, time_step=40.0, rec_cap=8000)
+    log("two tops: first AT step %d paths in %.1f s, records max %d, failed %d" %
+        (n, time.time() - t0, max(len(p) for p in paths), int(st0["fail"].sum())))
+    out = {"states": int(wl.basis.n_states), "pairs": int(wl.basis.total_size), "grid_points": int(wl.grid.G),
+           "r_points": int(len(wl.R)), "n_traj": n}
+
+    def propagate(batched=1, sample=None):
+        prop = Propagator(dev, wl.basis, wl.R, mass_amu=wl.mass_amu, from_build=True)
+        prop.load_paths(paths)
+        m = n if sample is None else sample
+        rows = []
+        for i in range(m):
+            rows += prop.trajectories([l[i]], float(E[i]), 1, time_step=10.0, eps=1e-8, hmin=args.hmin,
+                                      at_paths=[i])
+        t0 = time.perf_counter()
+        q, st = prop.run(L.MQCT_PROP_AT_RK4, rows, batched=batched)
+        el = time.perf_counter() - t0
+        return prop, q, st, m / el
+
+    mb = MatrixBuilder(dev, wl)
+    M64 = None
+    sig64 = None
+    for prec in ("fp64", "fp32", "bf16x3"):
+        M, D, S = mb.build(prec)
+        tm = {k: round(float(v), 3) for k, v in mb.timing().items()}
+        prop, q, st, tps = propagate(1)
+        sig = xsec_functional(prop, q, l, E)
+        rec = {"build_ms": tm, "batched_traj_per_s": round(tps, 2), "norm_err_max": float(st["norm_err"].max())}
+        if prec == "fp64":
+            M64, sig64 = M, sig
+            _, qp, _, tpp = propagate(0, sample=min(256, n))
+            rec["plain_traj_per_s"] = round(tpp, 2)
+            rec["plain_max_amp_diff"] = float(np.max(np.abs(qp[:, :2 * prop.N] - q[:len(qp), :2 * prop.N])))
+        else:
+            rec["elem_err"] = elem_err(M, M64)
+            rec["xsec_rel_err"], rec["channels"] = rel_err(sig, sig64)
+        out[prec] = rec
+        log("two tops %s: build %.1f ms (contract %.1f), batched %.1f traj/s%s" % (
+            prec, tm["total_ms"], tm["contract_ms"], tps,
+            "" if prec == "fp64" else ", xsec err %.2e, elem err max %.2e" % (rec["xsec_rel_err"],
+                                                                             rec["elem_err"]["max"])))
+    # the learned potential: trained on the GPU, fills the V grid, split-BF16 build
+    sur, stats = train_surrogate(dev, wl, n_samples=args.sur_samples, steps=args.sur_steps, r_lo=args.sur_rmin,
+                                 log=log)
+    mb2 = MatrixBuilder(dev, wl)  # plain V grid: the fallback below r_min
+    par = L.as_f64(wl.pes_par)
+    R = L.as_f64(wl.R)
+    t0 = time.perf_counter()
+    L.check(dev.lib.mqct_hip_eval_pes(dev.h, len(R), L.dptr(R), wl.pes_kind, len(par), L.dptr(par)), "eval_pes")
+    plain_fill_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    sur.fill_vgrid(wl.R)
+    fill_s = time.perf_counter() - t0
+    M, D, S = mb2.build("bf16x3")
+    tm = {k: round(float(v), 3) for k, v in mb2.timing().items()}
+    prop, q, st, tps = propagate(1)
+    sig = xsec_functional(prop, q, l, E)
+    e, nch = rel_err(sig, sig64)
+    stats.update({"grid_fill_s": round(fill_s, 4), "plain_pes_grid_fill_gpu_s": round(plain_fill_s, 4),
+                  "grid_points": int(wl.grid.G) * len(wl.R), "build_ms": tm, "elem_err": elem_err(M, M64),
+                  "xsec_rel_err": e, "channels": nch, "batched_traj_per_s": round(tps, 2)})
+    out["surrogate_bf16x3"] = stats
+    log("two tops surrogate: fit %.1f s, grid fill %.3f s, xsec err %.2e, elem err max %.2e" % (
+        stats["fit_s"], fill_s, e, stats["elem_err"]["max"]))
+    return out
 
 
 def traffic_from_profiles(knots):
@@ -295,9 +406,26 @@ def cpu_baseline(dev, wl, paths, rows, st):
                               hmin=r["hmin"], R_start=r["R_start"])
     el = time.perf_counter() - t0
     log("cpu oracle: trajectory %d (%d rhs) in %.1f s" % (i, so["n_rhs"], el))
-    return {"value": round(1.0 / el, 5), "unit": "traj/s", "cores": 1, "kind": "port",
-            "sample": "1 trajectory of the batch (index %d, %d right-hand sides; batch mean %.0f), "
-                      "C oracle gcc -O2, 1 thread" % (i, so["n_rhs"], st["n_rhs"].mean())}
+    out = {"value": round(1.0 / el, 5), "unit": "traj/s", "cores": 1, "kind": "port",
+           "sample": "1 trajectory of the batch (index %d, %d right-hand sides; batch mean %.0f), "
+                     "C oracle gcc -O2, 1 thread" % (i, so["n_rhs"], st["n_rhs"].mean())}
+    # the host PES path the surrogate replaces: the oracle's site-site
+    # potential on one core (C loop over an 8^5 x 8 grid), scaled to the
+    # 16^5 x 128 grid
+    try:
+        from mqct_amd import workloads as W
+        w3 = W.two_asym_tops()
+        ws = W.two_asym_tops(jmax1=0, jmax2=0, n_ang=8, n_r=8)
+        t0 = time.perf_counter()
+        O.oracle_for_workload(ws)
+        per = (time.perf_counter() - t0) / (ws.grid.G * len(ws.R))
+        out["host_pes_s_per_point"] = per
+        out["host_pes_grid_s_est"] = per * w3.grid.G * len(w3.R)
+        log("cpu oracle PES: %.3f us per point -> %.0f s for the two-tops grid" % (
+            per * 1e6, out["host_pes_grid_s_est"]))
+    except Exception as e:  # noqa: BLE001
+        log("host PES timing skipped:", e)
+    return out
 
 
 if __name__ == "__main__":

@@ -200,6 +200,12 @@ int mqct_hip_propagate_timing(mqct_hip_t *h, double *ms);
  * 2*S2 values each, plus 64*N bytes of amplitudes in and products out per
  * right-hand side). */
 int mqct_hip_batched_profile(mqct_hip_t *h, int on);
+/* Tile shape of the interval GEMM that the first batched run (with the
+ * coefficient precision of the last run) chose by timing the candidate
+ * shapes on its first iterations: bucket entries per column super-tile, K
+ * rows per LDS slice, 16-row tiles per wave. */
+int mqct_hip_batched_tile(mqct_hip_t *h, int *entries, int *k_rows,
+                          int *row_tiles);
 int mqct_hip_batched_kernel_stats(mqct_hip_t *h, int64_t *launches,
                                   double *gemm_ms, double *gemm_flops,
                                   double *gemm_bytes);

@@ -214,28 +214,22 @@ __device__ void prepare_stage(const BDev &d, int t, Ctl &c, double *sys, double 
   }
 }
 
-// Interval GEMM.  One workgroup per (knot r, m-block b, 128-row group):
+// Interval GEMM.  One workgroup per (knot r, m-block b, row group):
 //   W[t][side][rows] = [Y_r | Y2_r] [w_Y v_t ; w_Y2 v_t]
-// for every bucket entry (t, side) of knot r, 64 entries (128 real columns:
-// Re and Im of each v) per column super-tile.  Each wave owns two 16-row
-// tiles and keeps 8 column tiles of f64 16x16x4 MFMA accumulators for each.
-// K = 2 nb runs in slices of GK rows: the weighted vectors of a slice are
-// staged through a double-buffered LDS tile, and the next slice's matrix
-// fragments and vector values are loaded into registers while the current
-// slice's MFMAs run.  The matrix fragments come straight from HBM/L2 (the
-// blocks are symmetric, so lanes of one k read 16 consecutive rows).
+// for every bucket entry (t, side) of knot r, GE entries (2 GE real
+// columns: Re and Im of each v) per column super-tile.  Each wave owns GRT
+// 16-row tiles with 2 GE / 16 column tiles of f64 16x16x4 MFMA
+// accumulators each; the weighted vectors are staged through LDS in slices
+// of GK rows of K = 2 nb, the matrix fragments come straight from HBM/L2
+// (the blocks are symmetric, so lanes of one k read 16 consecutive rows).
 // Block (0,0,0) also clears the other parity's bucket counts for the step
-// kernel that follows.
-constexpr int GE = 64;             // entries per column super-tile
-constexpr int GC = 2 * GE / 16;    // MFMA column tiles
-constexpr int GK = 16;             // K rows per slice
-constexpr int GLD = 2 * GE + 16;   // LDS row stride in doubles (half-waves hit disjoint banks)
-constexpr int GRT = 2;             // row tiles per wave
-constexpr int GROWS = 4 * GRT * 16;  // rows per workgroup
-constexpr int GSTG = GK * GE / BS;   // staged (k, entry) values per thread per slice
-
-template <typename KT>
+// kernel that follows.  The tile shape is a template parameter; the runner
+// times the shapes on the first iterations and keeps the fastest.
+template <typename KT, int GE, int GK, int GRT>
 __global__ __launch_bounds__(BS) void gemm_kernel(BDev d, int par) {
+  constexpr int GC = 2 * GE / 16;       // MFMA column tiles
+  constexpr int GLD = 2 * GE + 16;      // LDS row stride in doubles (half-waves hit disjoint banks)
+  constexpr int GROWS = 4 * GRT * 16;   // rows per workgroup
   const int r = blockIdx.x, b = blockIdx.y;
   if (r == 0 && b == 0 && blockIdx.z == 0)
     for (int i = threadIdx.x; i < d.n_r; i += BS) d.cnt[(par ^ 1) * d.n_r + i] = 0;
@@ -246,10 +240,9 @@ __global__ __launch_bounds__(BS) void gemm_kernel(BDev d, int par) {
   if (row0 >= nb) return;
   if (b == 0 && blockIdx.z == 0 && threadIdx.x == 0) atomicAdd(d.ntouched, 1ULL);
   const int K = 2 * nb;
-  const int nsl = (K + GK - 1) / GK;
   const KT *Y = static_cast<const KT *>(d.KM) + ((long long)r * 2) * d.S2 + d.blk_off[b];
   const KT *Y2 = static_cast<const KT *>(d.KM) + ((long long)r * 2 + 1) * d.S2 + d.blk_off[b];
-  __shared__ double Bs[2][GK][GLD];
+  __shared__ double Bs[GK][GLD];
   __shared__ int s_t[GE], s_side[GE];
   __shared__ double s_wy[GE], s_wy2[GE];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -263,22 +256,10 @@ __global__ __launch_bounds__(BS) void gemm_kernel(BDev d, int par) {
     rr[q] = rt0 + (lane & 15);
   }
   const int2 *bk = d.bucket + ((long long)par * d.n_r + r) * d.n_traj;
-  // one slice's matrix fragments (lane: rows rr[q], k = k0 + 4 s4 + kq)
-  auto load_a = [&](int k0, double (&af)[GRT][GK / 4]) {
-#pragma unroll
-    for (int q = 0; q < GRT; q++)
-#pragma unroll
-      for (int s4 = 0; s4 < GK / 4; s4++) {
-        const int k = k0 + 4 * s4 + kq;
-        double a = 0.0;
-        if (rr[q] < nb && k < K)
-          a = (k < nb) ? (double)Y[(long long)k * nb + rr[q]] : (double)Y2[(long long)(k - nb) * nb + rr[q]];
-        af[q][s4] = a;
-      }
-  };
   for (int e0 = 0; e0 < ne; e0 += GE) {
     const int nE = min(GE, ne - e0);
     const int nct = (2 * nE + 15) / 16;
+    const int nEpad = nct * 8;  // entries staged (zero columns up to the tile edge)
     __syncthreads();
     if (tid < GE) {
       int t = 0, side = 0;
@@ -296,71 +277,50 @@ __global__ __launch_bounds__(BS) void gemm_kernel(BDev d, int par) {
       s_wy[tid] = wy;
       s_wy2[tid] = wy2;
     }
-    __syncthreads();
-    // one slice's weighted vector values: thread -> (k row kk, entry e),
-    // kk fastest so a group of threads reads 16 consecutive amplitudes
-    auto load_b = [&](int k0, double2 (&vb)[GSTG]) {
-#pragma unroll
-      for (int j = 0; j < GSTG; j++) {
-        const int idx = tid + j * BS, kk = idx % GK, e = idx / GK, k = k0 + kk;
-        double2 v = make_double2(0.0, 0.0);
-        if (e < nE && k < K) {
-          const bool lo = k < nb;
-          const double w = lo ? s_wy[e] : s_wy2[e];
-          const double2 x = d.V[(long long)s_t[e] * d.N + p0 + (lo ? k : k - nb)];
-          v = make_double2(w * x.x, w * x.y);
-        }
-        vb[j] = v;
-      }
-    };
-    auto store_b = [&](int buf, const double2 (&vb)[GSTG]) {
-#pragma unroll
-      for (int j = 0; j < GSTG; j++) {
-        const int idx = tid + j * BS, kk = idx % GK, e = idx / GK;
-        Bs[buf][kk][2 * e] = vb[j].x;
-        Bs[buf][kk][2 * e + 1] = vb[j].y;
-      }
-    };
     dbl4 acc[GRT][GC];
 #pragma unroll
     for (int q = 0; q < GRT; q++)
 #pragma unroll
       for (int ct = 0; ct < GC; ct++) acc[q][ct] = dbl4{0.0, 0.0, 0.0, 0.0};
-    double af[GRT][GK / 4], an[GRT][GK / 4];
-    double2 vb[GSTG];
-    load_a(0, af);
-    load_b(0, vb);
-    store_b(0, vb);
-    __syncthreads();
-    for (int sl = 0; sl < nsl; sl++) {
-      const int cur = sl & 1;
-      const bool more = sl + 1 < nsl;
-      if (more) {
-        load_a((sl + 1) * GK, an);
-        load_b((sl + 1) * GK, vb);
+    for (int k0 = 0; k0 < K; k0 += GK) {
+      __syncthreads();
+      // kk fastest: a group of threads reads 16 consecutive amplitudes
+      for (int idx = tid; idx < GK * nEpad; idx += BS) {
+        const int kk = idx % GK, e = idx / GK, k = k0 + kk;
+        double vx = 0.0, vy = 0.0;
+        if (e < nE && k < K) {
+          const bool lo = k < nb;
+          const double w = lo ? s_wy[e] : s_wy2[e];
+          const double2 x = d.V[(long long)s_t[e] * d.N + p0 + (lo ? k : k - nb)];
+          vx = w * x.x;
+          vy = w * x.y;
+        }
+        Bs[kk][2 * e] = vx;
+        Bs[kk][2 * e + 1] = vy;
       }
+      double af[GRT][GK / 4];
+#pragma unroll
+      for (int q = 0; q < GRT; q++)
+#pragma unroll
+        for (int s4 = 0; s4 < GK / 4; s4++) {
+          const int k = k0 + 4 * s4 + kq;
+          double a = 0.0;
+          if (rr[q] < nb && k < K)
+            a = (k < nb) ? (double)Y[(long long)k * nb + rr[q]] : (double)Y2[(long long)(k - nb) * nb + rr[q]];
+          af[q][s4] = a;
+        }
+      __syncthreads();
 #pragma unroll
       for (int s4 = 0; s4 < GK / 4; s4++) {
-        const double *brow = &Bs[cur][4 * s4 + kq][lane & 15];
-        double bv[GC];
-#pragma unroll
-        for (int ct = 0; ct < GC; ct++) bv[ct] = brow[16 * ct];
+        const double *brow = &Bs[4 * s4 + kq][lane & 15];
 #pragma unroll
         for (int q = 0; q < GRT; q++) {
           if (!tile_on[q]) continue;
 #pragma unroll
           for (int ct = 0; ct < GC; ct++)
-            if (ct < nct) acc[q][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[q][s4], bv[ct], acc[q][ct], 0, 0, 0);
+            if (ct < nct) acc[q][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[q][s4], brow[16 * ct], acc[q][ct], 0, 0, 0);
         }
       }
-      if (more) {
-        store_b(cur ^ 1, vb);
-#pragma unroll
-        for (int q = 0; q < GRT; q++)
-#pragma unroll
-          for (int s4 = 0; s4 < GK / 4; s4++) af[q][s4] = an[q][s4];
-      }
-      __syncthreads();
     }
     // f64 16x16x4 output: row lane/16 + 4i, column lane%16 of each tile
 #pragma unroll
@@ -380,6 +340,105 @@ __global__ __launch_bounds__(BS) void gemm_kernel(BDev d, int par) {
         }
       }
     }
+  }
+}
+
+// The same product without LDS or barriers: every wave owns one 16-row
+// tile and 8 NCT bucket entries (NCT column tiles) and streams its operands
+// straight into registers -- the matrix fragment (16 consecutive rows of one
+// k per lane group) and the weighted amplitude component its lane needs --
+// so waves run independently and occupancy hides the load latency.  Waves
+// loop over further groups of 8 NCT entries.
+template <typename KT, int NCT>
+__global__ __launch_bounds__(BS) void gemm_direct_kernel(BDev d, int par) {
+  const int r = blockIdx.x, b = blockIdx.y;
+  if (r == 0 && b == 0 && blockIdx.z == 0)
+    for (int i = threadIdx.x; i < d.n_r; i += BS) d.cnt[(par ^ 1) * d.n_r + i] = 0;
+  const int ne = d.cnt[par * d.n_r + r];
+  if (ne == 0) return;
+  const int p0 = d.blk_start[b], nb = d.blk_start[b + 1] - p0;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int rt0 = (blockIdx.z * 4 + wave) * 16;
+  if (rt0 >= nb) return;
+  if (b == 0 && blockIdx.z == 0 && threadIdx.x == 0) atomicAdd(d.ntouched, 1ULL);
+  const int K = 2 * nb, kq = lane >> 4, col = lane & 15, comp = col & 1;
+  const int rr = rt0 + (lane & 15);
+  const KT *Y = static_cast<const KT *>(d.KM) + ((long long)r * 2) * d.S2 + d.blk_off[b];
+  const KT *Y2 = static_cast<const KT *>(d.KM) + ((long long)r * 2 + 1) * d.S2 + d.blk_off[b];
+  const int2 *bk = d.bucket + ((long long)par * d.n_r + r) * d.n_traj;
+  for (int e0 = 0; e0 < ne; e0 += 8 * NCT) {
+    const double *vp[NCT];
+    double wy[NCT], wy2[NCT];
+    int tt[NCT], sd[NCT];
+    bool ok[NCT];
+#pragma unroll
+    for (int ct = 0; ct < NCT; ct++) {
+      const int e = e0 + ct * 8 + (col >> 1);
+      ok[ct] = e < ne;
+      int2 ent = ok[ct] ? bk[e] : make_int2(0, 0);
+      const Ctl &c = d.ctl[ent.x];
+      tt[ct] = ent.x;
+      sd[ct] = ent.y;
+      wy[ct] = ok[ct] ? (ent.y ? c.bb : c.aa) : 0.0;
+      wy2[ct] = ok[ct] ? (ent.y ? c.c2 : c.c1) : 0.0;
+      vp[ct] = reinterpret_cast<const double *>(d.V + (long long)ent.x * d.N + p0) + comp;
+    }
+    dbl4 acc[NCT];
+#pragma unroll
+    for (int ct = 0; ct < NCT; ct++) acc[ct] = dbl4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+    for (int k0 = 0; k0 < K; k0 += 4) {
+      const int k = k0 + kq;
+      const bool lo = k < nb, in = k < K;
+      const int kk = lo ? k : k - nb;
+      double a = 0.0;
+      if (rr < nb && in) a = lo ? (double)Y[(long long)k * nb + rr] : (double)Y2[(long long)kk * nb + rr];
+#pragma unroll
+      for (int ct = 0; ct < NCT; ct++) {
+        const double bv = in ? (lo ? wy[ct] : wy2[ct]) * vp[ct][2 * kk] : 0.0;
+        acc[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bv, acc[ct], 0, 0, 0);
+      }
+    }
+    // f64 16x16x4 output: row lane/16 + 4i, column lane%16 of each tile
+#pragma unroll
+    for (int ct = 0; ct < NCT; ct++) {
+      if (!ok[ct]) continue;
+      double *wb = reinterpret_cast<double *>(d.W + ((long long)tt[ct] * 2 + sd[ct]) * d.N + p0) + comp;
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int orow = rt0 + (lane >> 4) + 4 * i;
+        if (orow < nb) wb[2 * orow] = acc[ct][i];
+      }
+    }
+  }
+}
+
+// The tile shapes the runner chooses from: (entries per super-tile, K rows
+// per slice, row tiles per wave).
+struct GemmShape {
+  int ge, gk, grt;
+};
+// (entries 0 in k_rows: the barrier-free form with `entries` per wave)
+constexpr GemmShape kShapes[] = {{64, 16, 1}, {32, 16, 1}, {32, 32, 1}, {16, 32, 1},
+                                 {64, 16, 2}, {32, 0, 1},  {16, 0, 1},  {8, 0, 1}};
+constexpr int kNShapes = sizeof(kShapes) / sizeof(kShapes[0]);
+
+template <typename KT>
+void launch_gemm(int shape, dim3 knots_blocks, int maxnb, hipStream_t st, const BDev &d, int par) {
+  auto go = [&](auto kern, int grt) {
+    const int rows = 4 * grt * 16;
+    dim3 g(knots_blocks.x, knots_blocks.y, (maxnb + rows - 1) / rows);
+    kern<<<g, BS, 0, st>>>(d, par);
+  };
+  switch (shape) {
+    case 0: go(gemm_kernel<KT, 64, 16, 1>, 1); break;
+    case 1: go(gemm_kernel<KT, 32, 16, 1>, 1); break;
+    case 2: go(gemm_kernel<KT, 32, 32, 1>, 1); break;
+    case 3: go(gemm_kernel<KT, 16, 32, 1>, 1); break;
+    case 4: go(gemm_kernel<KT, 64, 16, 2>, 2); break;
+    case 5: go(gemm_direct_kernel<KT, 4>, 1); break;
+    case 6: go(gemm_direct_kernel<KT, 2>, 1); break;
+    default: go(gemm_direct_kernel<KT, 1>, 1); break;
   }
 }
 
@@ -706,18 +765,71 @@ int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *t
   init_kernel<<<n_traj, BS, lds_step, h->stream>>>(d, d_tr.as<mqct_traj_t>(), d_sp.as<int>());
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(h->stream));
-  const dim3 ggrid(n_r, nblk, (maxnb + GROWS - 1) / GROWS);
-  auto gemm = [&](int par) {
-    if (knots_fp32) gemm_kernel<float><<<ggrid, BS, 0, h->stream>>>(d, par);
-    else gemm_kernel<double><<<ggrid, BS, 0, h->stream>>>(d, par);
+  const dim3 kb(n_r, nblk);
+  // a shape chosen on a batch of a different size (by more than 2x) is
+  // chosen again: the best tile depends on the bucket fill
+  const int pi_ = knots_fp32 ? 1 : 0;
+  int shape = ds->gemm_shape[pi_];
+  if (shape >= 0 && (n_traj > 2 * ds->gemm_tuned_n[pi_] || 2 * n_traj < ds->gemm_tuned_n[pi_])) shape = -1;
+  auto gemm = [&](int sh, int par) {
+    if (knots_fp32) launch_gemm<float>(sh, kb, maxnb, h->stream, d, par);
+    else launch_gemm<double>(sh, kb, maxnb, h->stream, d, par);
   };
   const int ITER = 16;  // even: the bucket parity pattern repeats per replay
   const bool prof = ds->prof_on;
   int active = n_traj;
+  long long it_done = 0;  // iterations run so far (parity of the next one)
+  if (ds->prof_ev.empty()) {
+    ds->prof_ev.resize(2 * ITER);
+    for (auto &e : ds->prof_ev) HIP_TRY(hipEventCreate(&e));
+  }
+  // ITER iterations launched one by one with HIP events on this stream
+  // around every GEMM launch; returns the summed GEMM time
+  auto timed_iters = [&](int sh, int n_it, double &gemm_ms) -> int {
+    for (int it = 0; it < n_it; it++) {
+      const int par = (int)((it_done + it) & 1);
+      HIP_TRY(hipEventRecord(ds->prof_ev[2 * it], h->stream));
+      gemm(sh, par);
+      HIP_TRY(hipEventRecord(ds->prof_ev[2 * it + 1], h->stream));
+      step_kernel<<<n_traj, BS, lds_step, h->stream>>>(d, par ^ 1);
+    }
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(&active, d.active, 4, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    gemm_ms = 0.0;
+    for (int it = 0; it < n_it; it++) {
+      float ems = 0.f;
+      HIP_TRY(hipEventElapsedTime(&ems, ds->prof_ev[2 * it], ds->prof_ev[2 * it + 1]));
+      gemm_ms += ems;
+    }
+    it_done += n_it;
+    if (prof) {
+      ds->prof_ms += gemm_ms;
+      ds->prof_launches += n_it;
+    }
+    return MQCT_OK;
+  };
   // every trajectory ends after at most MAXSTP accepted steps and a bounded
   // number of step-size reductions; the cap only guards the host loop
   const int max_rounds = 1 << 17;
   HIP_TRY(hipEventRecord(h->ev[5], h->stream));
+  if (shape < 0) {
+    // tile-shape tuning on the first iterations (real iterations of this
+    // run): three rounds of two timed iterations per shape, the fastest
+    // total wins and is kept for later runs on this matrix
+    double tot[kNShapes] = {};
+    for (int round = 0; round < 3 && active > 0; round++)
+      for (int sh = 0; sh < kNShapes && active > 0; sh++) {
+        double gm = 0.0;
+        if (int rc = timed_iters(sh, 2, gm)) return rc;
+        tot[sh] += gm;
+      }
+    shape = 0;
+    for (int sh = 1; sh < kNShapes; sh++)
+      if (tot[sh] < tot[shape]) shape = sh;
+    ds->gemm_shape[pi_] = shape;
+    ds->gemm_tuned_n[pi_] = n_traj;
+  }
   if (!prof) {
     // ITER iterations (GEMM, step) captured once, replayed until every
     // trajectory has ended
@@ -729,42 +841,27 @@ int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *t
         if (g) (void)hipGraphDestroy(g);
       }
     } gg;
-    HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-    for (int it = 0; it < ITER; it++) {
-      gemm(it & 1);
-      step_kernel<<<n_traj, BS, lds_step, h->stream>>>(d, (it & 1) ^ 1);
+    if (active > 0) {
+      const int p0 = (int)(it_done & 1);
+      HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+      for (int it = 0; it < ITER; it++) {
+        const int par = (it + p0) & 1;
+        gemm(shape, par);
+        step_kernel<<<n_traj, BS, lds_step, h->stream>>>(d, par ^ 1);
+      }
+      const hipError_t cap = hipStreamEndCapture(h->stream, &gg.g);
+      HIP_TRY(cap);
+      HIP_TRY(hipGraphInstantiate(&gg.e, gg.g, nullptr, nullptr, 0));
     }
-    const hipError_t cap = hipStreamEndCapture(h->stream, &gg.g);
-    HIP_TRY(cap);
-    HIP_TRY(hipGraphInstantiate(&gg.e, gg.g, nullptr, nullptr, 0));
     for (int rep = 0; rep < max_rounds && active > 0; rep++) {
       HIP_TRY(hipGraphLaunch(gg.e, h->stream));
       HIP_TRY(hipMemcpyAsync(&active, d.active, 4, hipMemcpyDeviceToHost, h->stream));
       HIP_TRY(hipStreamSynchronize(h->stream));
     }
   } else {
-    // the same iterations launched one by one with HIP events on this
-    // stream around every GEMM launch (kernel timing)
-    if (ds->prof_ev.empty()) {
-      ds->prof_ev.resize(2 * ITER);
-      for (auto &e : ds->prof_ev) HIP_TRY(hipEventCreate(&e));
-    }
     for (int rep = 0; rep < max_rounds && active > 0; rep++) {
-      for (int it = 0; it < ITER; it++) {
-        HIP_TRY(hipEventRecord(ds->prof_ev[2 * it], h->stream));
-        gemm(it & 1);
-        HIP_TRY(hipEventRecord(ds->prof_ev[2 * it + 1], h->stream));
-        step_kernel<<<n_traj, BS, lds_step, h->stream>>>(d, (it & 1) ^ 1);
-      }
-      HIP_TRY(hipGetLastError());
-      HIP_TRY(hipMemcpyAsync(&active, d.active, 4, hipMemcpyDeviceToHost, h->stream));
-      HIP_TRY(hipStreamSynchronize(h->stream));
-      for (int it = 0; it < ITER; it++) {
-        float ems = 0.f;
-        HIP_TRY(hipEventElapsedTime(&ems, ds->prof_ev[2 * it], ds->prof_ev[2 * it + 1]));
-        ds->prof_ms += ems;
-      }
-      ds->prof_launches += ITER;
+      double gm = 0.0;
+      if (int rc = timed_iters(shape, ITER, gm)) return rc;
     }
   }
   HIP_TRY(hipEventRecord(h->ev[6], h->stream));
@@ -804,6 +901,17 @@ extern "C" int mqct_hip_batched_profile(mqct_hip_t *h, int on) {
   ds->prof_ms = 0.0;
   ds->prof_flops = 0.0;
   ds->prof_bytes = 0.0;
+  return MQCT_OK;
+}
+
+extern "C" int mqct_hip_batched_tile(mqct_hip_t *h, int *entries, int *k_rows, int *row_tiles) {
+  if (!h || !h->dyn || !entries || !k_rows || !row_tiles)
+    return mqct_fail(MQCT_ERR_ARG, "batched_tile: bad argument");
+  const int sh = h->dyn->km_fp32 ? h->dyn->gemm_shape[1] : h->dyn->gemm_shape[0];
+  if (sh < 0) return mqct_fail(MQCT_ERR_STATE, "batched_tile: no batched run yet");
+  *entries = kShapes[sh].ge;
+  *k_rows = kShapes[sh].gk;
+  *row_tiles = kShapes[sh].grt;
   return MQCT_OK;
 }
 

@@ -177,6 +177,139 @@ __global__ __launch_bounds__(256) void contract_k4_kernel(
 }
 
 // ---------------------------------------------------------------------------
+// The same contraction with 16-byte staging and a double-buffered LDS: the
+// next chunk's basis rows and weights are loaded into registers while the
+// current chunk's MFMAs run.  Needs G to be a multiple of the vector width
+// (16 / sizeof(T)); the launcher falls back to contract_k4_kernel otherwise.
+template <typename T> struct Vec16;
+template <> struct Vec16<double> { typedef double2 type; };
+template <> struct Vec16<float> { typedef float4 type; };
+
+template <typename T, int KC, int NRT>
+__global__ __launch_bounds__(256) void contract_v_kernel(
+    const T *__restrict__ pa_re, const T *__restrict__ pa_im,
+    const T *__restrict__ pb_re, const T *__restrict__ pb_im,
+    const T *__restrict__ W, long long G, int n_r,
+    const TileDesc *__restrict__ tiles, OutMap om, double *__restrict__ out) {
+  typedef typename Vec16<T>::type V;
+  constexpr int VW = 16 / sizeof(T);                  // elements per vector
+  constexpr int KV = KC / VW;                         // vectors per staged row
+  constexpr int KP = KC + (sizeof(T) == 8 ? 2 : 4);   // padded row: 16-B aligned, conflict-free reads
+  constexpr int NTT = NRT / 16;
+  constexpr int PA = 2 * TA * KV / 256;               // A (and B) vectors per thread per chunk
+  constexpr int PW = NRT * KV / 256;                  // W vectors per thread per chunk
+  static_assert(PA * 256 == 2 * TA * KV && PW * 256 == NRT * KV, "staging must tile the workgroup");
+  __shared__ __attribute__((aligned(16))) T sA[2][2][TA][KP];
+  __shared__ __attribute__((aligned(16))) T sB[2][2][TB][KP];
+  __shared__ __attribute__((aligned(16))) T sW[2][NRT][KP];
+  const TileDesc t = tiles[blockIdx.x];
+  const int r0 = blockIdx.y * NRT;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const bool has_im = (pa_im != nullptr);
+  const T *pa[2] = {pa_re, pa_im}, *pb[2] = {pb_re, pb_im};
+
+  typedef typename std::conditional<sizeof(T) == 8, dbl4, flt4>::type accv;
+  accv acc[AW][NTT];
+  dbl4 acc64[AW][NTT];
+#pragma unroll
+  for (int a = 0; a < AW; a++)
+#pragma unroll
+    for (int n = 0; n < NTT; n++) {
+      acc[a][n] = accv{0, 0, 0, 0};
+      acc64[a][n] = dbl4{0, 0, 0, 0};
+    }
+  V ra[PA], rb[PA], rw[PW];
+  const V vz = {};
+  auto load = [&](long long g0) {
+#pragma unroll
+    for (int j = 0; j < PA; j++) {
+      const int idx = tid + 256 * j, plane = idx / (TA * KV), rem = idx % (TA * KV);
+      const int row = rem / KV, kv = rem % KV;
+      const long long g = g0 + (long long)kv * VW;
+      const bool okp = (plane == 0 || has_im) && g < G;
+      ra[j] = (okp && row < t.na) ? *reinterpret_cast<const V *>(pa[plane] + (long long)(t.a0 + row) * G + g) : vz;
+      rb[j] = (okp && row < t.nb) ? *reinterpret_cast<const V *>(pb[plane] + (long long)(t.b0 + row) * G + g) : vz;
+    }
+#pragma unroll
+    for (int j = 0; j < PW; j++) {
+      const int idx = tid + 256 * j, row = idx / KV, kv = idx % KV;
+      const long long g = g0 + (long long)kv * VW;
+      const int rr = r0 + row;
+      rw[j] = (g < G && rr < n_r) ? *reinterpret_cast<const V *>(W + (long long)rr * G + g) : vz;
+    }
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int j = 0; j < PA; j++) {
+      const int idx = tid + 256 * j, plane = idx / (TA * KV), rem = idx % (TA * KV);
+      const int row = rem / KV, kv = rem % KV;
+      *reinterpret_cast<V *>(&sA[buf][plane][row][kv * VW]) = ra[j];
+      *reinterpret_cast<V *>(&sB[buf][plane][row][kv * VW]) = rb[j];
+    }
+#pragma unroll
+    for (int j = 0; j < PW; j++) {
+      const int idx = tid + 256 * j, row = idx / KV, kv = idx % KV;
+      *reinterpret_cast<V *>(&sW[buf][row][kv * VW]) = rw[j];
+    }
+  };
+  const long long nch = (G + KC - 1) / KC;
+  load(0);
+  store(0);
+  __syncthreads();
+  const int row = lane & 15, kq = lane >> 4;
+  for (long long ch = 0; ch < nch; ch++) {
+    const int buf = (int)(ch & 1);
+    const bool more = ch + 1 < nch;
+    if (more) load((ch + 1) * KC);
+#pragma unroll 2
+    for (int ks = 0; ks < KC; ks += 4) {
+      const int k = ks + kq;
+      const T br = sB[buf][0][row][k];
+      const T bi = has_im ? sB[buf][1][row][k] : T(0);
+      T wv[NTT];
+#pragma unroll
+      for (int n = 0; n < NTT; n++) wv[n] = sW[buf][n * 16 + row][k];
+#pragma unroll
+      for (int a = 0; a < AW; a++) {
+        const int al = wave * AW + a;
+        T av = sA[buf][0][al][k] * br;
+        if (has_im) av += sA[buf][1][al][k] * bi;
+#pragma unroll
+        for (int n = 0; n < NTT; n++) {
+          if constexpr (sizeof(T) == 8)
+            acc[a][n] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, wv[n], acc[a][n], 0, 0, 0);
+          else
+            acc[a][n] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, wv[n], acc[a][n], 0, 0, 0);
+        }
+      }
+    }
+    if constexpr (sizeof(T) == 4) {
+#pragma unroll
+      for (int a = 0; a < AW; a++)
+#pragma unroll
+        for (int n = 0; n < NTT; n++)
+#pragma unroll
+          for (int i = 0; i < 4; i++) {
+            acc64[a][n][i] += (double)acc[a][n][i];
+            acc[a][n][i] = 0.0f;
+          }
+    }
+    if (more) store(buf ^ 1);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int a = 0; a < AW; a++)
+#pragma unroll
+    for (int n = 0; n < NTT; n++) {
+      double v[4];
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+        v[i] = (sizeof(T) == 8) ? (double)acc[a][n][i] : acc64[a][n][i];
+      store_tile<sizeof(T) == 8>(om, t, wave * AW + a, r0, n, lane, v, out);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Three-term split-BF16 kernel.  Each operand is carried as three BF16
 // terms, x = x0 + x1 + x2 (about 24 significant bits): the weights come
 // split from HBM, the pair product p = Re_a Re_b + Im_a Im_b (FP32) is split
@@ -299,6 +432,147 @@ __global__ __launch_bounds__(256) void contract_bf16x3_kernel(
     }
 }
 
+// The three-term split-BF16 contraction with 16-byte staging and a
+// double-buffered LDS (needs G to be a multiple of 8).
+template <int KC>
+__global__ __launch_bounds__(256) void contract_bf16x3_v_kernel(
+    const float *__restrict__ pa_re, const float *__restrict__ pa_im,
+    const float *__restrict__ pb_re, const float *__restrict__ pb_im,
+    const __bf16 *__restrict__ W0, const __bf16 *__restrict__ W1, const __bf16 *__restrict__ W2,
+    long long G, int n_r, const TileDesc *__restrict__ tiles, OutMap om,
+    double *__restrict__ out) {
+  constexpr int KP = KC + 4;             // fp32 rows
+  constexpr int KPW = KC + 8;            // bf16 rows
+  constexpr int KV = KC / 4;             // float4 per fp32 row
+  constexpr int KVW = KC / 8;            // 8 x bf16 per weight row
+  constexpr int PA = 2 * TA * KV / 256;  // A (and B) vectors per thread
+  constexpr int PW = 3 * NR * KVW / 256; // weight vectors per thread
+  static_assert(PA * 256 == 2 * TA * KV && PW * 256 == 3 * NR * KVW, "staging must tile the workgroup");
+  __shared__ __attribute__((aligned(16))) float sA[2][2][TA][KP];
+  __shared__ __attribute__((aligned(16))) float sB[2][2][TB][KP];
+  __shared__ __attribute__((aligned(16))) __bf16 sW[2][3][NR][KPW];
+  const TileDesc t = tiles[blockIdx.x];
+  const int r0 = blockIdx.y * NR;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const bool has_im = (pa_im != nullptr);
+  const float *pa[2] = {pa_re, pa_im}, *pb[2] = {pb_re, pb_im};
+  const __bf16 *Wp[3] = {W0, W1, W2};
+  flt4 acc[AW][NT];
+  dbl4 acc64[AW][NT];
+#pragma unroll
+  for (int a = 0; a < AW; a++)
+#pragma unroll
+    for (int n = 0; n < NT; n++) {
+      acc[a][n] = flt4{0, 0, 0, 0};
+      acc64[a][n] = dbl4{0, 0, 0, 0};
+    }
+  float4 ra[PA], rb[PA];
+  uint4 rw[PW];
+  auto load = [&](long long g0) {
+#pragma unroll
+    for (int j = 0; j < PA; j++) {
+      const int idx = tid + 256 * j, plane = idx / (TA * KV), rem = idx % (TA * KV);
+      const int row = rem / KV, kv = rem % KV;
+      const long long g = g0 + (long long)kv * 4;
+      const bool okp = (plane == 0 || has_im) && g < G;
+      ra[j] = (okp && row < t.na) ? *reinterpret_cast<const float4 *>(pa[plane] + (long long)(t.a0 + row) * G + g)
+                                  : float4{0, 0, 0, 0};
+      rb[j] = (okp && row < t.nb) ? *reinterpret_cast<const float4 *>(pb[plane] + (long long)(t.b0 + row) * G + g)
+                                  : float4{0, 0, 0, 0};
+    }
+#pragma unroll
+    for (int j = 0; j < PW; j++) {
+      const int idx = tid + 256 * j, q = idx / (NR * KVW), rem = idx % (NR * KVW);
+      const int row = rem / KVW, kv = rem % KVW;
+      const long long g = g0 + (long long)kv * 8;
+      const int rr = r0 + row;
+      rw[j] = (g < G && rr < n_r) ? *reinterpret_cast<const uint4 *>(Wp[q] + (long long)rr * G + g)
+                                  : uint4{0, 0, 0, 0};
+    }
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int j = 0; j < PA; j++) {
+      const int idx = tid + 256 * j, plane = idx / (TA * KV), rem = idx % (TA * KV);
+      const int row = rem / KV, kv = rem % KV;
+      *reinterpret_cast<float4 *>(&sA[buf][plane][row][kv * 4]) = ra[j];
+      *reinterpret_cast<float4 *>(&sB[buf][plane][row][kv * 4]) = rb[j];
+    }
+#pragma unroll
+    for (int j = 0; j < PW; j++) {
+      const int idx = tid + 256 * j, q = idx / (NR * KVW), rem = idx % (NR * KVW);
+      const int row = rem / KVW, kv = rem % KVW;
+      *reinterpret_cast<uint4 *>(&sW[buf][q][row][kv * 8]) = rw[j];
+    }
+  };
+  const long long nch = (G + KC - 1) / KC;
+  load(0);
+  store(0);
+  __syncthreads();
+  const int row = lane & 15, kq = lane >> 4;
+  for (long long ch = 0; ch < nch; ch++) {
+    const int buf = (int)(ch & 1);
+    const bool more = ch + 1 < nch;
+    if (more) load((ch + 1) * KC);
+    for (int ks = 0; ks < KC; ks += 32) {
+      const int k0 = ks + 8 * kq;
+      float br[8], bi[8];
+#pragma unroll
+      for (int j = 0; j < 8; j++) {
+        br[j] = sB[buf][0][row][k0 + j];
+        bi[j] = has_im ? sB[buf][1][row][k0 + j] : 0.0f;
+      }
+#pragma unroll
+      for (int a = 0; a < AW; a++) {
+        const int al = wave * AW + a;
+        bf16x8 p0, p1, p2;
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+          float p = sA[buf][0][al][k0 + j] * br[j];
+          if (has_im) p += sA[buf][1][al][k0 + j] * bi[j];
+          __bf16 h0, h1, h2;
+          split3_bf16(p, h0, h1, h2);
+          p0[j] = h0;
+          p1[j] = h1;
+          p2[j] = h2;
+        }
+#pragma unroll
+        for (int n = 0; n < NT; n++) {
+          const bf16x8 w0 = *reinterpret_cast<const bf16x8 *>(&sW[buf][0][n * 16 + row][k0]);
+          const bf16x8 w1 = *reinterpret_cast<const bf16x8 *>(&sW[buf][1][n * 16 + row][k0]);
+          const bf16x8 w2 = *reinterpret_cast<const bf16x8 *>(&sW[buf][2][n * 16 + row][k0]);
+          acc[a][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p2, w0, acc[a][n], 0, 0, 0);
+          acc[a][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p1, w1, acc[a][n], 0, 0, 0);
+          acc[a][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p0, w2, acc[a][n], 0, 0, 0);
+          acc[a][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p1, w0, acc[a][n], 0, 0, 0);
+          acc[a][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p0, w1, acc[a][n], 0, 0, 0);
+          acc[a][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p0, w0, acc[a][n], 0, 0, 0);
+        }
+      }
+    }
+#pragma unroll
+    for (int a = 0; a < AW; a++)
+#pragma unroll
+      for (int n = 0; n < NT; n++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          acc64[a][n][i] += (double)acc[a][n][i];
+          acc[a][n][i] = 0.0f;
+        }
+    if (more) store(buf ^ 1);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int a = 0; a < AW; a++)
+#pragma unroll
+    for (int n = 0; n < NT; n++) {
+      double v[4];
+#pragma unroll
+      for (int i = 0; i < 4; i++) v[i] = acc64[a][n][i];
+      store_tile<false>(om, t, wave * AW + a, r0, n, lane, v, out);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Single-instruction probes of the lane layouts assumed above.
 __global__ void probe_kernel(int which, const double *A, const double *B,
@@ -381,26 +655,44 @@ int mqct_run_contraction(mqct_hip *h, int precision, double conv) {
     dim3 grid((unsigned)ntiles, (unsigned)((h->n_r + NR - 1) / NR));
     const double *pbr = h->ref_xz ? h->d_psi_re.as<double>() : h->d_psib_re.as<double>();
     const double *pbi = h->ref_xz ? h->d_psi_im.as<double>() : h->d_psib_im.as<double>();
-    contract_k4_kernel<double, 32, 64><<<grid, 256, 0, h->stream>>>(
-        h->d_psi_re.as<double>(), im ? h->d_psi_im.as<double>() : nullptr,
-        pbr, im ? pbi : nullptr, h->d_W64.as<double>(), h->G, h->n_r,
-        h->d_tiles.as<TileDesc>(), om, out);
+    if (h->G % 2 == 0)
+      contract_v_kernel<double, 32, 64><<<grid, 256, 0, h->stream>>>(
+          h->d_psi_re.as<double>(), im ? h->d_psi_im.as<double>() : nullptr,
+          pbr, im ? pbi : nullptr, h->d_W64.as<double>(), h->G, h->n_r,
+          h->d_tiles.as<TileDesc>(), om, out);
+    else
+      contract_k4_kernel<double, 32, 64><<<grid, 256, 0, h->stream>>>(
+          h->d_psi_re.as<double>(), im ? h->d_psi_im.as<double>() : nullptr,
+          pbr, im ? pbi : nullptr, h->d_W64.as<double>(), h->G, h->n_r,
+          h->d_tiles.as<TileDesc>(), om, out);
   } else if (precision == MQCT_PREC_FP32) {
     dim3 grid((unsigned)ntiles, (unsigned)((h->n_r + NR - 1) / NR));
     const float *pbr = h->ref_xz ? h->d_psi32_re.as<float>() : h->d_psib32_re.as<float>();
     const float *pbi = h->ref_xz ? h->d_psi32_im.as<float>() : h->d_psib32_im.as<float>();
-    contract_k4_kernel<float, 64, 64><<<grid, 256, 0, h->stream>>>(
-        h->d_psi32_re.as<float>(), im ? h->d_psi32_im.as<float>() : nullptr,
-        pbr, im ? pbi : nullptr, h->d_W32.as<float>(), h->G, h->n_r,
-        h->d_tiles.as<TileDesc>(), om, out);
+    if (h->G % 4 == 0)
+      contract_v_kernel<float, 64, 64><<<grid, 256, 0, h->stream>>>(
+          h->d_psi32_re.as<float>(), im ? h->d_psi32_im.as<float>() : nullptr,
+          pbr, im ? pbi : nullptr, h->d_W32.as<float>(), h->G, h->n_r,
+          h->d_tiles.as<TileDesc>(), om, out);
+    else
+      contract_k4_kernel<float, 64, 64><<<grid, 256, 0, h->stream>>>(
+          h->d_psi32_re.as<float>(), im ? h->d_psi32_im.as<float>() : nullptr,
+          pbr, im ? pbi : nullptr, h->d_W32.as<float>(), h->G, h->n_r,
+          h->d_tiles.as<TileDesc>(), om, out);
   } else {
     dim3 grid((unsigned)ntiles, (unsigned)((h->n_r + NR - 1) / NR));
     const float *pbr = h->ref_xz ? h->d_psi32_re.as<float>() : h->d_psib32_re.as<float>();
     const float *pbi = h->ref_xz ? h->d_psi32_im.as<float>() : h->d_psib32_im.as<float>();
-    contract_bf16x3_kernel<64><<<grid, 256, 0, h->stream>>>(
-        h->d_psi32_re.as<float>(), im ? h->d_psi32_im.as<float>() : nullptr,
-        pbr, im ? pbi : nullptr, h->d_Whi.as<__bf16>(), h->d_Wlo.as<__bf16>(),
-        h->d_Wlo2.as<__bf16>(), h->G, h->n_r, h->d_tiles.as<TileDesc>(), om, out);
+    if (h->G % 8 == 0)
+      contract_bf16x3_v_kernel<64><<<grid, 256, 0, h->stream>>>(
+          h->d_psi32_re.as<float>(), im ? h->d_psi32_im.as<float>() : nullptr,
+          pbr, im ? pbi : nullptr, h->d_Whi.as<__bf16>(), h->d_Wlo.as<__bf16>(),
+          h->d_Wlo2.as<__bf16>(), h->G, h->n_r, h->d_tiles.as<TileDesc>(), om, out);
+    else
+      contract_bf16x3_kernel<64><<<grid, 256, 0, h->stream>>>(
+          h->d_psi32_re.as<float>(), im ? h->d_psi32_im.as<float>() : nullptr,
+          pbr, im ? pbi : nullptr, h->d_Whi.as<__bf16>(), h->d_Wlo.as<__bf16>(),
+          h->d_Wlo2.as<__bf16>(), h->G, h->n_r, h->d_tiles.as<TileDesc>(), om, out);
   }
   HIP_TRY(hipGetLastError());
   return MQCT_OK;

@@ -20,6 +20,8 @@ struct DynState {
   long long bat_S2 = 0;
   bool km_valid = false;  // d_KM holds the current matrix
   bool km_fp32 = false;   // ... stored in FP32
+  int gemm_shape[2] = {-1, -1};  // interval-GEMM tile shape per coefficient precision (FP64, FP32)
+  int gemm_tuned_n[2] = {0, 0};  // batch size the shape was chosen on
   // batched-form kernel timing
   bool prof_on = false;
   long long prof_launches = 0;

@@ -147,3 +147,36 @@ class Surrogate:
         (points below r_min keep the plain potential already there)."""
         R = L.as_f64(R)
         L.check(self.lib.mqct_hip_surrogate_vgrid(self.h, len(R), L.dptr(R)), "surrogate_vgrid")
+
+
+def train_surrogate(dev, wl, n_samples=4_000_000, steps=20000, batch=4096, seed=20240506, r_lo=None,
+                    r_hi=None, lr0=2e-3, lr1=2e-5, scale=20.0, log=None):
+    """Trains a surrogate of the workload's potential on n_samples seeded
+    random geometries that the library's own PES kernel evaluates (R with
+    density ~ 1/R over [r_lo, r_hi], by default the workload's R range; grid
+    points below r_lo keep the plain potential).  Returns the network and a
+    dict of timings and fit errors on a held-out set."""
+    import time
+    r_lo = float(wl.R[0]) if r_lo is None else r_lo
+    r_hi = float(wl.R[-1]) if r_hi is None else r_hi
+    t0 = time.perf_counter()
+    geo = sample_geometries(n_samples, r_lo, r_hi, seed)
+    V = pes_points(dev, wl, geo)
+    t_data = time.perf_counter() - t0
+    sur = Surrogate(dev, seed=seed, scale=scale, shift=0.0, r_min=r_lo)
+    sur.set_data(geo, V)
+    t0 = time.perf_counter()
+    losses = sur.fit(steps, batch, lr0=lr0, lr1=lr1, log_every=max(1, steps // 10), log=log)
+    t_fit = time.perf_counter() - t0
+    test = sample_geometries(200_000, r_lo, r_hi, seed + 1)
+    Vt = pes_points(dev, wl, test)
+    Vp = sur.predict(test)
+    err = np.abs(Vp - Vt)
+    big = np.abs(Vt) > 100.0
+    stats = {"samples": n_samples, "steps": steps, "batch": batch, "data_s": t_data, "fit_s": t_fit,
+             "final_loss": float(np.mean(losses[-100:])),
+             "test_abs_err_small_V_cm": float(err[~big].max()) if (~big).any() else None,
+             "test_rel_err_large_V": float((err[big] / np.abs(Vt[big])).max()) if big.any() else None,
+             "test_rms_abs_err_cm": float(np.sqrt(np.mean(err ** 2)))}
+    return sur, stats
+

@@ -154,3 +154,12 @@ def at_batch(n_traj=4096, b_max=18.0, e_min=200.0, e_max=2000.0, seed=20240504):
     b = rng.uniform(0.0, b_max, n_traj)
     E = rng.uniform(e_min, e_max, n_traj)
     return b, E
+
+
+def two_tops_at_batch(n_traj=1024, b_max=15.0, e_min=200.0, e_max=1500.0, seed=20240505):
+    """AT-MQCT trajectories of the two-asymmetric-tops system: impact
+    parameters uniform in [0, b_max] bohr and collision energies uniform in
+    [e_min, e_max] cm^-1 (seeded)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    return rng.uniform(0.0, b_max, n_traj), rng.uniform(e_min, e_max, n_traj)
+

@@ -122,7 +122,9 @@ def test_asym_top_atom_full_size_sampled(dev):
     assert worst <= 1.0
 
 
-MIXED_TOL = {"fp32": 4e-6, "bf16x3": 6e-5}
+# FP32: operands and chunk sums in FP32; split-BF16: three BF16 terms per
+# operand (about 24 bits), six cross products; both with FP64 chunk totals.
+MIXED_TOL = {"fp32": 4e-6, "bf16x3": 4e-6}
 
 
 def check_mixed(dev, wl, prec):
@@ -147,3 +149,11 @@ def test_mixed_precision_diatom_atom(dev, prec):
 @pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
 def test_mixed_precision_asym_top(dev, prec):
     check_mixed(dev, W.asym_top_atom(jmax=3, n_beta=12, n_gamma=24, n_r=20), prec)
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+def test_mixed_precision_two_asym_tops(dev, prec):
+    """Type 0 with a grid of 192 points (a multiple of the 16-byte staging
+    width of every form)."""
+    check_mixed(dev, W.two_asym_tops(jmax1=1, jmax2=1, n_ang_each=(2, 4, 3, 2, 4), n_r=8), prec)
+
