@@ -70,7 +70,7 @@ def main():
     ap.add_argument("--n-traj", type=int, default=4096)
     ap.add_argument("--hmin", type=float, default=20.0,
                     help="MIN_TMSTP (a.u.): the step floor of odeint's error control")
-    ap.add_argument("--knots", choices=("fp32", "fp64"), default="fp32",
+    ap.add_argument("--knots", choices=("fp32", "fp64"), default="fp64",
                     help="storage precision of the spline-coefficient matrices of the timed form")
     ap.add_argument("--plain-sample", type=int, default=512)
     ap.add_argument("--xsec-sample", type=int, default=512)
@@ -78,6 +78,8 @@ def main():
                     help="time the steps with per-launch HIP events instead of replaying the captured "
                          "graph (for runs under a kernel tracer)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-other", action="store_true",
+                    help="skip the run with the other coefficient precision")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-xsec", action="store_true")
     ap.add_argument("--no-build", action="store_true")
@@ -183,19 +185,22 @@ def main():
             "%.0f%% of a %.2f s event-timed step" % (la.value, sec * 1e6, tflops, 100 * f_mfma, gbs,
                                                     100 * f_hbm, 100 * ms.value / (pel * 1e3), pel))
 
-    # -- the FP64-coefficient form and its cross sections -----------------------
+    # -- the other coefficient precision and the cross-section difference -----
     other = None
-    if form == 2:
+    if not args.no_other:
+        oform = 1 if form == 2 else 2
         t0 = time.perf_counter()
         if args.no_graph:
             L.check(dev.lib.mqct_hip_batched_profile(dev.h, 1), "batched_profile")
-        q64, _ = prop.run(L.MQCT_PROP_AT_ODEINT, rows, batched=1)
+        qo, _ = prop.run(L.MQCT_PROP_AT_ODEINT, rows, batched=oform)
         oel = time.perf_counter() - t0
-        e, nch = rel_err(xsec_functional(prop, q, l, E), xsec_functional(prop, q64, l, E))
-        other = {"traj_per_s": round(n / oel, 3), "xsec_rel_err_fp32_vs_fp64_coefficients": e, "channels": nch,
-                 "max_amp_diff": float(np.max(np.abs(q[:, :2 * prop.N] - q64[:, :2 * prop.N])))}
-        log("batched AT (fp64 coefficients): %.2f s -> %.1f traj/s; xsec fp32 vs fp64 coefficients %.2e over %d "
-            "channels" % (oel, n / oel, e, nch))
+        q32, q64 = (q, qo) if form == 2 else (qo, q)
+        e, nch = rel_err(xsec_functional(prop, q32, l, E), xsec_functional(prop, q64, l, E))
+        other = {"knots": "fp64" if oform == 1 else "fp32", "traj_per_s": round(n / oel, 3),
+                 "xsec_rel_err_fp32_vs_fp64_coefficients": e, "channels": nch,
+                 "max_amp_diff": float(np.max(np.abs(q32[:, :2 * prop.N] - q64[:, :2 * prop.N])))}
+        log("batched AT (%s coefficients): %.2f s -> %.1f traj/s; xsec fp32 vs fp64 coefficients %.2e over %d "
+            "channels" % (other["knots"], oel, n / oel, e, nch))
     else:
         q64 = q
 
@@ -267,7 +272,7 @@ def main():
                    "knots": int(len(wl.R)), "form": "batched", "graph": not args.no_graph},
         "roofline": roof,
         "cpu_baseline": cpu,
-        "batched_fp64_coefficients": other,
+        "batched_other_coefficients": other,
         "plain_fp64": plain,
         "speedup_vs_plain": round(value / plain["traj_per_s"], 2) if plain else None,
         "matrix_build_ms": build,

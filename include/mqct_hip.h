@@ -206,6 +206,9 @@ int mqct_hip_batched_profile(mqct_hip_t *h, int on);
  * rows per LDS slice, 16-row tiles per wave. */
 int mqct_hip_batched_tile(mqct_hip_t *h, int *entries, int *k_rows,
                           int *row_tiles);
+/* Fixes the interval-GEMM tile shape (index into the candidate list, 0..7)
+ * for the following batched runs, or -1 to choose by timing again. */
+int mqct_hip_batched_set_tile(mqct_hip_t *h, int shape);
 int mqct_hip_batched_kernel_stats(mqct_hip_t *h, int64_t *launches,
                                   double *gemm_ms, double *gemm_flops,
                                   double *gemm_bytes);

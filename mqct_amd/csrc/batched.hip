@@ -37,12 +37,16 @@ namespace {
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 constexpr int BS = 256;
-constexpr int NSLOT = 8;       // y, k1..k6, yscal
+constexpr int NSLOT = 7;       // y, k1..k6
 constexpr int MAXSTP = 10000;  // odeint (suppl_routines.f:1362)
+constexpr int MAXPT = 16;      // states per thread in the step kernel
+static_assert(MAXPT * BS * 16 >= 64 * 1024, "N allowed by the LDS limit must fit MAXPT per thread");
 
 struct Ctl {
   double x, x1, x2, h, hnext, hmin, eps;
   double tau, aa, bb, c1, c2;  // stage being evaluated: time, spline weights
+  double dphidt;               // ... and dPhi/dt of the path there
+  double h0;                   // odeint: h when the step began (error scale)
   double norm_err;
   int stage, nstp, status, zero;  // status: 0 running, 1 done
   int klo, path, nrhs, rk4;       // rk4: the stages are rk4's (AT rk4, or odeint's last step)
@@ -115,9 +119,9 @@ __device__ inline int interval(const double *xa, int stride, int n, double x, in
 }
 
 // Prepare stage c.stage (whole block; c uniform on entry): the stage vector
-// into sys (LDS, 2N), v = exp(-i E tau) y_stage into V, the Coriolis part of
-// k into its slot; thread 0 evaluates the path at tau, finds the knots and
-// spline weights, records them in its c and appends the trajectory to the
+// into sys (LDS, 2N) and v = exp(-i E tau) y_stage into V; thread 0
+// evaluates the path at tau (dPhi/dt for the Coriolis term), finds the knots
+// and spline weights, records them in its c and appends the trajectory to the
 // buckets of parity `par`.
 __device__ void prepare_stage(const BDev &d, int t, Ctl &c, double *sys, double *shm, int par) {
   const int tid = threadIdx.x, N = d.N, n2 = 2 * N;
@@ -167,7 +171,7 @@ __device__ void prepare_stage(const BDev &d, int t, Ctl &c, double *sys, double 
       pv[q] = a * rec[lo * 8 + 1 + col[q]] + b * rec[hi * 8 + 1 + col[q]] +
               (ca * der[lo * 7 + col[q]] + cb * der[hi * 7 + col[q]]) * (hr * hr) / 6.0;
     const double R = pv[0], st = sin(pv[1]);
-    shm[0] = pv[2] / R / R / d.R[-1] / (st * st);  // dPhi/dt
+    c.dphidt = pv[2] / R / R / d.R[-1] / (st * st);
     // knots of R (splint_bk, mqct_eqtns.f:575-607); R > R_n: no coupling
     const int nr = d.n_r;
     c.tau = tau;
@@ -191,27 +195,15 @@ __device__ void prepare_stage(const BDev &d, int t, Ctl &c, double *sys, double 
       bk[(long long)khi * d.n_traj + p1] = make_int2(t, 1);
     }
   }
-  __syncthreads();
-  const double dphidt = shm[0];
+  // the rotated amplitudes for the GEMM (the Coriolis part of k is formed
+  // from them when the stage completes)
   double2 *v = d.V + (long long)t * N;
-  double *kp = S + (1 + s) * n2;
   for (int i = tid; i < N; i += BS) {
     const double cs = cos(tau * d.E[i]), sn = sin(tau * d.E[i]);
     const double xr = sys[i], xi = sys[i + N];
     v[i] = make_double2(xr * cs + xi * sn, xi * cs - xr * sn);
-    double re = 0.0, im = 0.0;
-    const int lo = d.nb_lo[i], hi = d.nb_hi[i];
-    if (lo >= 0) {
-      re += dphidt / 2.0 * d.sq1[i] * sys[lo + N];
-      im -= dphidt / 2.0 * d.sq1[i] * sys[lo];
-    }
-    if (hi >= 0) {
-      re += dphidt / 2.0 * d.sq2[i] * sys[hi + N];
-      im -= dphidt / 2.0 * d.sq2[i] * sys[hi];
-    }
-    kp[i] = re;
-    kp[i + N] = im;
   }
+  (void)shm;
 }
 
 // Interval GEMM.  One workgroup per (knot r, m-block b, row group):
@@ -453,15 +445,50 @@ __global__ __launch_bounds__(BS) void step_kernel(BDev d, int par) {
   double *S = d.S + (long long)t * NSLOT * n2;
   double *y = S;
   const int s = c.stage;
-  double *k = S + (1 + s) * n2;  // k1 .. k6 (rk4 uses k1..k4); holds the Coriolis part
-  if (!c.zero) {
+  double *k = S + (1 + s) * n2;  // k1 .. k6 (rk4 uses k1..k4)
+  {
+    // the stage vector back from its rotated amplitudes (into LDS, for the
+    // Coriolis neighbours) and -i exp(i E tau) (W_lo + W_hi) of this
+    // thread's states (registers)
+    const double2 *vv = d.V + (long long)t * N;
     const double2 *W0 = d.W + ((long long)t * 2) * N, *W1 = W0 + N;
-    for (int i = tid; i < N; i += BS) {
-      const double wx = W0[i].x + W1[i].x, wy = W0[i].y + W1[i].y;
-      const double cs = cos(c.tau * d.E[i]), sn = sin(c.tau * d.E[i]);
-      // -i exp(i E tau) w
-      k[i] = k[i] + (sn * wx + cs * wy);
-      k[i + N] = k[i + N] - (cs * wx - sn * wy);
+    double kr[MAXPT], ki[MAXPT];
+#pragma unroll
+    for (int j = 0; j < MAXPT; j++) {
+      const int i = tid + j * BS;
+      kr[j] = ki[j] = 0.0;
+      if (i < N) {
+        const double cs = cos(c.tau * d.E[i]), sn = sin(c.tau * d.E[i]);
+        const double2 vi = vv[i];
+        sys[i] = vi.x * cs - vi.y * sn;
+        sys[i + N] = vi.y * cs + vi.x * sn;
+        if (!c.zero) {
+          const double wx = W0[i].x + W1[i].x, wy = W0[i].y + W1[i].y;
+          kr[j] = sn * wx + cs * wy;
+          ki[j] = -(cs * wx - sn * wy);
+        }
+      }
+    }
+    __syncthreads();
+    const double hd = c.dphidt / 2.0;
+#pragma unroll
+    for (int j = 0; j < MAXPT; j++) {
+      const int i = tid + j * BS;
+      if (i < N) {
+        // state-diagonal Coriolis coupling (mqct_eqtns.f:845-877)
+        double re = 0.0, im = 0.0;
+        const int lo = d.nb_lo[i], hi = d.nb_hi[i];
+        if (lo >= 0) {
+          re += hd * d.sq1[i] * sys[lo + N];
+          im -= hd * d.sq1[i] * sys[lo];
+        }
+        if (hi >= 0) {
+          re += hd * d.sq2[i] * sys[hi + N];
+          im -= hd * d.sq2[i] * sys[hi];
+        }
+        k[i] = re + kr[j];
+        k[i + N] = im + ki[j];
+      }
     }
   }
   __syncthreads();
@@ -500,9 +527,10 @@ __global__ __launch_bounds__(BS) void step_kernel(BDev d, int par) {
       c.h = copysign(fabs(c.x2 - c.x), c.x2 - c.x1);
       c.stage = 1;
     } else {
-      // odeint: yscal from dydx at the start of the step, then clamp h
-      double *ysc = S + 7 * n2;
-      for (int i = tid; i < n2; i += BS) ysc[i] = fabs(y[i]) + fabs(c.h * k[i]) + TINY;
+      // odeint: yscal = |y| + |h dydx| + TINY with h at the start of the
+      // step (formed again from y and k1 when the error is measured), then
+      // clamp h
+      c.h0 = c.h;
       if ((c.x + c.h - c.x2) * (c.x + c.h - c.x1) > 0.0) c.h = c.x2 - c.x;
       c.stage = 1;
     }
@@ -513,11 +541,11 @@ __global__ __launch_bounds__(BS) void step_kernel(BDev d, int par) {
     const double DC1 = C1 - 2825.0 / 27648.0, DC3 = C3 - 18575.0 / 48384.0,
                  DC4 = C4 - 13525.0 / 55296.0, DC5 = -277.0 / 14336.0, DC6 = C6 - .25;
     const double *k1 = S + n2, *k3 = S + 3 * n2, *k4 = S + 4 * n2, *k5 = S + 5 * n2, *k6 = S + 6 * n2;
-    const double *ysc = S + 7 * n2;
     double em = 0.0;
     for (int i = tid; i < n2; i += BS) {
       const double yerr = c.h * (DC1 * k1[i] + DC3 * k3[i] + DC4 * k4[i] + DC5 * k5[i] + DC6 * k6[i]);
-      em = fmax(em, fabs(yerr / ysc[i]));
+      const double ysc = fabs(y[i]) + fabs(c.h0 * k1[i]) + TINY;
+      em = fmax(em, fabs(yerr / ysc));
     }
     em = block_max(em, red) / c.eps;
     // rkqs (suppl_routines.f:1229-1283): a step at or below hmin is taken as is
@@ -770,7 +798,9 @@ int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *t
   // chosen again: the best tile depends on the bucket fill
   const int pi_ = knots_fp32 ? 1 : 0;
   int shape = ds->gemm_shape[pi_];
-  if (shape >= 0 && (n_traj > 2 * ds->gemm_tuned_n[pi_] || 2 * n_traj < ds->gemm_tuned_n[pi_])) shape = -1;
+  if (shape >= 0 && !ds->gemm_fixed &&
+      (n_traj > 2 * ds->gemm_tuned_n[pi_] || 2 * n_traj < ds->gemm_tuned_n[pi_]))
+    shape = -1;
   auto gemm = [&](int sh, int par) {
     if (knots_fp32) launch_gemm<float>(sh, kb, maxnb, h->stream, d, par);
     else launch_gemm<double>(sh, kb, maxnb, h->stream, d, par);
@@ -901,6 +931,18 @@ extern "C" int mqct_hip_batched_profile(mqct_hip_t *h, int on) {
   ds->prof_ms = 0.0;
   ds->prof_flops = 0.0;
   ds->prof_bytes = 0.0;
+  return MQCT_OK;
+}
+
+extern "C" int mqct_hip_batched_set_tile(mqct_hip_t *h, int shape) {
+  if (!h || !h->dyn) return mqct_fail(MQCT_ERR_STATE, "batched_set_tile: no matrix loaded");
+  if (shape < -1 || shape >= kNShapes) return mqct_fail(MQCT_ERR_ARG, "batched_set_tile: unknown shape");
+  DynState *ds = h->dyn;
+  for (int p = 0; p < 2; p++) {
+    ds->gemm_shape[p] = shape;
+    ds->gemm_tuned_n[p] = shape < 0 ? 0 : (1 << 30);  // a fixed shape is kept for any batch size
+  }
+  ds->gemm_fixed = shape >= 0;
   return MQCT_OK;
 }
 

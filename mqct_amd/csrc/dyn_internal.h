@@ -22,6 +22,7 @@ struct DynState {
   bool km_fp32 = false;   // ... stored in FP32
   int gemm_shape[2] = {-1, -1};  // interval-GEMM tile shape per coefficient precision (FP64, FP32)
   int gemm_tuned_n[2] = {0, 0};  // batch size the shape was chosen on
+  bool gemm_fixed = false;       // shape set by the caller
   // batched-form kernel timing
   bool prof_on = false;
   long long prof_launches = 0;
