@@ -84,8 +84,11 @@ def main():
     ap.add_argument("--no-xsec", action="store_true")
     ap.add_argument("--no-build", action="store_true")
     ap.add_argument("--no-c3", action="store_true",
-                    help="skip the two-asymmetric-tops forms (builds, surrogate, 1024 trajectories)")
-    ap.add_argument("--c3-traj", type=int, default=1024)
+                    help="skip the two-asymmetric-tops forms (builds, surrogate, 256 trajectories)")
+    ap.add_argument("--c3-traj", type=int, default=256)
+    ap.add_argument("--c3-dt", type=float, default=4.0,
+                    help="TIME_STEP (a.u.) of the two-tops rk4 propagation")
+    ap.add_argument("--c3-plain", type=int, default=32, help="two-tops trajectories timed in the plain form")
     ap.add_argument("--sur-samples", type=int, default=4_000_000)
     ap.add_argument("--sur-steps", type=int, default=20000)
     ap.add_argument("--sur-rmin", type=float, default=5.0,
@@ -158,9 +161,17 @@ def main():
     L.check(dev.lib.mqct_hip_batched_kernel_stats(dev.h, ctypes.byref(la), ctypes.byref(ms), ctypes.byref(fl),
                                                   ctypes.byref(by)), "kernel_stats")
     L.check(dev.lib.mqct_hip_batched_profile(dev.h, 0), "batched_profile")
-    ge, gk, grt = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
-    L.check(dev.lib.mqct_hip_batched_tile(dev.h, ctypes.byref(ge), ctypes.byref(gk), ctypes.byref(grt)), "tile")
-    tile = {"entries": ge.value, "k_rows": gk.value, "row_tiles_per_wave": grt.value}
+    FORMS = ("lds", "direct", "stream", "queue", "lds_queue")
+
+    def shape_desc(sh):
+        fo, ge, gk, grt = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        L.check(dev.lib.mqct_hip_batched_tile(dev.h, sh, ctypes.byref(fo), ctypes.byref(ge), ctypes.byref(gk),
+                                              ctypes.byref(grt)), "tile")
+        return "%s e%d k%d r%d" % (FORMS[fo.value], ge.value, gk.value, grt.value)
+    lv = (ctypes.c_int * 24)()
+    L.check(dev.lib.mqct_hip_batched_tile_levels(dev.h, 1 if form == 2 else 0, 24, lv), "tile_levels")
+    tile = {"first": shape_desc(-1),
+            "per_level": {"2^%d" % i: shape_desc(v) for i, v in enumerate(lv) if v >= 0}}
     roof = None
     if la.value > 0 and ms.value > 0:
         sec = ms.value * 1e-3 / la.value
@@ -174,8 +185,9 @@ def main():
             roof = {"bound": "mfma", "achieved": round(tflops, 3), "peak": PEAK_FP64_TFLOPS,
                     "unit": "TFLOP/s", "frac": round(f_mfma, 4)}
         roof.update({"traffic": traffic_from_profiles(args.knots),
-                     "kernel": "gemm_kernel<%s> (interval GEMM, FP64 MFMA 16x16x4)" %
-                               ("float" if form == 2 else "double"),
+                     "kernel": "interval GEMM over %s knot matrices, FP64 MFMA 16x16x4 (gemm_kernel / "
+                               "gemm_direct_kernel / gemm_stream_kernel, tile shape per running-trajectory "
+                               "level)" % ("FP32" if form == 2 else "FP64"),
                      "tile": tile, "launches": la.value, "avg_launch_us": round(sec * 1e6, 2),
                      "flops_per_launch": fl.value / la.value, "bytes_per_launch": by.value / la.value,
                      "fp64_mfma_tflops": round(tflops, 3), "mfma_frac": round(f_mfma, 4),
@@ -248,7 +260,7 @@ def main():
             "%s %s %.1f" % (w, p, build[w][p]["total_ms"]) for w in build for p in build[w]))
     build = {w: {p: {k: round(float(v), 3) for k, v in t.items()} for p, t in d.items()} for w, d in build.items()}
 
-    # -- two asymmetric tops: precision and surrogate forms through 1024 AT
+    # -- two asymmetric tops: precision and surrogate forms through 256 AT
     #    trajectories -------------------------------------------------------
     c3 = None
     if not args.no_c3:
@@ -297,10 +309,13 @@ def two_tops_section(dev, args):
     pairs, 16^5 grid, 128 R): FP64, FP32 and split-BF16 builds with the
     plain PES, and a split-BF16 build on the surrogate PES grid; each matrix
     drives the same batch of AT trajectories (paths from a first step on
-    j1, j2 <= 1; rk4 with TIME_STEP = 10, the reference's default
-    propagator: the strong couplings near the inner wall of this potential
-    need steps below the odeint step floor that the asymmetric-top workload
-    uses), and the cross sections are compared with the FP64 build's."""
+    j1, j2 <= 1), and the cross sections are compared with the FP64
+    build's.  The propagator is rk4 with TIME_STEP = 4 a.u.: the site-site
+    potential couples the states strongly near the inner turning points, so
+    rk4 at 10 a.u. and odeint (whose error scale |y| + |h dydx| lets long
+    steps from the uncoupled region run into the coupled one) lose the norm
+    by orders of magnitude on some trajectories, while rk4 at 4 a.u. keeps
+    it within 4e-4 and its cross sections within 7e-3 of rk4 at 2 a.u."""
     from mqct_amd import _lib as L
     from mqct_amd import workloads as W
     from mqct_amd.matrix import MatrixBuilder
@@ -322,7 +337,7 @@ def two_tops_section(dev, args):
         m = n if sample is None else sample
         rows = []
         for i in range(m):
-            rows += prop.trajectories([l[i]], float(E[i]), 1, time_step=10.0, eps=1e-8, hmin=args.hmin,
+            rows += prop.trajectories([l[i]], float(E[i]), 1, time_step=args.c3_dt, eps=1e-8, hmin=args.hmin,
                                       at_paths=[i])
         t0 = time.perf_counter()
         q, st = prop.run(L.MQCT_PROP_AT_RK4, rows, batched=batched)
@@ -340,7 +355,7 @@ def two_tops_section(dev, args):
         rec = {"build_ms": tm, "batched_traj_per_s": round(tps, 2), "norm_err_max": float(st["norm_err"].max())}
         if prec == "fp64":
             M64, sig64 = M, sig
-            _, qp, _, tpp = propagate(0, sample=min(256, n))
+            _, qp, _, tpp = propagate(0, sample=min(args.c3_plain, n))
             rec["plain_traj_per_s"] = round(tpp, 2)
             rec["plain_max_amp_diff"] = float(np.max(np.abs(qp[:, :2 * prop.N] - q[:len(qp), :2 * prop.N])))
         else:

@@ -200,14 +200,29 @@ int mqct_hip_propagate_timing(mqct_hip_t *h, double *ms);
  * 2*S2 values each, plus 64*N bytes of amplitudes in and products out per
  * right-hand side). */
 int mqct_hip_batched_profile(mqct_hip_t *h, int on);
-/* Tile shape of the interval GEMM that the first batched run (with the
- * coefficient precision of the last run) chose by timing the candidate
- * shapes on its first iterations: bucket entries per column super-tile, K
- * rows per LDS slice, 16-row tiles per wave. */
-int mqct_hip_batched_tile(mqct_hip_t *h, int *entries, int *k_rows,
-                          int *row_tiles);
-/* Fixes the interval-GEMM tile shape (index into the candidate list, 0..7)
- * for the following batched runs, or -1 to choose by timing again. */
+/* The GEMM timing of the profiled runs since the last batched_profile
+ * call, per group of event-timed iterations: rows of (running trajectories
+ * at the start of the group, iterations, summed GEMM ms, shape index);
+ * copies min(cap, n) rows of 4 doubles into rows and returns their number
+ * in n. */
+int mqct_hip_batched_prof_trace(mqct_hip_t *h, int cap, double *rows, int *n);
+/* The interval-GEMM candidates: shape index 0..14 (shape = -1: the shape
+ * the last batched run started with, for the coefficient precision of that
+ * run).  form: 0 LDS-staged per knot, 1 per-wave loads per knot, 2 the same
+ * over the fragment-ordered knot tiles, 3 fragment-ordered work queue of
+ * (knot, tile group, entry chunk) items over persistent workgroups, 4
+ * LDS-staged work queue of (knot, 64-row group, entry chunk) items;
+ * entries: bucket entries per super-tile (per wave for forms 1-3); k_rows:
+ * K rows per LDS slice (0 without LDS); row_tiles: 16-row tiles per wave.
+ * The runner chooses a shape per level floor(log2(running trajectories)) by
+ * timing the candidates on the first iterations at that level. */
+int mqct_hip_batched_tile(mqct_hip_t *h, int shape, int *form, int *entries,
+                          int *k_rows, int *row_tiles);
+/* The chosen shapes (indices into the candidate list, -1 none yet) of
+ * levels 0..n-1 for the FP64 (fp32=0) or FP32 (fp32=1) knot matrices. */
+int mqct_hip_batched_tile_levels(mqct_hip_t *h, int fp32, int n, int *shapes);
+/* Fixes the interval-GEMM tile shape (index into the candidate list,
+ * 0..14) for the following batched runs, or -1 to choose by timing again. */
 int mqct_hip_batched_set_tile(mqct_hip_t *h, int shape);
 int mqct_hip_batched_kernel_stats(mqct_hip_t *h, int64_t *launches,
                                   double *gemm_ms, double *gemm_flops,

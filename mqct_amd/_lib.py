@@ -89,8 +89,10 @@ SIGNATURES = {
                                           ctypes.c_int, c_dbl_p, ctypes.POINTER(TrajStatT)]),
     "mqct_hip_propagate_timing": (ctypes.c_int, [ctypes.c_void_p, c_dbl_p]),
     "mqct_hip_batched_profile": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
-    "mqct_hip_batched_tile": (ctypes.c_int, [ctypes.c_void_p, c_int_p, c_int_p, c_int_p]),
+    "mqct_hip_batched_tile": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, c_int_p, c_int_p, c_int_p, c_int_p]),
     "mqct_hip_batched_set_tile": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "mqct_hip_batched_prof_trace": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, c_dbl_p, c_int_p]),
+    "mqct_hip_batched_tile_levels": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, c_int_p]),
     "mqct_hip_batched_kernel_stats": (ctypes.c_int, [ctypes.c_void_p, c_i64_p, c_dbl_p, c_dbl_p, c_dbl_p]),
     "mqct_hip_propagate_record": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(TrajT),
                                                  ctypes.c_int, c_dbl_p, c_int_p, c_dbl_p,

@@ -37,9 +37,9 @@ namespace {
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 constexpr int BS = 256;
-constexpr int NSLOT = 7;       // y, k1..k6
+constexpr int NSLOT = 6;       // y, k1..k5 (k6 and rk4's k4 stay in registers)
 constexpr int MAXSTP = 10000;  // odeint (suppl_routines.f:1362)
-constexpr int MAXPT = 16;      // states per thread in the step kernel
+constexpr int MAXPT = 16;      // states per thread in the step kernel (N <= 4096)
 static_assert(MAXPT * BS * 16 >= 64 * 1024, "N allowed by the LDS limit must fit MAXPT per thread");
 
 struct Ctl {
@@ -60,6 +60,15 @@ struct BDev {
   long long S2;
   const int *blk_start;       // [nblk+1] positions
   const long long *blk_off;   // [nblk] offset of the block in S2
+  const void *KT;             // [n_r][KS] 16-row tiles of [Y | Y2] in fragment order (double or float)
+  long long KS;               // elements per knot in KT
+  int ntile;                  // 16-row tiles over all blocks
+  const int *tile_blk;        // [ntile] block of the tile
+  const int *tile_r0;         // [ntile] first row of the tile in its block
+  const long long *tile_off;  // [ntile] offset of the tile in a knot's KT
+  int nrg;                    // 64-row groups over all blocks
+  const int *rg_blk;          // [nrg] block of the row group
+  const int *rg_row0;         // [nrg] first row of the group in its block
   const double *E;            // per position, Hartree
   const int *nb_lo, *nb_hi;   // per position: position of state-1 / state+1, -1 none
   const double *sq1, *sq2;    // per position: Coriolis factors
@@ -118,39 +127,32 @@ __device__ inline int interval(const double *xa, int stride, int n, double x, in
   return lo;
 }
 
-// Prepare stage c.stage (whole block; c uniform on entry): the stage vector
-// into sys (LDS, 2N) and v = exp(-i E tau) y_stage into V; thread 0
-// evaluates the path at tau (dPhi/dt for the Coriolis term), finds the knots
-// and spline weights, records them in its c and appends the trajectory to the
-// buckets of parity `par`.
-__device__ void prepare_stage(const BDev &d, int t, Ctl &c, double *sys, double *shm, int par) {
-  const int tid = threadIdx.x, N = d.N, n2 = 2 * N;
-  double *S = d.S + (long long)t * NSLOT * n2;
-  const double *y = S;
+// The time of stage c.stage (Cash-Karp abscissae for odeint; x, x+h/2,
+// x+h/2, x+h for rk4).
+__device__ inline double stage_time(const Ctl &c) {
+  if (!c.rk4) return c.x + CK_A[c.stage] * c.h;
   const int s = c.stage;
-  double tau;
-  if (!c.rk4) {
-    tau = c.x + CK_A[s] * c.h;
-    for (int i = tid; i < n2; i += BS) {
-      double v = y[i];
-      if (s == 1) {
-        v = y[i] + CK_B[1][0] * c.h * S[n2 + i];
-      } else if (s > 1) {
-        double acc = 0.0;
-        for (int q = 0; q < s; q++) acc += CK_B[s][q] * S[(1 + q) * n2 + i];
-        v = y[i] + c.h * acc;
-      }
-      sys[i] = v;
-    }
-  } else {  // rk4: x, x+h/2, x+h/2, x+h
-    const double hh = c.h * 0.5;
-    tau = (s == 0) ? c.x : (s == 3 ? c.x + c.h : c.x + hh);
-    for (int i = tid; i < n2; i += BS) {
-      double v = y[i];
-      if (s == 1) v = y[i] + hh * S[n2 + i];
-      else if (s == 2) v = y[i] + hh * S[2 * n2 + i];
-      else if (s == 3) v = y[i] + c.h * S[3 * n2 + i];
-      sys[i] = v;
+  return (s == 0) ? c.x : (s == 3 ? c.x + c.h : c.x + 0.5 * c.h);
+}
+
+// Prepares stage c.stage from its vector x (the thread's states in
+// registers: position i = tid + j BS, Re at i, Im at i + N): writes
+// v = exp(-i E tau) x to V; thread 0 evaluates the recorded path at tau
+// (dPhi/dt for the Coriolis term), finds the knots and spline weights,
+// records them in its c and appends the trajectory to the buckets of
+// parity `par`.
+template <int PT>
+__device__ void prepare_stage(const BDev &d, int t, Ctl &c, const double *xr, const double *xi, int par) {
+  const int tid = threadIdx.x, N = d.N;
+  const double tau = stage_time(c);
+  double2 *v = d.V + (long long)t * N;
+#pragma unroll
+  for (int j = 0; j < PT; j++) {
+    const int i = tid + j * BS;
+    if (i < N) {
+      double sn, cs;
+      sincos(tau * d.E[i], &sn, &cs);
+      v[i] = make_double2(xr[j] * cs + xi[j] * sn, xi[j] * cs - xr[j] * sn);
     }
   }
   if (tid == 0) {
@@ -195,15 +197,6 @@ __device__ void prepare_stage(const BDev &d, int t, Ctl &c, double *sys, double 
       bk[(long long)khi * d.n_traj + p1] = make_int2(t, 1);
     }
   }
-  // the rotated amplitudes for the GEMM (the Coriolis part of k is formed
-  // from them when the stage completes)
-  double2 *v = d.V + (long long)t * N;
-  for (int i = tid; i < N; i += BS) {
-    const double cs = cos(tau * d.E[i]), sn = sin(tau * d.E[i]);
-    const double xr = sys[i], xi = sys[i + N];
-    v[i] = make_double2(xr * cs + xi * sn, xi * cs - xr * sn);
-  }
-  (void)shm;
 }
 
 // Interval GEMM.  One workgroup per (knot r, m-block b, row group):
@@ -217,26 +210,25 @@ __device__ void prepare_stage(const BDev &d, int t, Ctl &c, double *sys, double 
 // Block (0,0,0) also clears the other parity's bucket counts for the step
 // kernel that follows.  The tile shape is a template parameter; the runner
 // times the shapes on the first iterations and keeps the fastest.
+// Shared memory of one LDS-staged super-tile.
+template <int GE, int GK>
+struct LdsTile {
+  double Bs[GK][2 * GE + 16];  // row stride 2 GE + 16 doubles: half-waves hit disjoint banks
+  int t[GE], side[GE];
+  double wy[GE], wy2[GE];
+};
+
+// One super-tile of the LDS-staged product (whole workgroup): rows
+// row0 .. row0 + 64 GRT of block b at knot r times bucket entries
+// e0 .. e0 + nE (nE <= GE) of parity `par`.
 template <typename KT, int GE, int GK, int GRT>
-__global__ __launch_bounds__(BS) void gemm_kernel(BDev d, int par) {
-  constexpr int GC = 2 * GE / 16;       // MFMA column tiles
-  constexpr int GLD = 2 * GE + 16;      // LDS row stride in doubles (half-waves hit disjoint banks)
-  constexpr int GROWS = 4 * GRT * 16;   // rows per workgroup
-  const int r = blockIdx.x, b = blockIdx.y;
-  if (r == 0 && b == 0 && blockIdx.z == 0)
-    for (int i = threadIdx.x; i < d.n_r; i += BS) d.cnt[(par ^ 1) * d.n_r + i] = 0;
-  const int ne = d.cnt[par * d.n_r + r];
-  if (ne == 0) return;
+__device__ void lds_super_tile(const BDev &d, int par, int r, int b, int row0, int e0, int nE,
+                               LdsTile<GE, GK> &sh) {
+  constexpr int GC = 2 * GE / 16;  // MFMA column tiles
   const int p0 = d.blk_start[b], nb = d.blk_start[b + 1] - p0;
-  const int row0 = blockIdx.z * GROWS;
-  if (row0 >= nb) return;
-  if (b == 0 && blockIdx.z == 0 && threadIdx.x == 0) atomicAdd(d.ntouched, 1ULL);
   const int K = 2 * nb;
   const KT *Y = static_cast<const KT *>(d.KM) + ((long long)r * 2) * d.S2 + d.blk_off[b];
   const KT *Y2 = static_cast<const KT *>(d.KM) + ((long long)r * 2 + 1) * d.S2 + d.blk_off[b];
-  __shared__ double Bs[GK][GLD];
-  __shared__ int s_t[GE], s_side[GE];
-  __shared__ double s_wy[GE], s_wy2[GE];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int kq = lane >> 4;
   int rr[GRT];
@@ -248,91 +240,104 @@ __global__ __launch_bounds__(BS) void gemm_kernel(BDev d, int par) {
     rr[q] = rt0 + (lane & 15);
   }
   const int2 *bk = d.bucket + ((long long)par * d.n_r + r) * d.n_traj;
-  for (int e0 = 0; e0 < ne; e0 += GE) {
-    const int nE = min(GE, ne - e0);
-    const int nct = (2 * nE + 15) / 16;
-    const int nEpad = nct * 8;  // entries staged (zero columns up to the tile edge)
-    __syncthreads();
-    if (tid < GE) {
-      int t = 0, side = 0;
-      double wy = 0.0, wy2 = 0.0;
-      if (tid < nE) {
-        const int2 ent = bk[e0 + tid];
-        const Ctl &c = d.ctl[ent.x];
-        t = ent.x;
-        side = ent.y;
-        wy = side ? c.bb : c.aa;
-        wy2 = side ? c.c2 : c.c1;
-      }
-      s_t[tid] = t;
-      s_side[tid] = side;
-      s_wy[tid] = wy;
-      s_wy2[tid] = wy2;
+  const int nct = (2 * nE + 15) / 16;
+  const int nEpad = nct * 8;  // entries staged (zero columns up to the tile edge)
+  __syncthreads();
+  if (tid < GE) {
+    int t = 0, side = 0;
+    double wy = 0.0, wy2 = 0.0;
+    if (tid < nE) {
+      const int2 ent = bk[e0 + tid];
+      const Ctl &c = d.ctl[ent.x];
+      t = ent.x;
+      side = ent.y;
+      wy = side ? c.bb : c.aa;
+      wy2 = side ? c.c2 : c.c1;
     }
-    dbl4 acc[GRT][GC];
+    sh.t[tid] = t;
+    sh.side[tid] = side;
+    sh.wy[tid] = wy;
+    sh.wy2[tid] = wy2;
+  }
+  dbl4 acc[GRT][GC];
+#pragma unroll
+  for (int q = 0; q < GRT; q++)
+#pragma unroll
+    for (int ct = 0; ct < GC; ct++) acc[q][ct] = dbl4{0.0, 0.0, 0.0, 0.0};
+  for (int k0 = 0; k0 < K; k0 += GK) {
+    __syncthreads();
+    // kk fastest: a group of threads reads 16 consecutive amplitudes
+    for (int idx = tid; idx < GK * nEpad; idx += BS) {
+      const int kk = idx % GK, e = idx / GK, k = k0 + kk;
+      double vx = 0.0, vy = 0.0;
+      if (e < nE && k < K) {
+        const bool lo = k < nb;
+        const double w = lo ? sh.wy[e] : sh.wy2[e];
+        const double2 x = d.V[(long long)sh.t[e] * d.N + p0 + (lo ? k : k - nb)];
+        vx = w * x.x;
+        vy = w * x.y;
+      }
+      sh.Bs[kk][2 * e] = vx;
+      sh.Bs[kk][2 * e + 1] = vy;
+    }
+    double af[GRT][GK / 4];
 #pragma unroll
     for (int q = 0; q < GRT; q++)
 #pragma unroll
-      for (int ct = 0; ct < GC; ct++) acc[q][ct] = dbl4{0.0, 0.0, 0.0, 0.0};
-    for (int k0 = 0; k0 < K; k0 += GK) {
-      __syncthreads();
-      // kk fastest: a group of threads reads 16 consecutive amplitudes
-      for (int idx = tid; idx < GK * nEpad; idx += BS) {
-        const int kk = idx % GK, e = idx / GK, k = k0 + kk;
-        double vx = 0.0, vy = 0.0;
-        if (e < nE && k < K) {
-          const bool lo = k < nb;
-          const double w = lo ? s_wy[e] : s_wy2[e];
-          const double2 x = d.V[(long long)s_t[e] * d.N + p0 + (lo ? k : k - nb)];
-          vx = w * x.x;
-          vy = w * x.y;
-        }
-        Bs[kk][2 * e] = vx;
-        Bs[kk][2 * e + 1] = vy;
-      }
-      double af[GRT][GK / 4];
-#pragma unroll
-      for (int q = 0; q < GRT; q++)
-#pragma unroll
-        for (int s4 = 0; s4 < GK / 4; s4++) {
-          const int k = k0 + 4 * s4 + kq;
-          double a = 0.0;
-          if (rr[q] < nb && k < K)
-            a = (k < nb) ? (double)Y[(long long)k * nb + rr[q]] : (double)Y2[(long long)(k - nb) * nb + rr[q]];
-          af[q][s4] = a;
-        }
-      __syncthreads();
-#pragma unroll
       for (int s4 = 0; s4 < GK / 4; s4++) {
-        const double *brow = &Bs[4 * s4 + kq][lane & 15];
-#pragma unroll
-        for (int q = 0; q < GRT; q++) {
-          if (!tile_on[q]) continue;
-#pragma unroll
-          for (int ct = 0; ct < GC; ct++)
-            if (ct < nct) acc[q][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[q][s4], brow[16 * ct], acc[q][ct], 0, 0, 0);
-        }
+        const int k = k0 + 4 * s4 + kq;
+        double a = 0.0;
+        if (rr[q] < nb && k < K)
+          a = (k < nb) ? (double)Y[(long long)k * nb + rr[q]] : (double)Y2[(long long)(k - nb) * nb + rr[q]];
+        af[q][s4] = a;
       }
-    }
-    // f64 16x16x4 output: row lane/16 + 4i, column lane%16 of each tile
+    __syncthreads();
 #pragma unroll
-    for (int q = 0; q < GRT; q++) {
-      if (!tile_on[q]) continue;
-      const int rt0 = row0 + (wave * GRT + q) * 16;
+    for (int s4 = 0; s4 < GK / 4; s4++) {
+      const double *brow = &sh.Bs[4 * s4 + kq][lane & 15];
 #pragma unroll
-      for (int ct = 0; ct < GC; ct++) {
-        if (ct >= nct) break;
-        const int col = 16 * ct + (lane & 15), e = col >> 1;
-        if (e >= nE) continue;
-        double *wb = reinterpret_cast<double *>(d.W + ((long long)s_t[e] * 2 + s_side[e]) * d.N + p0) + (col & 1);
+      for (int q = 0; q < GRT; q++) {
+        if (!tile_on[q]) continue;
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-          const int orow = rt0 + (lane >> 4) + 4 * i;
-          if (orow < nb) wb[2 * orow] = acc[q][ct][i];
-        }
+        for (int ct = 0; ct < GC; ct++)
+          if (ct < nct) acc[q][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[q][s4], brow[16 * ct], acc[q][ct], 0, 0, 0);
       }
     }
   }
+  // f64 16x16x4 output: row lane/16 + 4i, column lane%16 of each tile
+#pragma unroll
+  for (int q = 0; q < GRT; q++) {
+    if (!tile_on[q]) continue;
+    const int rt0 = row0 + (wave * GRT + q) * 16;
+#pragma unroll
+    for (int ct = 0; ct < GC; ct++) {
+      if (ct >= nct) break;
+      const int col = 16 * ct + (lane & 15), e = col >> 1;
+      if (e >= nE) continue;
+      double *wb = reinterpret_cast<double *>(d.W + ((long long)sh.t[e] * 2 + sh.side[e]) * d.N + p0) + (col & 1);
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int orow = rt0 + (lane >> 4) + 4 * i;
+        if (orow < nb) wb[2 * orow] = acc[q][ct][i];
+      }
+    }
+  }
+}
+
+template <typename KT, int GE, int GK, int GRT>
+__global__ __launch_bounds__(BS) void gemm_kernel(BDev d, int par) {
+  constexpr int GROWS = 4 * GRT * 16;  // rows per workgroup
+  const int r = blockIdx.x, b = blockIdx.y;
+  if (r == 0 && b == 0 && blockIdx.z == 0)
+    for (int i = threadIdx.x; i < d.n_r; i += BS) d.cnt[(par ^ 1) * d.n_r + i] = 0;
+  const int ne = d.cnt[par * d.n_r + r];
+  if (ne == 0) return;
+  const int nb = d.blk_start[b + 1] - d.blk_start[b];
+  const int row0 = blockIdx.z * GROWS;
+  if (row0 >= nb) return;
+  if (b == 0 && blockIdx.z == 0 && threadIdx.x == 0) atomicAdd(d.ntouched, 1ULL);
+  __shared__ LdsTile<GE, GK> sh;
+  for (int e0 = 0; e0 < ne; e0 += GE) lds_super_tile<KT, GE, GK, GRT>(d, par, r, b, row0, e0, min(GE, ne - e0), sh);
 }
 
 // The same product without LDS or barriers: every wave owns one 16-row
@@ -405,18 +410,297 @@ __global__ __launch_bounds__(BS) void gemm_direct_kernel(BDev d, int par) {
   }
 }
 
-// The tile shapes the runner chooses from: (entries per super-tile, K rows
-// per slice, row tiles per wave).
-struct GemmShape {
-  int ge, gk, grt;
+// 16-byte fragment loads of the tile layout: U consecutive MFMA steps of
+// one lane
+template <typename KT>
+struct Frag;
+template <>
+struct Frag<double> {
+  using V = double2;
+  static constexpr int U = 2;
+  __device__ static double get(const V &v, int u) { return u == 0 ? v.x : v.y; }
 };
-// (entries 0 in k_rows: the barrier-free form with `entries` per wave)
-constexpr GemmShape kShapes[] = {{64, 16, 1}, {32, 16, 1}, {32, 32, 1}, {16, 32, 1},
-                                 {64, 16, 2}, {32, 0, 1},  {16, 0, 1},  {8, 0, 1}};
+template <>
+struct Frag<float> {
+  using V = float4;
+  static constexpr int U = 4;
+  __device__ static double get(const V &v, int u) {
+    return (double)(u == 0 ? v.x : (u == 1 ? v.y : (u == 2 ? v.z : v.w)));
+  }
+};
+
+// The same product over the fragment-ordered tiles: each wave owns one
+// 16-row tile of one block at knot r and groups of 8 NCT bucket entries.  A
+// lane's 16-byte load holds its matrix fragments for U consecutive MFMA steps
+// (U = 2 doubles or 4 floats), so one load instruction moves 1 KB per wave
+// with no LDS and no barriers.  Inside a load the contraction order is
+// permuted (lane group q holds k = 4U s + U q + u for step u) and the
+// weighted amplitudes follow the same order.  Tile rows past the block and
+// columns past nb are zero; amplitude reads clamp k to the block.
+template <typename KT, int NCT>
+__global__ __launch_bounds__(BS) void gemm_stream_kernel(BDev d, int par) {
+  using F = Frag<KT>;
+  using FV = typename F::V;
+  constexpr int U = F::U, KB = 4 * U;
+  const int r = blockIdx.x;
+  if (r == 0 && blockIdx.y == 0)
+    for (int i = threadIdx.x; i < d.n_r; i += BS) d.cnt[(par ^ 1) * d.n_r + i] = 0;
+  const int ne = d.cnt[par * d.n_r + r];
+  if (ne == 0) return;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int g = blockIdx.y * 4 + wave;
+  if (g >= d.ntile) return;
+  if (g == 0 && lane == 0) atomicAdd(d.ntouched, 1ULL);
+  const int b = d.tile_blk[g], rt0 = d.tile_r0[g];
+  const int p0 = d.blk_start[b], nb = d.blk_start[b + 1] - p0;
+  const int ns = (nb + KB - 1) / KB;  // loads per half ([Y | Y2])
+  const int q = lane >> 4, col = lane & 15, comp = col & 1;
+  const FV *A = reinterpret_cast<const FV *>(static_cast<const KT *>(d.KT) + (long long)r * d.KS + d.tile_off[g]) +
+                lane;
+  const int2 *bk = d.bucket + ((long long)par * d.n_r + r) * d.n_traj;
+  for (int e0 = 0; e0 < ne; e0 += 8 * NCT) {
+    const double *vp[NCT];
+    double wy[NCT], wy2[NCT];
+    int tt[NCT], sd[NCT];
+    bool ok[NCT];
+#pragma unroll
+    for (int ct = 0; ct < NCT; ct++) {
+      const int e = e0 + ct * 8 + (col >> 1);
+      ok[ct] = e < ne;
+      const int2 ent = ok[ct] ? bk[e] : make_int2(0, 0);
+      const Ctl &c = d.ctl[ent.x];
+      tt[ct] = ent.x;
+      sd[ct] = ent.y;
+      wy[ct] = ok[ct] ? (ent.y ? c.bb : c.aa) : 0.0;
+      wy2[ct] = ok[ct] ? (ent.y ? c.c2 : c.c1) : 0.0;
+      vp[ct] = reinterpret_cast<const double *>(d.V + (long long)ent.x * d.N + p0) + comp;
+    }
+    dbl4 acc[NCT];
+#pragma unroll
+    for (int ct = 0; ct < NCT; ct++) acc[ct] = dbl4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int hf = 0; hf < 2; hf++) {
+      double w[NCT];
+#pragma unroll
+      for (int ct = 0; ct < NCT; ct++) w[ct] = hf ? wy2[ct] : wy[ct];
+      const FV *Ah = A + hf * ns * 64;
+#pragma unroll 2
+      for (int s = 0; s < ns; s++) {
+        const FV av = Ah[s * 64];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+          const int kc = min(KB * s + U * q + u, nb - 1);
+          const double a = F::get(av, u);
+#pragma unroll
+          for (int ct = 0; ct < NCT; ct++)
+            acc[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, w[ct] * vp[ct][2 * kc], acc[ct], 0, 0, 0);
+        }
+      }
+    }
+#pragma unroll
+    for (int ct = 0; ct < NCT; ct++) {
+      if (!ok[ct]) continue;
+      double *wb = reinterpret_cast<double *>(d.W + ((long long)tt[ct] * 2 + sd[ct]) * d.N + p0) + comp;
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int orow = rt0 + (lane >> 4) + 4 * i;
+        if (orow < nb) wb[2 * orow] = acc[ct][i];
+      }
+    }
+  }
+}
+
+// The fragment-ordered product as a work queue.  Trajectories bunch in R
+// (all start at the same R), so a few knots can hold thousands of bucket
+// entries while most hold none; here the entries of every knot are cut into
+// chunks of 8 NCT and the work items (knot, group of four 16-row tiles,
+// chunk), in that order, are shared out over a fixed grid of persistent
+// workgroups.  Each XCD (workgroup index mod 8) takes a contiguous range of
+// items, so the chunks of one tile group meet in the same L2.  Every
+// workgroup forms the chunk prefix over the knots from the bucket counts in
+// LDS (n_r + 1 ints).
+constexpr int QGRID = 2048;  // persistent workgroups (a multiple of 8)
+
+// The work-queue preamble (whole workgroup): workgroup 0 clears the other
+// parity's bucket counts and counts the touched knots; every workgroup forms
+// in pre[0..n_r] the exclusive prefix over knots of the number of chunks of
+// CH bucket entries (a contiguous run of knots per thread, then a block scan
+// of the run totals).  Returns the total number of chunks.
+__device__ int chunk_prefix(const BDev &d, int par, int CH, int *pre, int *wtot) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int nr = d.n_r;
+  const int *cnt = d.cnt + par * nr;
+  if (blockIdx.x == 0)
+    for (int i = tid; i < nr; i += BS) d.cnt[(par ^ 1) * nr + i] = 0;
+  const int per = (nr + BS - 1) / BS, ra = min(nr, tid * per), rb = min(nr, ra + per);
+  int own = 0, busy = 0;
+  for (int r = ra; r < rb; r++) {
+    const int c = cnt[r];
+    own += (c + CH - 1) / CH;
+    busy += c > 0;
+  }
+  int inc = own;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += y;
+  }
+  if (lane == 63) wtot[wave] = inc;
+  __syncthreads();
+  int run = inc - own;
+  for (int w = 0; w < wave; w++) run += wtot[w];
+  for (int r = ra; r < rb; r++) {
+    pre[r] = run;
+    run += (cnt[r] + CH - 1) / CH;
+  }
+  if (tid == BS - 1) pre[nr] = run;
+  if (blockIdx.x == 0 && busy) atomicAdd(d.ntouched, (unsigned long long)busy);
+  __syncthreads();
+  return pre[nr];
+}
+
+// The largest knot r with pre[r] <= q (pre non-decreasing, pre[0] = 0 <= q
+// < pre[n_r]): the knot whose chunks hold chunk index q.
+__device__ inline int chunk_knot(const int *pre, int nr, int q) {
+  int a = 0, b = nr - 1;
+  while (a < b) {
+    const int m = (a + b + 1) >> 1;
+    if (pre[m] <= q) a = m;
+    else b = m - 1;
+  }
+  return a;
+}
+
+template <typename KT, int NCT>
+__global__ __launch_bounds__(BS) void gemm_queue_kernel(BDev d, int par) {
+  using F = Frag<KT>;
+  using FV = typename F::V;
+  constexpr int U = F::U, KB = 4 * U, CH = 8 * NCT;
+  extern __shared__ int pre[];  // [n_r + 1] chunk prefix
+  __shared__ int wtot[BS / 64];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int nr = d.n_r, TG = (d.ntile + 3) / 4;
+  const int *cnt = d.cnt + par * nr;
+  chunk_prefix(d, par, CH, pre, wtot);
+  const long long M = (long long)pre[nr] * TG;
+  const int X = 8, xcd = blockIdx.x % X, slot = blockIdx.x / X, Gx = gridDim.x / X;
+  const long long lo = M * xcd / X, hi = M * (xcd + 1) / X;
+  const int q = lane >> 4, col = lane & 15, comp = col & 1;
+  for (long long it = lo + slot; it < hi; it += Gx) {
+    // item -> (knot r, tile group tg, chunk c): pre[r] <= it / TG < pre[r+1]
+    const int r = chunk_knot(pre, nr, (int)(it / TG)), cr = pre[r + 1] - pre[r];
+    const int loc = (int)(it - (long long)pre[r] * TG), tg = loc / cr, ch = loc % cr;
+    const int g = tg * 4 + wave;
+    if (g >= d.ntile) continue;
+    const int ne = cnt[r], e0 = ch * CH;
+    const int bl = d.tile_blk[g], rt0 = d.tile_r0[g];
+    const int p0 = d.blk_start[bl], nb = d.blk_start[bl + 1] - p0;
+    const int ns = (nb + KB - 1) / KB;
+    const FV *A =
+        reinterpret_cast<const FV *>(static_cast<const KT *>(d.KT) + (long long)r * d.KS + d.tile_off[g]) + lane;
+    const int2 *bk = d.bucket + ((long long)par * nr + r) * d.n_traj;
+    const double *vp[NCT];
+    double wy[NCT], wy2[NCT];
+    int tt[NCT], sd[NCT];
+    bool ok[NCT];
+#pragma unroll
+    for (int ct = 0; ct < NCT; ct++) {
+      const int e = e0 + ct * 8 + (col >> 1);
+      ok[ct] = e < ne;
+      const int2 ent = ok[ct] ? bk[e] : make_int2(0, 0);
+      const Ctl &c = d.ctl[ent.x];
+      tt[ct] = ent.x;
+      sd[ct] = ent.y;
+      wy[ct] = ok[ct] ? (ent.y ? c.bb : c.aa) : 0.0;
+      wy2[ct] = ok[ct] ? (ent.y ? c.c2 : c.c1) : 0.0;
+      vp[ct] = reinterpret_cast<const double *>(d.V + (long long)ent.x * d.N + p0) + comp;
+    }
+    dbl4 acc[NCT];
+#pragma unroll
+    for (int ct = 0; ct < NCT; ct++) acc[ct] = dbl4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int hf = 0; hf < 2; hf++) {
+      double w[NCT];
+#pragma unroll
+      for (int ct = 0; ct < NCT; ct++) w[ct] = hf ? wy2[ct] : wy[ct];
+      const FV *Ah = A + hf * ns * 64;
+#pragma unroll 2
+      for (int s = 0; s < ns; s++) {
+        const FV av = Ah[s * 64];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+          const int kc = min(KB * s + U * q + u, nb - 1);
+          const double av_u = F::get(av, u);
+#pragma unroll
+          for (int ct = 0; ct < NCT; ct++)
+            acc[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(av_u, w[ct] * vp[ct][2 * kc], acc[ct], 0, 0, 0);
+        }
+      }
+    }
+#pragma unroll
+    for (int ct = 0; ct < NCT; ct++) {
+      if (!ok[ct]) continue;
+      double *wb = reinterpret_cast<double *>(d.W + ((long long)tt[ct] * 2 + sd[ct]) * d.N + p0) + comp;
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int orow = rt0 + (lane >> 4) + 4 * i;
+        if (orow < nb) wb[2 * orow] = acc[ct][i];
+      }
+    }
+  }
+}
+
+// The LDS-staged product as a work queue: items (knot, group of 64 rows of
+// one block, chunk of GE bucket entries), in that order, over persistent
+// workgroups, each item one super-tile of the whole workgroup (the four
+// waves share the staged amplitudes).
+template <typename KT, int GE, int GK>
+__global__ __launch_bounds__(BS) void gemm_lds_queue_kernel(BDev d, int par) {
+  extern __shared__ int pre[];  // [n_r + 1] chunk prefix
+  __shared__ int wtot[BS / 64];
+  __shared__ LdsTile<GE, GK> sh;
+  const int nr = d.n_r, RG = d.nrg;
+  const int *cnt = d.cnt + par * nr;
+  const long long M = (long long)chunk_prefix(d, par, GE, pre, wtot) * RG;
+  const int X = 8, xcd = blockIdx.x % X, slot = blockIdx.x / X, Gx = gridDim.x / X;
+  const long long lo = M * xcd / X, hi = M * (xcd + 1) / X;
+  for (long long it = lo + slot; it < hi; it += Gx) {
+    const int r = chunk_knot(pre, nr, (int)(it / RG)), cr = pre[r + 1] - pre[r];
+    const int loc = (int)(it - (long long)pre[r] * RG), rg = loc / cr, ch = loc % cr;
+    const int e0 = ch * GE;
+    lds_super_tile<KT, GE, GK, 1>(d, par, r, d.rg_blk[rg], d.rg_row0[rg], e0, min(GE, cnt[r] - e0), sh);
+  }
+}
+
+// The GEMM forms and tile shapes the runner chooses from: form, bucket
+// entries per super-tile (per wave for the LDS-free forms), K rows per LDS
+// slice, 16-row tiles per wave.
+enum GemmForm { kLds = 0, kDirect = 1, kStream = 2, kQueue = 3, kLdsQueue = 4 };
+struct GemmShape {
+  int form, ge, gk, grt;
+};
+constexpr GemmShape kShapes[] = {
+    {kLds, 64, 16, 1},      {kLds, 32, 16, 1},    {kLds, 32, 32, 1},    {kLds, 16, 32, 1},
+    {kLds, 64, 16, 2},      {kDirect, 32, 0, 1},  {kDirect, 16, 0, 1},  {kDirect, 8, 0, 1},
+    {kStream, 8, 0, 1},     {kStream, 16, 0, 1},  {kStream, 32, 0, 1},  {kQueue, 16, 0, 1},
+    {kQueue, 32, 0, 1},     {kLdsQueue, 64, 16, 1}, {kLdsQueue, 32, 16, 1}};
 constexpr int kNShapes = sizeof(kShapes) / sizeof(kShapes[0]);
+// the work-queue forms keep the chunk prefix of all knots in LDS
+constexpr bool uses_queue(int shape) { return kShapes[shape].form >= kQueue; }
 
 template <typename KT>
 void launch_gemm(int shape, dim3 knots_blocks, int maxnb, hipStream_t st, const BDev &d, int par) {
+  const size_t shq = (size_t)(d.n_r + 1) * 4;
+  const dim3 gs(knots_blocks.x, (d.ntile + 3) / 4);
+  switch (shape) {
+    case 8: gemm_stream_kernel<KT, 1><<<gs, BS, 0, st>>>(d, par); return;
+    case 9: gemm_stream_kernel<KT, 2><<<gs, BS, 0, st>>>(d, par); return;
+    case 10: gemm_stream_kernel<KT, 4><<<gs, BS, 0, st>>>(d, par); return;
+    case 11: gemm_queue_kernel<KT, 2><<<QGRID, BS, shq, st>>>(d, par); return;
+    case 12: gemm_queue_kernel<KT, 4><<<QGRID, BS, shq, st>>>(d, par); return;
+    case 13: gemm_lds_queue_kernel<KT, 64, 16><<<QGRID, BS, shq, st>>>(d, par); return;
+    case 14: gemm_lds_queue_kernel<KT, 32, 16><<<QGRID, BS, shq, st>>>(d, par); return;
+    default: break;
+  }
   auto go = [&](auto kern, int grt) {
     const int rows = 4 * grt * 16;
     dim3 g(knots_blocks.x, knots_blocks.y, (maxnb + rows - 1) / rows);
@@ -430,35 +714,39 @@ void launch_gemm(int shape, dim3 knots_blocks, int maxnb, hipStream_t st, const 
     case 4: go(gemm_kernel<KT, 64, 16, 2>, 2); break;
     case 5: go(gemm_direct_kernel<KT, 4>, 1); break;
     case 6: go(gemm_direct_kernel<KT, 2>, 1); break;
-    default: go(gemm_direct_kernel<KT, 1>, 1); break;
+    case 7: go(gemm_direct_kernel<KT, 1>, 1); break;
   }
 }
 
 // Completes the stage the GEMM evaluated, advances the integrator and
-// prepares the next stage (into the buckets of parity `par`).
+// prepares the next stage (into the buckets of parity `par`).  Every thread
+// keeps the states i = tid + j BS (j < PT) in registers through the three
+// phases -- the right-hand side k of the completed stage, the integrator
+// update, the next stage vector -- so k is stored only when a later stage
+// reads it, and a finished step's y feeds the next stage without a reload.
+template <int PT>
 __global__ __launch_bounds__(BS) void step_kernel(BDev d, int par) {
-  extern __shared__ double sys[];  // 2N: stage vector of the next stage
-  __shared__ double red[8], shm[4];
+  extern __shared__ double sys[];  // 2N: the completed stage's vector (Coriolis neighbours)
+  __shared__ double red[8];
   const int t = blockIdx.x, tid = threadIdx.x, N = d.N, n2 = 2 * N;
   Ctl c = d.ctl[t];
   if (c.status != 0) return;
   double *S = d.S + (long long)t * NSLOT * n2;
   double *y = S;
   const int s = c.stage;
-  double *k = S + (1 + s) * n2;  // k1 .. k6 (rk4 uses k1..k4)
+  // -- k = -i exp(i E tau) (W_lo + W_hi) + Coriolis(stage vector), the stage
+  //    vector formed again from its rotated amplitudes
+  double kr[PT], ki[PT];
   {
-    // the stage vector back from its rotated amplitudes (into LDS, for the
-    // Coriolis neighbours) and -i exp(i E tau) (W_lo + W_hi) of this
-    // thread's states (registers)
     const double2 *vv = d.V + (long long)t * N;
     const double2 *W0 = d.W + ((long long)t * 2) * N, *W1 = W0 + N;
-    double kr[MAXPT], ki[MAXPT];
 #pragma unroll
-    for (int j = 0; j < MAXPT; j++) {
+    for (int j = 0; j < PT; j++) {
       const int i = tid + j * BS;
       kr[j] = ki[j] = 0.0;
       if (i < N) {
-        const double cs = cos(c.tau * d.E[i]), sn = sin(c.tau * d.E[i]);
+        double sn, cs;
+        sincos(c.tau * d.E[i], &sn, &cs);
         const double2 vi = vv[i];
         sys[i] = vi.x * cs - vi.y * sn;
         sys[i + N] = vi.y * cs + vi.x * sn;
@@ -472,35 +760,68 @@ __global__ __launch_bounds__(BS) void step_kernel(BDev d, int par) {
     __syncthreads();
     const double hd = c.dphidt / 2.0;
 #pragma unroll
-    for (int j = 0; j < MAXPT; j++) {
+    for (int j = 0; j < PT; j++) {
       const int i = tid + j * BS;
       if (i < N) {
         // state-diagonal Coriolis coupling (mqct_eqtns.f:845-877)
-        double re = 0.0, im = 0.0;
         const int lo = d.nb_lo[i], hi = d.nb_hi[i];
         if (lo >= 0) {
-          re += hd * d.sq1[i] * sys[lo + N];
-          im -= hd * d.sq1[i] * sys[lo];
+          kr[j] += hd * d.sq1[i] * sys[lo + N];
+          ki[j] -= hd * d.sq1[i] * sys[lo];
         }
         if (hi >= 0) {
-          re += hd * d.sq2[i] * sys[hi + N];
-          im -= hd * d.sq2[i] * sys[hi];
+          kr[j] += hd * d.sq2[i] * sys[hi + N];
+          ki[j] -= hd * d.sq2[i] * sys[hi];
         }
-        k[i] = re + kr[j];
-        k[i + N] = im + ki[j];
       }
     }
   }
-  __syncthreads();
   c.nrhs++;
   const double TINY = 1.e-30;
+  double xr[PT], xi[PT];  // the next stage's vector
+  // k into its slot (k1 .. k6 at S + (1 + s) n2) for the stages that read it
+  auto store_k = [&]() {
+    double *k = S + (1 + s) * n2;
+#pragma unroll
+    for (int j = 0; j < PT; j++) {
+      const int i = tid + j * BS;
+      if (i < N) {
+        k[i] = kr[j];
+        k[i + N] = ki[j];
+      }
+    }
+  };
+  // x = y + f k (one earlier slot at most: rk4 stages, odeint stage 1)
+  auto stage_from_k = [&](double f) {
+#pragma unroll
+    for (int j = 0; j < PT; j++) {
+      const int i = tid + j * BS;
+      if (i < N) {
+        xr[j] = y[i] + f * kr[j];
+        xi[j] = y[i + N] + f * ki[j];
+      }
+    }
+  };
   if (c.rk4) {
     if (s < 3) {
+      store_k();
       c.stage = s + 1;
+      stage_from_k(s < 2 ? 0.5 * c.h : c.h);
     } else {
       const double h6 = c.h / 6.0;
-      const double *k1 = S + n2, *k2 = S + 2 * n2, *k3 = S + 3 * n2, *k4 = S + 4 * n2;
-      for (int i = tid; i < n2; i += BS) y[i] = y[i] + h6 * (k1[i] + k4[i] + 2.0 * (k2[i] + k3[i]));
+      const double *k1 = S + n2, *k2 = S + 2 * n2, *k3 = S + 3 * n2;
+      double nrm = 0.0;
+#pragma unroll
+      for (int j = 0; j < PT; j++) {
+        const int i = tid + j * BS;
+        if (i < N) {
+          xr[j] = y[i] + h6 * (k1[i] + kr[j] + 2.0 * (k2[i] + k3[i]));
+          xi[j] = y[i + N] + h6 * (k1[i + N] + ki[j] + 2.0 * (k2[i + N] + k3[i + N]));
+          y[i] = xr[j];
+          y[i + N] = xi[j];
+          nrm += xr[j] * xr[j] + xi[j] * xi[j];
+        }
+      }
       c.nstp++;
       c.stage = 0;
       if (d.mode == MQCT_PROP_AT_ODEINT) {
@@ -514,18 +835,17 @@ __global__ __launch_bounds__(BS) void step_kernel(BDev d, int par) {
         if (c.last) c.status = 1;
         else if (c.x + c.h >= c.x2) { c.h = c.x2 - c.x; c.last = 1; }
       }
-      __syncthreads();
-      double nrm = 0.0;
-      for (int i = tid; i < N; i += BS) nrm += y[i] * y[i] + y[i + N] * y[i + N];
       nrm = block_sum(nrm, red);
       c.norm_err = fmax(c.norm_err, fabs(1.0 - nrm));
     }
   } else if (s == 0) {
+    store_k();
     if (c.nstp == MAXSTP - 1) {
       // continue as rk4 with k1 = dydx at x
       c.rk4 = 1;
       c.h = copysign(fabs(c.x2 - c.x), c.x2 - c.x1);
       c.stage = 1;
+      stage_from_k(0.5 * c.h);
     } else {
       // odeint: yscal = |y| + |h dydx| + TINY with h at the start of the
       // step (formed again from y and k1 when the error is measured), then
@@ -533,32 +853,78 @@ __global__ __launch_bounds__(BS) void step_kernel(BDev d, int par) {
       c.h0 = c.h;
       if ((c.x + c.h - c.x2) * (c.x + c.h - c.x1) > 0.0) c.h = c.x2 - c.x;
       c.stage = 1;
+      stage_from_k(CK_B[1][0] * c.h);
     }
   } else if (s < 5) {
+    // Cash-Karp stage s + 1 = y + h sum_q B[s+1][q] k_{q+1}, the last k in registers
+    store_k();
     c.stage = s + 1;
+#pragma unroll
+    for (int j = 0; j < PT; j++) {
+      const int i = tid + j * BS;
+      if (i < N) {
+        double ar = 0.0, ai = 0.0;
+        for (int q = 0; q < s; q++) {
+          ar += CK_B[s + 1][q] * S[(1 + q) * n2 + i];
+          ai += CK_B[s + 1][q] * S[(1 + q) * n2 + i + N];
+        }
+        ar += CK_B[s + 1][s] * kr[j];
+        ai += CK_B[s + 1][s] * ki[j];
+        xr[j] = y[i] + c.h * ar;
+        xi[j] = y[i + N] + c.h * ai;
+      }
+    }
   } else {
+    // rkck's error estimate and fifth-order solution in one pass (k6 in
+    // registers), then rkqs's step control (suppl_routines.f:1229-1283)
     const double C1 = 37.0 / 378.0, C3 = 250.0 / 621.0, C4 = 125.0 / 594.0, C6 = 512.0 / 1771.0;
     const double DC1 = C1 - 2825.0 / 27648.0, DC3 = C3 - 18575.0 / 48384.0,
                  DC4 = C4 - 13525.0 / 55296.0, DC5 = -277.0 / 14336.0, DC6 = C6 - .25;
-    const double *k1 = S + n2, *k3 = S + 3 * n2, *k4 = S + 4 * n2, *k5 = S + 5 * n2, *k6 = S + 6 * n2;
+    const double *k1 = S + n2, *k3 = S + 3 * n2, *k4 = S + 4 * n2, *k5 = S + 5 * n2;
     double em = 0.0;
-    for (int i = tid; i < n2; i += BS) {
-      const double yerr = c.h * (DC1 * k1[i] + DC3 * k3[i] + DC4 * k4[i] + DC5 * k5[i] + DC6 * k6[i]);
-      const double ysc = fabs(y[i]) + fabs(c.h0 * k1[i]) + TINY;
-      em = fmax(em, fabs(yerr / ysc));
+#pragma unroll
+    for (int j = 0; j < PT; j++) {
+      const int i = tid + j * BS;
+      if (i < N) {
+#pragma unroll
+        for (int p = 0; p < 2; p++) {
+          const int ii = i + p * N;
+          const double k6 = p ? ki[j] : kr[j];
+          const double yerr = c.h * (DC1 * k1[ii] + DC3 * k3[ii] + DC4 * k4[ii] + DC5 * k5[ii] + DC6 * k6);
+          const double ysc = fabs(y[ii]) + fabs(c.h0 * k1[ii]) + TINY;
+          em = fmax(em, fabs(yerr / ysc));
+          const double yn = y[ii] + c.h * (C1 * k1[ii] + C3 * k3[ii] + C4 * k4[ii] + C6 * k6);
+          if (p) xi[j] = yn;
+          else xr[j] = yn;
+        }
+      }
     }
     em = block_max(em, red) / c.eps;
-    // rkqs (suppl_routines.f:1229-1283): a step at or below hmin is taken as is
+    // a step at or below hmin is taken as is
     const bool accept = (c.h <= c.hmin) || !(em > 1.0);
     if (!accept) {
       const double htemp = 0.9 * c.h * pow(em, -.25);
       const double a = fabs(htemp), b = 0.1 * fabs(c.h);
       c.h = copysign(a > b ? a : b, c.h);
       c.stage = 1;  // retry with the same k1
+#pragma unroll
+      for (int j = 0; j < PT; j++) {
+        const int i = tid + j * BS;
+        if (i < N) {
+          xr[j] = y[i] + CK_B[1][0] * c.h * k1[i];
+          xi[j] = y[i + N] + CK_B[1][0] * c.h * k1[i + N];
+        }
+      }
     } else {
       if (c.h > c.hmin) c.hnext = (em > 1.89e-4) ? 0.9 * c.h * pow(em, -.2) : 5.0 * c.h;
-      for (int i = tid; i < n2; i += BS)
-        y[i] = y[i] + c.h * (C1 * k1[i] + C3 * k3[i] + C4 * k4[i] + C6 * k6[i]);
+#pragma unroll
+      for (int j = 0; j < PT; j++) {
+        const int i = tid + j * BS;
+        if (i < N) {
+          y[i] = xr[j];
+          y[i + N] = xi[j];
+        }
+      }
       c.x += c.h;
       c.nstp++;
       c.stage = 0;
@@ -570,7 +936,6 @@ __global__ __launch_bounds__(BS) void step_kernel(BDev d, int par) {
       }
     }
   }
-  __syncthreads();
   if (c.status != 0) {
     if (tid == 0) {
       atomicSub(d.active, 1);
@@ -578,7 +943,7 @@ __global__ __launch_bounds__(BS) void step_kernel(BDev d, int par) {
     }
     return;
   }
-  prepare_stage(d, t, c, sys, shm, par);
+  prepare_stage<PT>(d, t, c, xr, xi, par);
   if (tid == 0) d.ctl[t] = c;
 }
 
@@ -607,17 +972,57 @@ __global__ void knot_matrices_kernel(long long T, int n_r, const double *__restr
   Y2r[(long long)lb * nb + la] = m2;
 }
 
+// the same matrices in the fragment order of the tile layout: element
+// (row i, column k) of block [Y | Y2] (k < 2 nbp, nbp = nb rounded up to
+// 4U) lies in tile i / 16 at ((k / 4U) 64 + lane) U + k % U with lane =
+// 16 ((k % 4U) / U) + i % 16
+template <typename KT>
+__global__ void knot_tiles_kernel(long long T, int n_r, const double *__restrict__ Mt,
+                                  const double *__restrict__ Y2t, const int *__restrict__ pi,
+                                  const int *__restrict__ pj, const int *__restrict__ state_pos,
+                                  const int *__restrict__ pos_blk, const int *__restrict__ blk_start,
+                                  const int *__restrict__ blk_tile0, const long long *__restrict__ tile_off,
+                                  long long KS, KT *__restrict__ KT_) {
+  constexpr int U = Frag<KT>::U, KB = 4 * U;
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= T * n_r) return;
+  const int r = (int)(idx / T);
+  const long long k = idx % T;
+  const int a = state_pos[pi[k]], b = state_pos[pj[k]];
+  const int blk = pos_blk[a];
+  const int nb = blk_start[blk + 1] - blk_start[blk];
+  const int nbp = (nb + KB - 1) / KB * KB;
+  const int la = a - blk_start[blk], lb = b - blk_start[blk];
+  const KT m = (KT)Mt[(long long)r * T + k], m2 = (KT)Y2t[(long long)r * T + k];
+  KT *base = KT_ + (long long)r * KS;
+  auto put = [&](int row, int kc, KT v) {
+    const int lane = 16 * ((kc % KB) / U) + (row & 15);
+    base[tile_off[blk_tile0[blk] + row / 16] + ((long long)(kc / KB) * 64 + lane) * U + kc % U] = v;
+  };
+  put(la, lb, m);
+  put(lb, la, m);
+  put(la, nbp + lb, m2);
+  put(lb, nbp + la, m2);
+}
+
+template <int PT>
 __global__ __launch_bounds__(BS) void init_kernel(BDev d, const mqct_traj_t *__restrict__ tr,
                                                   const int *__restrict__ state_pos) {
-  extern __shared__ double sys[];
-  __shared__ double shm[4];
-  const int t = blockIdx.x, tid = threadIdx.x, n2 = 2 * d.N;
+  const int t = blockIdx.x, tid = threadIdx.x, N = d.N, n2 = 2 * N;
   double *S = d.S + (long long)t * NSLOT * n2;
   const mqct_traj_t p = tr[t];
-  for (int i = tid; i < n2; i += BS) S[i] = 0.0;
-  __syncthreads();
-  if (tid == 0) S[state_pos[p.s_st - 1]] = 1.0;
-  __syncthreads();
+  const int s0 = state_pos[p.s_st - 1];
+  double xr[PT], xi[PT];
+#pragma unroll
+  for (int j = 0; j < PT; j++) {
+    const int i = tid + j * BS;
+    xr[j] = xi[j] = 0.0;
+    if (i < N) {
+      xr[j] = (i == s0) ? 1.0 : 0.0;
+      S[i] = xr[j];
+      S[i + N] = 0.0;
+    }
+  }
   Ctl c{};
   const long long o1 = d.poff[p.at_path + 1];
   c.x = 0.0;
@@ -633,7 +1038,7 @@ __global__ __launch_bounds__(BS) void init_kernel(BDev d, const mqct_traj_t *__r
   c.hmin = p.hmin;
   c.eps = p.eps;
   c.path = p.at_path;
-  prepare_stage(d, t, c, sys, shm, 0);
+  prepare_stage<PT>(d, t, c, xr, xi, 0);
   if (tid == 0) d.ctl[t] = c;
 }
 
@@ -747,6 +1152,55 @@ int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *t
     ds->km_fp32 = knots_fp32;
   }
   ds->bat_S2 = S2;
+  // the fragment-ordered tiles: 16 rows of a block each, [Y | Y2] with both
+  // halves padded to a multiple of 4U columns
+  const int KBh = knots_fp32 ? 16 : 8;
+  std::vector<int> tile_blk, tile_r0, blk_tile0(nblk);
+  std::vector<long long> tile_off;
+  long long KS = 0;
+  for (int b = 0; b < nblk; b++) {
+    const int nb = blk_start[b + 1] - blk_start[b], nbp = (nb + KBh - 1) / KBh * KBh;
+    blk_tile0[b] = (int)tile_blk.size();
+    for (int r0 = 0; r0 < nb; r0 += 16) {
+      tile_blk.push_back(b);
+      tile_r0.push_back(r0);
+      tile_off.push_back(KS);
+      KS += 16LL * 2 * nbp;
+    }
+  }
+  const int ntile = (int)tile_blk.size();
+  // 64-row groups of the blocks (the LDS-staged work queue)
+  std::vector<int> rg_blk, rg_row0;
+  for (int b = 0; b < nblk; b++)
+    for (int r0 = 0; r0 < blk_start[b + 1] - blk_start[b]; r0 += 64) {
+      rg_blk.push_back(b);
+      rg_row0.push_back(r0);
+    }
+  const int nrg = (int)rg_blk.size();
+  DevBuf d_tb, d_tr0, d_to, d_bt0, d_rgb, d_rgr;
+  if (up(d_tb, tile_blk.data(), ntile * 4) || up(d_tr0, tile_r0.data(), ntile * 4) ||
+      up(d_to, tile_off.data(), ntile * 8) || up(d_bt0, blk_tile0.data(), nblk * 4) ||
+      up(d_rgb, rg_blk.data(), nrg * 4) || up(d_rgr, rg_row0.data(), nrg * 4))
+    return mqct_fail(MQCT_ERR_NOMEM, "batched: tile tables");
+  if (!ds->kt_valid || ds->kt_fp32 != knots_fp32) {
+    const size_t need_kt = (size_t)n_r * KS * (knots_fp32 ? 4 : 8);
+    if (ds->d_KT.ensure(need_kt)) return mqct_fail(MQCT_ERR_NOMEM, "batched: knot tiles");
+    HIP_TRY(hipMemsetAsync(ds->d_KT.p, 0, need_kt, h->stream));
+    const unsigned g = (unsigned)((T * n_r + 255) / 256);
+    if (knots_fp32)
+      knot_tiles_kernel<float><<<g, 256, 0, h->stream>>>(
+          T, n_r, ds->d_Mt.as<double>(), ds->d_Y2t.as<double>(), ds->d_pi.as<int>(), ds->d_pj.as<int>(),
+          d_sp.as<int>(), d_pb.as<int>(), d_bs.as<int>(), d_bt0.as<int>(), d_to.as<long long>(), KS,
+          ds->d_KT.as<float>());
+    else
+      knot_tiles_kernel<double><<<g, 256, 0, h->stream>>>(
+          T, n_r, ds->d_Mt.as<double>(), ds->d_Y2t.as<double>(), ds->d_pi.as<int>(), ds->d_pj.as<int>(),
+          d_sp.as<int>(), d_pb.as<int>(), d_bs.as<int>(), d_bt0.as<int>(), d_to.as<long long>(), KS,
+          ds->d_KT.as<double>());
+    HIP_TRY(hipGetLastError());
+    ds->kt_valid = true;
+    ds->kt_fp32 = knots_fp32;
+  }
   // trajectory state: slots, rotated amplitudes, GEMM outputs, controls,
   // double-buffered buckets, counters
   auto al = [](size_t x) { return (x + 255) / 256 * 256; };
@@ -770,6 +1224,15 @@ int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *t
   d.S2 = S2;
   d.blk_start = d_bs.as<int>();
   d.blk_off = d_bo.as<long long>();
+  d.KT = ds->d_KT.p;
+  d.KS = KS;
+  d.ntile = ntile;
+  d.tile_blk = d_tb.as<int>();
+  d.tile_r0 = d_tr0.as<int>();
+  d.tile_off = d_to.as<long long>();
+  d.nrg = nrg;
+  d.rg_blk = d_rgb.as<int>();
+  d.rg_row0 = d_rgr.as<int>();
   d.E = d_E.as<double>();
   d.nb_lo = d_nlo.as<int>();
   d.nb_hi = d_nhi.as<int>();
@@ -790,17 +1253,20 @@ int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *t
   HIP_TRY(hipMemsetAsync(d.cnt, 0, (size_t)2 * n_r * 4, h->stream));
   HIP_TRY(hipMemsetAsync(d.active, 0, 32, h->stream));
   HIP_TRY(hipMemcpyAsync(d.active, &n_traj, 4, hipMemcpyHostToDevice, h->stream));
-  init_kernel<<<n_traj, BS, lds_step, h->stream>>>(d, d_tr.as<mqct_traj_t>(), d_sp.as<int>());
+  // states per thread of the step kernel
+  const int PT = N <= 4 * BS ? 4 : (N <= 8 * BS ? 8 : MAXPT);
+  auto step = [&](int par) {
+    if (PT == 4) step_kernel<4><<<n_traj, BS, lds_step, h->stream>>>(d, par);
+    else if (PT == 8) step_kernel<8><<<n_traj, BS, lds_step, h->stream>>>(d, par);
+    else step_kernel<MAXPT><<<n_traj, BS, lds_step, h->stream>>>(d, par);
+  };
+  if (PT == 4) init_kernel<4><<<n_traj, BS, 0, h->stream>>>(d, d_tr.as<mqct_traj_t>(), d_sp.as<int>());
+  else if (PT == 8) init_kernel<8><<<n_traj, BS, 0, h->stream>>>(d, d_tr.as<mqct_traj_t>(), d_sp.as<int>());
+  else init_kernel<MAXPT><<<n_traj, BS, 0, h->stream>>>(d, d_tr.as<mqct_traj_t>(), d_sp.as<int>());
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(h->stream));
   const dim3 kb(n_r, nblk);
-  // a shape chosen on a batch of a different size (by more than 2x) is
-  // chosen again: the best tile depends on the bucket fill
   const int pi_ = knots_fp32 ? 1 : 0;
-  int shape = ds->gemm_shape[pi_];
-  if (shape >= 0 && !ds->gemm_fixed &&
-      (n_traj > 2 * ds->gemm_tuned_n[pi_] || 2 * n_traj < ds->gemm_tuned_n[pi_]))
-    shape = -1;
   auto gemm = [&](int sh, int par) {
     if (knots_fp32) launch_gemm<float>(sh, kb, maxnb, h->stream, d, par);
     else launch_gemm<double>(sh, kb, maxnb, h->stream, d, par);
@@ -813,15 +1279,16 @@ int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *t
     ds->prof_ev.resize(2 * ITER);
     for (auto &e : ds->prof_ev) HIP_TRY(hipEventCreate(&e));
   }
-  // ITER iterations launched one by one with HIP events on this stream
+  // n_it iterations launched one by one with HIP events on this stream
   // around every GEMM launch; returns the summed GEMM time
   auto timed_iters = [&](int sh, int n_it, double &gemm_ms) -> int {
+    const int active0 = active;
     for (int it = 0; it < n_it; it++) {
       const int par = (int)((it_done + it) & 1);
       HIP_TRY(hipEventRecord(ds->prof_ev[2 * it], h->stream));
       gemm(sh, par);
       HIP_TRY(hipEventRecord(ds->prof_ev[2 * it + 1], h->stream));
-      step_kernel<<<n_traj, BS, lds_step, h->stream>>>(d, par ^ 1);
+      step(par ^ 1);
     }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(&active, d.active, 4, hipMemcpyDeviceToHost, h->stream));
@@ -836,63 +1303,113 @@ int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *t
     if (prof) {
       ds->prof_ms += gemm_ms;
       ds->prof_launches += n_it;
+      if (ds->prof_trace.size() < (1u << 22))
+        ds->prof_trace.insert(ds->prof_trace.end(), {(double)active0, (double)n_it, gemm_ms, (double)sh});
     }
     return MQCT_OK;
+  };
+  // the tile shape for the current number of running trajectories: the
+  // bucket fill, and with it the best shape, changes as trajectories end, so
+  // shapes are chosen per level floor(log2(active)) -- on the first
+  // iterations at that level (real iterations of this run): three rounds of
+  // two timed iterations per shape, the fastest total wins and is kept for
+  // later runs on this matrix
+  auto level_of = [](int a) {
+    int l = 0;
+    while (l + 1 < DynState::kLevels && (2LL << l) <= a) l++;
+    return l;
+  };
+  // (the work-queue shapes keep the chunk prefix of all knots in LDS)
+  const bool queue_ok = (size_t)(n_r + 1) * 4 <= 64 * 1024;
+  if (ds->gemm_fixed >= 0 && uses_queue(ds->gemm_fixed) && !queue_ok)
+    return mqct_fail(MQCT_ERR_ARG, "batched: the work-queue GEMM shapes take at most 16383 knots");
+  auto shape_for = [&](int &sh) -> int {
+    for (;;) {
+      if (ds->gemm_fixed >= 0) {
+        sh = ds->gemm_fixed;
+        return MQCT_OK;
+      }
+      const int lv = level_of(active);
+      if (ds->gemm_shape[pi_][lv] >= 0 || active == 0) {
+        sh = ds->gemm_shape[pi_][lv];
+        return MQCT_OK;
+      }
+      // candidates more than 1.5x slower than the best after the first
+      // round are not timed again
+      const int ncand = kNShapes;
+      double tot[kNShapes] = {};
+      bool keep[kNShapes];
+      for (int s = 0; s < kNShapes; s++) keep[s] = queue_ok || !uses_queue(s);
+      for (int round = 0; round < 3 && active > 0; round++) {
+        for (int s = 0; s < ncand && active > 0; s++) {
+          if (!keep[s]) continue;
+          double gm = 0.0;
+          if (int rc = timed_iters(s, 2, gm)) return rc;
+          tot[s] += gm;
+        }
+        double best_t = 1e300;
+        for (int s = 0; s < ncand; s++)
+          if (keep[s]) best_t = std::min(best_t, tot[s]);
+        for (int s = 0; s < ncand; s++)
+          if (keep[s] && tot[s] > 1.5 * best_t) keep[s] = false;
+      }
+      int best = -1;
+      for (int s = 0; s < ncand; s++)
+        if (keep[s] && (best < 0 || tot[s] < tot[best])) best = s;
+      ds->gemm_shape[pi_][lv] = best < 0 ? 0 : best;
+      // loop: the level is read again (trajectories ended while timing)
+    }
   };
   // every trajectory ends after at most MAXSTP accepted steps and a bounded
   // number of step-size reductions; the cap only guards the host loop
   const int max_rounds = 1 << 17;
   HIP_TRY(hipEventRecord(h->ev[5], h->stream));
-  if (shape < 0) {
-    // tile-shape tuning on the first iterations (real iterations of this
-    // run): three rounds of two timed iterations per shape, the fastest
-    // total wins and is kept for later runs on this matrix
-    double tot[kNShapes] = {};
-    for (int round = 0; round < 3 && active > 0; round++)
-      for (int sh = 0; sh < kNShapes && active > 0; sh++) {
-        double gm = 0.0;
-        if (int rc = timed_iters(sh, 2, gm)) return rc;
-        tot[sh] += gm;
-      }
-    shape = 0;
-    for (int sh = 1; sh < kNShapes; sh++)
-      if (tot[sh] < tot[shape]) shape = sh;
-    ds->gemm_shape[pi_] = shape;
-    ds->gemm_tuned_n[pi_] = n_traj;
-  }
-  if (!prof) {
-    // ITER iterations (GEMM, step) captured once, replayed until every
-    // trajectory has ended
-    struct GraphGuard {
-      hipGraph_t g = nullptr;
-      hipGraphExec_t e = nullptr;
-      ~GraphGuard() {
-        if (e) (void)hipGraphExecDestroy(e);
-        if (g) (void)hipGraphDestroy(g);
-      }
-    } gg;
-    if (active > 0) {
-      const int p0 = (int)(it_done & 1);
+  int shape = -1;
+  if (int rc = shape_for(shape)) return rc;
+  ds->gemm_first[pi_] = shape;
+  // ITER iterations (GEMM, step) captured per shape and bucket parity,
+  // replayed until every trajectory has ended; without graphs (profiling)
+  // the same iterations are launched one by one between HIP events
+  struct GraphGuard {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t e = nullptr;
+    int shape = -1, par = -1;
+    void reset() {
+      if (e) (void)hipGraphExecDestroy(e);
+      if (g) (void)hipGraphDestroy(g);
+      e = nullptr;
+      g = nullptr;
+      shape = par = -1;
+    }
+    ~GraphGuard() { reset(); }
+  } gg;
+  for (int rep = 0; rep < max_rounds && active > 0; rep++) {
+    if (int rc = shape_for(shape)) return rc;
+    if (active == 0) break;
+    if (prof) {
+      double gm = 0.0;
+      if (int rc = timed_iters(shape, ITER, gm)) return rc;
+      continue;
+    }
+    const int p0 = (int)(it_done & 1);
+    if (gg.shape != shape || gg.par != p0) {
+      gg.reset();
       HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
       for (int it = 0; it < ITER; it++) {
         const int par = (it + p0) & 1;
         gemm(shape, par);
-        step_kernel<<<n_traj, BS, lds_step, h->stream>>>(d, par ^ 1);
+        step(par ^ 1);
       }
       const hipError_t cap = hipStreamEndCapture(h->stream, &gg.g);
       HIP_TRY(cap);
       HIP_TRY(hipGraphInstantiate(&gg.e, gg.g, nullptr, nullptr, 0));
+      gg.shape = shape;
+      gg.par = p0;
     }
-    for (int rep = 0; rep < max_rounds && active > 0; rep++) {
-      HIP_TRY(hipGraphLaunch(gg.e, h->stream));
-      HIP_TRY(hipMemcpyAsync(&active, d.active, 4, hipMemcpyDeviceToHost, h->stream));
-      HIP_TRY(hipStreamSynchronize(h->stream));
-    }
-  } else {
-    for (int rep = 0; rep < max_rounds && active > 0; rep++) {
-      double gm = 0.0;
-      if (int rc = timed_iters(shape, ITER, gm)) return rc;
-    }
+    HIP_TRY(hipGraphLaunch(gg.e, h->stream));
+    it_done += ITER;
+    HIP_TRY(hipMemcpyAsync(&active, d.active, 4, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
   }
   HIP_TRY(hipEventRecord(h->ev[6], h->stream));
   HIP_TRY(hipEventSynchronize(h->ev[6]));
@@ -927,6 +1444,7 @@ extern "C" int mqct_hip_batched_profile(mqct_hip_t *h, int on) {
   if (!h || !h->dyn) return mqct_fail(MQCT_ERR_STATE, "batched_profile: no matrix loaded");
   DynState *ds = h->dyn;
   ds->prof_on = on != 0;
+  ds->prof_trace.clear();
   ds->prof_launches = 0;
   ds->prof_ms = 0.0;
   ds->prof_flops = 0.0;
@@ -934,26 +1452,46 @@ extern "C" int mqct_hip_batched_profile(mqct_hip_t *h, int on) {
   return MQCT_OK;
 }
 
+extern "C" int mqct_hip_batched_prof_trace(mqct_hip_t *h, int cap, double *rows, int *n) {
+  if (!h || !h->dyn || !n || cap < 0 || (cap > 0 && !rows)) return mqct_fail(MQCT_ERR_ARG, "batched_prof_trace: bad argument");
+  const auto &tr = h->dyn->prof_trace;
+  *n = (int)(tr.size() / 4);
+  const size_t m = std::min((size_t)cap, tr.size() / 4) * 4;
+  std::copy(tr.begin(), tr.begin() + m, rows);
+  return MQCT_OK;
+}
+
 extern "C" int mqct_hip_batched_set_tile(mqct_hip_t *h, int shape) {
   if (!h || !h->dyn) return mqct_fail(MQCT_ERR_STATE, "batched_set_tile: no matrix loaded");
   if (shape < -1 || shape >= kNShapes) return mqct_fail(MQCT_ERR_ARG, "batched_set_tile: unknown shape");
   DynState *ds = h->dyn;
-  for (int p = 0; p < 2; p++) {
-    ds->gemm_shape[p] = shape;
-    ds->gemm_tuned_n[p] = shape < 0 ? 0 : (1 << 30);  // a fixed shape is kept for any batch size
-  }
-  ds->gemm_fixed = shape >= 0;
+  ds->gemm_fixed = shape;
+  if (shape < 0)
+    for (auto &p : ds->gemm_shape)
+      for (int &v : p) v = -1;
   return MQCT_OK;
 }
 
-extern "C" int mqct_hip_batched_tile(mqct_hip_t *h, int *entries, int *k_rows, int *row_tiles) {
-  if (!h || !h->dyn || !entries || !k_rows || !row_tiles)
+extern "C" int mqct_hip_batched_tile(mqct_hip_t *h, int shape, int *form, int *entries, int *k_rows,
+                                     int *row_tiles) {
+  if (!h || !form || !entries || !k_rows || !row_tiles || shape < -1 || shape >= kNShapes)
     return mqct_fail(MQCT_ERR_ARG, "batched_tile: bad argument");
-  const int sh = h->dyn->km_fp32 ? h->dyn->gemm_shape[1] : h->dyn->gemm_shape[0];
-  if (sh < 0) return mqct_fail(MQCT_ERR_STATE, "batched_tile: no batched run yet");
+  int sh = shape;
+  if (sh < 0) {
+    if (!h->dyn) return mqct_fail(MQCT_ERR_STATE, "batched_tile: no matrix loaded");
+    sh = h->dyn->gemm_first[h->dyn->km_fp32 ? 1 : 0];
+    if (sh < 0) return mqct_fail(MQCT_ERR_STATE, "batched_tile: no batched run yet");
+  }
+  *form = kShapes[sh].form;
   *entries = kShapes[sh].ge;
   *k_rows = kShapes[sh].gk;
   *row_tiles = kShapes[sh].grt;
+  return MQCT_OK;
+}
+
+extern "C" int mqct_hip_batched_tile_levels(mqct_hip_t *h, int fp32, int n, int *shapes) {
+  if (!h || !h->dyn || !shapes || n < 0) return mqct_fail(MQCT_ERR_ARG, "batched_tile_levels: bad argument");
+  for (int l = 0; l < n; l++) shapes[l] = l < DynState::kLevels ? h->dyn->gemm_shape[fp32 ? 1 : 0][l] : -1;
   return MQCT_OK;
 }
 

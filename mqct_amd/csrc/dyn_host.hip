@@ -30,7 +30,9 @@ int up(mqct_hip *h, DevBuf &b, const T *src, size_t n) {
 // Tables of resize (mqct_eqtns.f:1-58): pair indices and Coriolis factors.
 int set_tables(mqct_hip *h, DynState *ds, const int *ind_mat) {
   ds->km_valid = false;
-  if (!ds->gemm_fixed) ds->gemm_shape[0] = ds->gemm_shape[1] = -1;
+  ds->kt_valid = false;
+  for (auto &p : ds->gemm_shape)
+    for (int &v : p) v = -1;
   const long long T = ds->T;
   ds->pi.resize(T);
   ds->pj.resize(T);

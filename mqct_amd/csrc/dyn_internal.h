@@ -20,14 +20,25 @@ struct DynState {
   long long bat_S2 = 0;
   bool km_valid = false;  // d_KM holds the current matrix
   bool km_fp32 = false;   // ... stored in FP32
-  int gemm_shape[2] = {-1, -1};  // interval-GEMM tile shape per coefficient precision (FP64, FP32)
-  int gemm_tuned_n[2] = {0, 0};  // batch size the shape was chosen on
-  bool gemm_fixed = false;       // shape set by the caller
+  DevBuf d_KT;       // the same matrices in MFMA-fragment order (16-row tiles)
+  bool kt_valid = false, kt_fp32 = false;
+  // interval-GEMM tile shape per coefficient precision (FP64, FP32) and
+  // level of the running-trajectory count (floor(log2(active))), -1 not yet
+  // chosen; a shape fixed by the caller (>= 0) overrides them
+  static constexpr int kLevels = 24;
+  int gemm_shape[2][kLevels];
+  int gemm_fixed = -1;
+  int gemm_first[2] = {-1, -1};  // shape the last run started with
+  DynState() {
+    for (auto &p : gemm_shape)
+      for (int &v : p) v = -1;
+  }
   // batched-form kernel timing
   bool prof_on = false;
   long long prof_launches = 0;
   double prof_ms = 0.0, prof_flops = 0.0, prof_bytes = 0.0;
   std::vector<hipEvent_t> prof_ev;
+  std::vector<double> prof_trace;  // per timed chunk: running trajectories, iterations, GEMM ms, shape
   ~DynState() {
     for (auto e : prof_ev) (void)hipEventDestroy(e);
   }

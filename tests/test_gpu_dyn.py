@@ -201,7 +201,10 @@ def test_at_batched_fp32_coefficients(dev):
     assert np.all(s32["norm_err"] < 1e-6)
 
 
-@pytest.mark.parametrize("shape", list(range(8)))
+N_SHAPES = 15
+
+
+@pytest.mark.parametrize("shape", list(range(N_SHAPES)))
 def test_at_batched_every_gemm_tile(dev, shape):
     """Every tile shape the interval-GEMM tuner may choose gives the plain
     kernel's RK4 trajectories (asymmetric top + atom, j <= 3, 40
@@ -227,4 +230,33 @@ def test_at_batched_every_gemm_tile(dev, shape):
     N = wl.basis.n_states
     d = np.max(np.abs(q1[:, :2 * N] - q0[:, :2 * N]))
     assert d <= 1e-9, f"shape {shape}: batched vs plain amplitude diff {d}"
+    assert np.array_equal(s0["n_steps"], s1["n_steps"])
+
+
+@pytest.mark.parametrize("shape", [0, 5, 8, 10, 11, 12, 13, 14])
+def test_at_batched_fp32_knots_every_layout(dev, shape):
+    """FP32 knot matrices through the LDS-staged, the per-wave and the
+    fragment-ordered GEMM forms: the same trajectories as the FP64
+    coefficients to the FP32 rounding of the coupling (1e-6)."""
+    from mqct_amd.matrix import MatrixBuilder
+    from mqct_amd.propagate import at_first_step
+    wl = W.asym_top_atom(jmax=3, n_beta=12, n_gamma=24, n_r=120)
+    bb, E = W.at_batch(24, seed=13)
+    paths, l, _ = at_first_step(dev, W.asym_top_atom(jmax=1, n_beta=12, n_gamma=24, n_r=120), bb, E,
+                                time_step=20.0, rec_cap=20000)
+    MatrixBuilder(dev, wl).build("fp64", copy_out=False)
+    prop = Propagator(dev, wl.basis, wl.R, mass_amu=wl.mass_amu, from_build=True)
+    prop.load_paths(paths)
+    rows = []
+    for i in range(len(bb)):
+        rows += prop.trajectories([l[i]], float(E[i]), 1, time_step=10.0, at_paths=[i])
+    q0, s0 = prop.run(L.MQCT_PROP_AT_RK4, rows, batched=0)
+    L.check(dev.lib.mqct_hip_batched_set_tile(dev.h, shape), "set_tile")
+    try:
+        q1, s1 = prop.run(L.MQCT_PROP_AT_RK4, rows, batched=2)
+    finally:
+        L.check(dev.lib.mqct_hip_batched_set_tile(dev.h, -1), "set_tile")
+    N = wl.basis.n_states
+    d = np.max(np.abs(q1[:, :2 * N] - q0[:, :2 * N]))
+    assert 0.0 < d <= 1e-6, f"shape {shape}: fp32-knot batched vs plain amplitude diff {d}"
     assert np.array_equal(s0["n_steps"], s1["n_steps"])
