@@ -11,7 +11,8 @@ in HBM.
 
 Reported beside it:
   * the same batch with FP64 coefficients, and the cross-section error of
-    the FP32-coefficient form against it;
+    the FP32-coefficient form against it (FP32 is the storage the workload
+    names for the spline coefficients: "complex FP32 1771x1771");
   * the plain FP64 per-trajectory kernel on a sample of the batch (the
     comparison the metric names);
   * the roofline of the dominant kernel (the interval GEMM) from HIP events
@@ -22,7 +23,7 @@ Reported beside it:
     mixed-precision builds through the AT propagation on a sample;
   * the CPU baseline: the C oracle's AT-RK45 trajectory on one host core.
 
-    python bench.py [--gpus 1] [--steps 1] [--warmup 0]
+    python bench.py [--gpus 1] [--steps 1] [--warmup 1]
 """
 import argparse
 import ctypes
@@ -66,11 +67,12 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
-    ap.add_argument("--warmup", type=int, default=0)
+    ap.add_argument("--warmup", type=int, default=1,
+                    help="untimed full-batch runs first (the GEMM tile tuner chooses its shapes in the first)")
     ap.add_argument("--n-traj", type=int, default=4096)
     ap.add_argument("--hmin", type=float, default=20.0,
                     help="MIN_TMSTP (a.u.): the step floor of odeint's error control")
-    ap.add_argument("--knots", choices=("fp32", "fp64"), default="fp64",
+    ap.add_argument("--knots", choices=("fp32", "fp64"), default="fp32",
                     help="storage precision of the spline-coefficient matrices of the timed form")
     ap.add_argument("--plain-sample", type=int, default=512)
     ap.add_argument("--xsec-sample", type=int, default=512)
@@ -160,8 +162,27 @@ def main():
     la, ms, fl, by = ctypes.c_int64(), ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
     L.check(dev.lib.mqct_hip_batched_kernel_stats(dev.h, ctypes.byref(la), ctypes.byref(ms), ctypes.byref(fl),
                                                   ctypes.byref(by)), "kernel_stats")
+    # GEMM and step time per level floor(log2(running trajectories))
+    ntr = ctypes.c_int()
+    L.check(dev.lib.mqct_hip_batched_prof_trace(dev.h, 0, None, ctypes.byref(ntr)), "prof_trace")
+    trace = np.zeros(5 * max(ntr.value, 1))
+    L.check(dev.lib.mqct_hip_batched_prof_trace(dev.h, ntr.value, L.dptr(trace), ctypes.byref(ntr)), "prof_trace")
+    levels = {}
+    for a0, nit, gms, _sh, sms in trace[:5 * ntr.value].reshape(-1, 5):
+        key = "2^%d" % int(np.log2(max(a0, 1)))
+        e = levels.setdefault(key, {"iterations": 0, "gemm_ms": 0.0, "step_ms": 0.0})
+        e["iterations"] += int(nit)
+        e["gemm_ms"] += float(gms)
+        e["step_ms"] += float(sms)
+    levels = {k: {"iterations": v["iterations"], "gemm_ms": round(v["gemm_ms"], 1), "step_ms": round(v["step_ms"], 1),
+                  "gemm_us_per_it": round(1e3 * v["gemm_ms"] / max(v["iterations"], 1), 1),
+                  "step_us_per_it": round(1e3 * v["step_ms"] / max(v["iterations"], 1), 1)}
+              for k, v in sorted(levels.items(), key=lambda kv: -int(kv[0][2:]))}
+    log("per level: " + "; ".join("%s %d it, gemm %.1f us, step %.1f us" % (k, v["iterations"], v["gemm_us_per_it"],
+                                                                             v["step_us_per_it"])
+                                   for k, v in levels.items()))
     L.check(dev.lib.mqct_hip_batched_profile(dev.h, 0), "batched_profile")
-    FORMS = ("lds", "direct", "stream", "queue", "lds_queue")
+    FORMS = ("lds", "direct", "stream", "queue", "lds_queue", "pipe")
 
     def shape_desc(sh):
         fo, ge, gk, grt = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
@@ -185,14 +206,14 @@ def main():
             roof = {"bound": "mfma", "achieved": round(tflops, 3), "peak": PEAK_FP64_TFLOPS,
                     "unit": "TFLOP/s", "frac": round(f_mfma, 4)}
         roof.update({"traffic": traffic_from_profiles(args.knots),
-                     "kernel": "interval GEMM over %s knot matrices, FP64 MFMA 16x16x4 (gemm_kernel / "
-                               "gemm_direct_kernel / gemm_stream_kernel, tile shape per running-trajectory "
-                               "level)" % ("FP32" if form == 2 else "FP64"),
+                     "kernel": "interval GEMM over %s knot matrices, FP64 MFMA 16x16x4 (gemm_*_kernel: "
+                               "form and tile shape chosen per running-trajectory level)"
+                               % ("FP32" if form == 2 else "FP64"),
                      "tile": tile, "launches": la.value, "avg_launch_us": round(sec * 1e6, 2),
                      "flops_per_launch": fl.value / la.value, "bytes_per_launch": by.value / la.value,
                      "fp64_mfma_tflops": round(tflops, 3), "mfma_frac": round(f_mfma, 4),
                      "hbm_gbs": round(gbs, 1), "hbm_frac": round(f_hbm, 4),
-                     "gemm_share_of_step": round(ms.value / (pel * 1e3), 3)})
+                     "gemm_share_of_step": round(ms.value / (pel * 1e3), 3), "levels": levels})
         log("gemm: %d launches, avg %.1f us, %.2f TFLOP/s (%.1f%% of FP64 peak), %.0f GB/s (%.1f%% of HBM), "
             "%.0f%% of a %.2f s event-timed step" % (la.value, sec * 1e6, tflops, 100 * f_mfma, gbs,
                                                     100 * f_hbm, 100 * ms.value / (pel * 1e3), pel))

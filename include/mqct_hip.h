@@ -123,7 +123,9 @@ int mqct_hip_matrix_get(mqct_hip_t *h, double *mat_el, double *mat_el_der);
 
 /* Per-stage device times of the last build, milliseconds:
  * [0] basis functions on the grid, [1] potential/weights, [2] contraction,
- * [3] cutoff+spline, [4] total. */
+ * [3] cutoff+spline, [4] total.  The first build of a basis and grid in a
+ * precision form also times the contraction variants on a sample of tiles
+ * (mqct_hip_contract_variant); that time is not in [2] or [4]. */
 int mqct_hip_build_timing(mqct_hip_t *h, double *ms5);
 
 /* ---------------------------------------------------------------------- */
@@ -200,18 +202,20 @@ int mqct_hip_propagate_timing(mqct_hip_t *h, double *ms);
  * 2*S2 values each, plus 64*N bytes of amplitudes in and products out per
  * right-hand side). */
 int mqct_hip_batched_profile(mqct_hip_t *h, int on);
-/* The GEMM timing of the profiled runs since the last batched_profile
- * call, per group of event-timed iterations: rows of (running trajectories
- * at the start of the group, iterations, summed GEMM ms, shape index);
- * copies min(cap, n) rows of 4 doubles into rows and returns their number
- * in n. */
+/* The GEMM and step timing of the profiled runs since the last
+ * batched_profile call, per group of event-timed iterations: rows of
+ * (running trajectories at the start of the group, iterations, summed GEMM
+ * ms, shape index, summed step-kernel ms); copies min(cap, n) rows of 5
+ * doubles into rows and returns their number in n. */
 int mqct_hip_batched_prof_trace(mqct_hip_t *h, int cap, double *rows, int *n);
-/* The interval-GEMM candidates: shape index 0..14 (shape = -1: the shape
+/* The interval-GEMM candidates: shape index 0..16 (shape = -1: the shape
  * the last batched run started with, for the coefficient precision of that
  * run).  form: 0 LDS-staged per knot, 1 per-wave loads per knot, 2 the same
  * over the fragment-ordered knot tiles, 3 fragment-ordered work queue of
  * (knot, tile group, entry chunk) items over persistent workgroups, 4
- * LDS-staged work queue of (knot, 64-row group, entry chunk) items;
+ * LDS-staged work queue of (knot, 64-row group, entry chunk) items, 5
+ * software-pipelined work queue of (knot, m-block, entry chunk) items
+ * (m-blocks of at most 128 states);
  * entries: bucket entries per super-tile (per wave for forms 1-3); k_rows:
  * K rows per LDS slice (0 without LDS); row_tiles: 16-row tiles per wave.
  * The runner chooses a shape per level floor(log2(running trajectories)) by
@@ -222,11 +226,23 @@ int mqct_hip_batched_tile(mqct_hip_t *h, int shape, int *form, int *entries,
  * levels 0..n-1 for the FP64 (fp32=0) or FP32 (fp32=1) knot matrices. */
 int mqct_hip_batched_tile_levels(mqct_hip_t *h, int fp32, int n, int *shapes);
 /* Fixes the interval-GEMM tile shape (index into the candidate list,
- * 0..14) for the following batched runs, or -1 to choose by timing again. */
+ * 0..16) for the following batched runs, or -1 to choose by timing again. */
 int mqct_hip_batched_set_tile(mqct_hip_t *h, int shape);
 int mqct_hip_batched_kernel_stats(mqct_hip_t *h, int64_t *launches,
                                   double *gemm_ms, double *gemm_flops,
                                   double *gemm_bytes);
+
+/* Contraction kernel variants of the matrix build (no reference
+ * counterpart: INTEGRATOR_TEMP is one scalar loop).  Per precision form the
+ * library times its candidates -- grid points per LDS stage (kc) and waves
+ * per workgroup -- on a sample of tiles at the first build of a basis and
+ * grid and keeps the fastest.  set_contract_variant fixes candidate v
+ * (0..n-1) of a precision for the following builds, or -1 to time again;
+ * contract_variant returns the candidate in use (-1 none yet), its kc and
+ * waves, and the number of candidates of that precision. */
+int mqct_hip_set_contract_variant(mqct_hip_t *h, int precision, int v);
+int mqct_hip_contract_variant(mqct_hip_t *h, int precision, int *v, int *kc,
+                              int *waves, int *n_candidates);
 
 /* CC_RK4 propagation that also writes the SAVE_TRAJECT records of the first
  * AT run (propagator.f:3457-3462, 4330-4333): per trajectory up to rec_cap

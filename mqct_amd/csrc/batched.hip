@@ -65,6 +65,7 @@ struct BDev {
   int ntile;                  // 16-row tiles over all blocks
   const int *tile_blk;        // [ntile] block of the tile
   const int *tile_r0;         // [ntile] first row of the tile in its block
+  const int *blk_tile0;       // [nblk] first tile of the block
   const long long *tile_off;  // [ntile] offset of the tile in a knot's KT
   int nrg;                    // 64-row groups over all blocks
   const int *rg_blk;          // [nrg] block of the row group
@@ -188,7 +189,7 @@ __device__ void prepare_stage(const BDev &d, int t, Ctl &c, const double *xr, co
     c.c1 = (aa * aa * aa - aa) * cc5;
     c.c2 = (bb * bb * bb - bb) * cc5;
     if (!c.zero) {
-      atomicAdd(d.nbucketed, 1ULL);
+      if (d.nbucketed) atomicAdd(d.nbucketed, 1ULL);  // counted only in profiled runs
       int *cnt = d.cnt + par * nr;
       int2 *bk = d.bucket + (long long)par * nr * d.n_traj;
       const int p0 = atomicAdd(&cnt[klo], 1);
@@ -671,10 +672,167 @@ __global__ __launch_bounds__(BS) void gemm_lds_queue_kernel(BDev d, int par) {
   }
 }
 
+// The software-pipelined product as a work queue: items (knot r, m-block b,
+// chunk of GE bucket entries), in that order, so the chunks of one block at
+// one knot follow each other and meet its matrix in L2.  The whole
+// workgroup takes an item: wave w owns the block's 16-row tiles w and w + 4
+// (up to 128 rows) and keeps their fragment-ordered [Y | Y2] operands in
+// registers, one slice of GK = 16 contraction rows at a time; the weighted
+// amplitudes of the chunk pass through a double-buffered LDS slice.  The
+// next slice's matrix fragments and amplitudes are loaded while the MFMAs
+// of the current one run, so there is one barrier per slice and the loads
+// overlap the math.
+template <typename KT, int GE>
+__global__ __launch_bounds__(BS, GE >= 64 ? 1 : 2) void gemm_pipe_kernel(BDev d, int par) {
+  using F = Frag<KT>;
+  using FV = typename F::V;
+  constexpr int U = F::U, KB = 4 * U;
+  constexpr int GK = 16, LPS = GK / KB;  // contraction rows per slice, fragment loads per tile per slice
+  constexpr int GC = 2 * GE / 16;        // MFMA column tiles
+  constexpr int BW = 2 * GE + 16;        // LDS row stride (doubles)
+  constexpr int SPT = GK * GE / BS;      // (row, entry) pairs staged per thread per slice
+  static_assert(SPT * BS == GK * GE && LPS * KB == GK, "slice shape");
+  extern __shared__ int pre[];  // [n_r + 1] chunk prefix
+  __shared__ int wtot[BS / 64];
+  __shared__ double Bs[2][GK][BW];
+  __shared__ int s_t[GE], s_side[GE];
+  __shared__ double s_wy[GE], s_wy2[GE];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int q = lane >> 4, col = lane & 15;
+  const int nr = d.n_r, nblk = d.nblk;
+  const int *cnt = d.cnt + par * nr;
+  const long long M = (long long)chunk_prefix(d, par, GE, pre, wtot) * nblk;
+  const int X = 8, xcd = blockIdx.x % X, slot = blockIdx.x / X, Gx = gridDim.x / X;
+  const long long lo = M * xcd / X, hi = M * (xcd + 1) / X;
+  for (long long it = lo + slot; it < hi; it += Gx) {
+    // item -> (knot r, block b, chunk c)
+    const int r = chunk_knot(pre, nr, (int)(it / nblk)), cr = pre[r + 1] - pre[r];
+    const int loc = (int)(it - (long long)pre[r] * nblk), b = loc / cr, c = loc % cr;
+    const int p0 = d.blk_start[b], nb = d.blk_start[b + 1] - p0;
+    const int nbp = (nb + KB - 1) / KB * KB, nsl = 2 * nbp / GK;
+    const int e0 = c * GE, nE = min(GE, cnt[r] - e0), nct = (2 * nE + 15) / 16;
+    const int2 *bk = d.bucket + ((long long)par * nr + r) * d.n_traj;
+    __syncthreads();  // the previous item is done with the entry table and Bs
+    if (tid < GE) {
+      int t = 0, side = 0;
+      double wy = 0.0, wy2 = 0.0;
+      if (tid < nE) {
+        const int2 ent = bk[e0 + tid];
+        const Ctl &cc = d.ctl[ent.x];
+        t = ent.x;
+        side = ent.y;
+        wy = side ? cc.bb : cc.aa;
+        wy2 = side ? cc.c2 : cc.c1;
+      }
+      s_t[tid] = t;
+      s_side[tid] = side;
+      s_wy[tid] = wy;
+      s_wy2[tid] = wy2;
+    }
+    __syncthreads();
+    // the wave's tiles and their fragment streams
+    const int ntl = (nb + 15) / 16;
+    bool on[2];
+    const FV *A[2];
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      const int tl = wave + 4 * h;
+      on[h] = tl < ntl;
+      const int g = d.blk_tile0[b] + (on[h] ? tl : 0);
+      A[h] = reinterpret_cast<const FV *>(static_cast<const KT *>(d.KT) + (long long)r * d.KS + d.tile_off[g]) +
+             lane;
+    }
+    FV an[2][LPS];
+    double2 bn[SPT];
+    double bw[SPT];
+    // loads of slice s into registers: fragments of the owned tiles and the
+    // chunk's amplitudes for rows s GK .. s GK + GK - 1 of [Y | Y2]
+    auto load = [&](int s) {
+#pragma unroll
+      for (int h = 0; h < 2; h++)
+#pragma unroll
+        for (int l = 0; l < LPS; l++) an[h][l] = on[h] ? A[h][(s * LPS + l) * 64] : FV{};
+#pragma unroll
+      for (int j = 0; j < SPT; j++) {
+        const int idx = tid + j * BS, kk = idx % GK, e = idx / GK, k = s * GK + kk;
+        const bool y2 = k >= nbp;
+        const int kv = y2 ? k - nbp : k;
+        const bool ok = e < nE && kv < nb;
+        bn[j] = ok ? d.V[(long long)s_t[e] * d.N + p0 + kv] : make_double2(0.0, 0.0);
+        bw[j] = y2 ? s_wy2[e] : s_wy[e];
+      }
+    };
+    auto stage = [&](int buf) {
+#pragma unroll
+      for (int j = 0; j < SPT; j++) {
+        const int idx = tid + j * BS, kk = idx % GK, e = idx / GK;
+        Bs[buf][kk][2 * e] = bw[j] * bn[j].x;
+        Bs[buf][kk][2 * e + 1] = bw[j] * bn[j].y;
+      }
+    };
+    dbl4 acc[2][GC];
+#pragma unroll
+    for (int h = 0; h < 2; h++)
+#pragma unroll
+      for (int ct = 0; ct < GC; ct++) acc[h][ct] = dbl4{0.0, 0.0, 0.0, 0.0};
+    load(0);
+    stage(0);
+    __syncthreads();
+    for (int s = 0; s < nsl; s++) {
+      const int buf = s & 1;
+      FV ac[2][LPS];
+#pragma unroll
+      for (int h = 0; h < 2; h++)
+#pragma unroll
+        for (int l = 0; l < LPS; l++) ac[h][l] = an[h][l];
+      const bool more = s + 1 < nsl;
+      if (more) load(s + 1);
+#pragma unroll
+      for (int l = 0; l < LPS; l++)
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+          // fragment order: lane group q holds row KB l + U q + u at step u
+          const double *brow = &Bs[buf][KB * l + U * q + u][col];
+          double bf[GC];
+#pragma unroll
+          for (int ct = 0; ct < GC; ct++) bf[ct] = (ct < nct) ? brow[16 * ct] : 0.0;
+#pragma unroll
+          for (int h = 0; h < 2; h++) {
+            if (!on[h]) continue;
+            const double a = F::get(ac[h][l], u);
+#pragma unroll
+            for (int ct = 0; ct < GC; ct++)
+              if (ct < nct) acc[h][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bf[ct], acc[h][ct], 0, 0, 0);
+          }
+        }
+      if (more) stage(buf ^ 1);
+      __syncthreads();
+    }
+    // f64 16x16x4 output: row lane/16 + 4i, column lane%16 of each tile
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      if (!on[h]) continue;
+      const int rt0 = (wave + 4 * h) * 16;
+#pragma unroll
+      for (int ct = 0; ct < GC; ct++) {
+        if (ct >= nct) break;
+        const int cc = 16 * ct + col, e = cc >> 1;
+        if (e >= nE) continue;
+        double *wb = reinterpret_cast<double *>(d.W + ((long long)s_t[e] * 2 + s_side[e]) * d.N + p0) + (cc & 1);
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          const int orow = rt0 + q + 4 * i;
+          if (orow < nb) wb[2 * orow] = acc[h][ct][i];
+        }
+      }
+    }
+  }
+}
+
 // The GEMM forms and tile shapes the runner chooses from: form, bucket
 // entries per super-tile (per wave for the LDS-free forms), K rows per LDS
 // slice, 16-row tiles per wave.
-enum GemmForm { kLds = 0, kDirect = 1, kStream = 2, kQueue = 3, kLdsQueue = 4 };
+enum GemmForm { kLds = 0, kDirect = 1, kStream = 2, kQueue = 3, kLdsQueue = 4, kPipe = 5 };
 struct GemmShape {
   int form, ge, gk, grt;
 };
@@ -682,7 +840,8 @@ constexpr GemmShape kShapes[] = {
     {kLds, 64, 16, 1},      {kLds, 32, 16, 1},    {kLds, 32, 32, 1},    {kLds, 16, 32, 1},
     {kLds, 64, 16, 2},      {kDirect, 32, 0, 1},  {kDirect, 16, 0, 1},  {kDirect, 8, 0, 1},
     {kStream, 8, 0, 1},     {kStream, 16, 0, 1},  {kStream, 32, 0, 1},  {kQueue, 16, 0, 1},
-    {kQueue, 32, 0, 1},     {kLdsQueue, 64, 16, 1}, {kLdsQueue, 32, 16, 1}};
+    {kQueue, 32, 0, 1},     {kLdsQueue, 64, 16, 1}, {kLdsQueue, 32, 16, 1}, {kPipe, 64, 16, 2},
+    {kPipe, 32, 16, 2}};
 constexpr int kNShapes = sizeof(kShapes) / sizeof(kShapes[0]);
 // the work-queue forms keep the chunk prefix of all knots in LDS
 constexpr bool uses_queue(int shape) { return kShapes[shape].form >= kQueue; }
@@ -699,6 +858,8 @@ void launch_gemm(int shape, dim3 knots_blocks, int maxnb, hipStream_t st, const 
     case 12: gemm_queue_kernel<KT, 4><<<QGRID, BS, shq, st>>>(d, par); return;
     case 13: gemm_lds_queue_kernel<KT, 64, 16><<<QGRID, BS, shq, st>>>(d, par); return;
     case 14: gemm_lds_queue_kernel<KT, 32, 16><<<QGRID, BS, shq, st>>>(d, par); return;
+    case 15: gemm_pipe_kernel<KT, 64><<<QGRID, BS, shq, st>>>(d, par); return;
+    case 16: gemm_pipe_kernel<KT, 32><<<QGRID, BS, shq, st>>>(d, par); return;
     default: break;
   }
   auto go = [&](auto kern, int grt) {
@@ -1177,6 +1338,11 @@ int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *t
       rg_row0.push_back(r0);
     }
   const int nrg = (int)rg_blk.size();
+  if (maxnb > 128) {
+    // the pipelined form holds at most two 16-row tiles per wave (128 rows)
+    if (ds->gemm_fixed >= 0 && kShapes[ds->gemm_fixed].form == kPipe)
+      return mqct_fail(MQCT_ERR_ARG, "batched: the pipelined GEMM form takes m-blocks of at most 128 states");
+  }
   DevBuf d_tb, d_tr0, d_to, d_bt0, d_rgb, d_rgr;
   if (up(d_tb, tile_blk.data(), ntile * 4) || up(d_tr0, tile_r0.data(), ntile * 4) ||
       up(d_to, tile_off.data(), ntile * 8) || up(d_bt0, blk_tile0.data(), nblk * 4) ||
@@ -1229,6 +1395,7 @@ int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *t
   d.ntile = ntile;
   d.tile_blk = d_tb.as<int>();
   d.tile_r0 = d_tr0.as<int>();
+  d.blk_tile0 = d_bt0.as<int>();
   d.tile_off = d_to.as<long long>();
   d.nrg = nrg;
   d.rg_blk = d_rgb.as<int>();
@@ -1248,7 +1415,7 @@ int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *t
   d.cnt = reinterpret_cast<int *>(base + off_cnt);
   d.bucket = reinterpret_cast<int2 *>(base + off_bk);
   d.active = reinterpret_cast<int *>(base + off_act);
-  d.nbucketed = reinterpret_cast<unsigned long long *>(base + off_act + 8);
+  d.nbucketed = ds->prof_on ? reinterpret_cast<unsigned long long *>(base + off_act + 8) : nullptr;
   d.ntouched = reinterpret_cast<unsigned long long *>(base + off_act + 16);
   HIP_TRY(hipMemsetAsync(d.cnt, 0, (size_t)2 * n_r * 4, h->stream));
   HIP_TRY(hipMemsetAsync(d.active, 0, 32, h->stream));
@@ -1276,35 +1443,40 @@ int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *t
   int active = n_traj;
   long long it_done = 0;  // iterations run so far (parity of the next one)
   if (ds->prof_ev.empty()) {
-    ds->prof_ev.resize(2 * ITER);
+    ds->prof_ev.resize(3 * ITER);
     for (auto &e : ds->prof_ev) HIP_TRY(hipEventCreate(&e));
   }
   // n_it iterations launched one by one with HIP events on this stream
-  // around every GEMM launch; returns the summed GEMM time
+  // around every GEMM and step launch; returns the summed GEMM time
   auto timed_iters = [&](int sh, int n_it, double &gemm_ms) -> int {
     const int active0 = active;
     for (int it = 0; it < n_it; it++) {
       const int par = (int)((it_done + it) & 1);
-      HIP_TRY(hipEventRecord(ds->prof_ev[2 * it], h->stream));
+      HIP_TRY(hipEventRecord(ds->prof_ev[3 * it], h->stream));
       gemm(sh, par);
-      HIP_TRY(hipEventRecord(ds->prof_ev[2 * it + 1], h->stream));
+      HIP_TRY(hipEventRecord(ds->prof_ev[3 * it + 1], h->stream));
       step(par ^ 1);
+      HIP_TRY(hipEventRecord(ds->prof_ev[3 * it + 2], h->stream));
     }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(&active, d.active, 4, hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
     gemm_ms = 0.0;
+    double step_ms = 0.0;
     for (int it = 0; it < n_it; it++) {
-      float ems = 0.f;
-      HIP_TRY(hipEventElapsedTime(&ems, ds->prof_ev[2 * it], ds->prof_ev[2 * it + 1]));
+      float ems = 0.f, sms = 0.f;
+      HIP_TRY(hipEventElapsedTime(&ems, ds->prof_ev[3 * it], ds->prof_ev[3 * it + 1]));
+      HIP_TRY(hipEventElapsedTime(&sms, ds->prof_ev[3 * it + 1], ds->prof_ev[3 * it + 2]));
       gemm_ms += ems;
+      step_ms += sms;
     }
     it_done += n_it;
     if (prof) {
       ds->prof_ms += gemm_ms;
       ds->prof_launches += n_it;
       if (ds->prof_trace.size() < (1u << 22))
-        ds->prof_trace.insert(ds->prof_trace.end(), {(double)active0, (double)n_it, gemm_ms, (double)sh});
+        ds->prof_trace.insert(ds->prof_trace.end(),
+                              {(double)active0, (double)n_it, gemm_ms, (double)sh, step_ms});
     }
     return MQCT_OK;
   };
@@ -1339,7 +1511,8 @@ int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *t
       const int ncand = kNShapes;
       double tot[kNShapes] = {};
       bool keep[kNShapes];
-      for (int s = 0; s < kNShapes; s++) keep[s] = queue_ok || !uses_queue(s);
+      for (int s = 0; s < kNShapes; s++)
+        keep[s] = (queue_ok || !uses_queue(s)) && (kShapes[s].form != kPipe || maxnb <= 128);
       for (int round = 0; round < 3 && active > 0; round++) {
         for (int s = 0; s < ncand && active > 0; s++) {
           if (!keep[s]) continue;
@@ -1419,7 +1592,7 @@ int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *t
   if (active > 0) return mqct_fail(MQCT_ERR_HIP, "batched: trajectories did not finish");
   if (prof) {
     unsigned long long cnts[2] = {0, 0};
-    HIP_TRY(hipMemcpy(cnts, d.nbucketed, 16, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(cnts, base + off_act + 8, 16, hipMemcpyDeviceToHost));
     // per bucketed right-hand side: 2 sides x 2 matrices x nb x nb x (Re, Im)
     // multiply-adds summed over the blocks = 16 S2 flops; it reads its
     // rotated amplitudes for both entries and writes both partial products
@@ -1455,8 +1628,8 @@ extern "C" int mqct_hip_batched_profile(mqct_hip_t *h, int on) {
 extern "C" int mqct_hip_batched_prof_trace(mqct_hip_t *h, int cap, double *rows, int *n) {
   if (!h || !h->dyn || !n || cap < 0 || (cap > 0 && !rows)) return mqct_fail(MQCT_ERR_ARG, "batched_prof_trace: bad argument");
   const auto &tr = h->dyn->prof_trace;
-  *n = (int)(tr.size() / 4);
-  const size_t m = std::min((size_t)cap, tr.size() / 4) * 4;
+  *n = (int)(tr.size() / 5);
+  const size_t m = std::min((size_t)cap, tr.size() / 5) * 5;
   std::copy(tr.begin(), tr.begin() + m, rows);
   return MQCT_OK;
 }

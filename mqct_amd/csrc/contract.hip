@@ -20,6 +20,7 @@
 //          sums per chunk, FP64 accumulators.
 #include "mqct_internal.h"
 
+#include <chrono>
 #include <type_traits>
 
 namespace {
@@ -185,8 +186,8 @@ template <typename T> struct Vec16;
 template <> struct Vec16<double> { typedef double2 type; };
 template <> struct Vec16<float> { typedef float4 type; };
 
-template <typename T, int KC, int NRT>
-__global__ __launch_bounds__(256) void contract_v_kernel(
+template <typename T, int KC, int NRT, int NW = WAVES>
+__global__ __launch_bounds__(64 * NW) void contract_v_kernel(
     const T *__restrict__ pa_re, const T *__restrict__ pa_im,
     const T *__restrict__ pb_re, const T *__restrict__ pb_im,
     const T *__restrict__ W, long long G, int n_r,
@@ -196,9 +197,10 @@ __global__ __launch_bounds__(256) void contract_v_kernel(
   constexpr int KV = KC / VW;                         // vectors per staged row
   constexpr int KP = KC + (sizeof(T) == 8 ? 2 : 4);   // padded row: 16-B aligned, conflict-free reads
   constexpr int NTT = NRT / 16;
-  constexpr int PA = 2 * TA * KV / 256;               // A (and B) vectors per thread per chunk
-  constexpr int PW = NRT * KV / 256;                  // W vectors per thread per chunk
-  static_assert(PA * 256 == 2 * TA * KV && PW * 256 == NRT * KV, "staging must tile the workgroup");
+  constexpr int BT = 64 * NW, AWn = TA / NW;        // threads; a-states per wave
+  constexpr int PA = 2 * TA * KV / BT;               // A (and B) vectors per thread per chunk
+  constexpr int PW = NRT * KV / BT;                  // W vectors per thread per chunk
+  static_assert(PA * BT == 2 * TA * KV && PW * BT == NRT * KV && AWn * NW == TA, "staging must tile the workgroup");
   __shared__ __attribute__((aligned(16))) T sA[2][2][TA][KP];
   __shared__ __attribute__((aligned(16))) T sB[2][2][TB][KP];
   __shared__ __attribute__((aligned(16))) T sW[2][NRT][KP];
@@ -206,13 +208,16 @@ __global__ __launch_bounds__(256) void contract_v_kernel(
   const int r0 = blockIdx.y * NRT;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const bool has_im = (pa_im != nullptr);
-  const T *pa[2] = {pa_re, pa_im}, *pb[2] = {pb_re, pb_im};
+  // operand planes chosen by select, not by indexing a private array
+  // (which would live in scratch memory)
+  auto pa = [&](int plane) { return plane ? pa_im : pa_re; };
+  auto pb = [&](int plane) { return plane ? pb_im : pb_re; };
 
   typedef typename std::conditional<sizeof(T) == 8, dbl4, flt4>::type accv;
-  accv acc[AW][NTT];
-  dbl4 acc64[AW][NTT];
+  accv acc[AWn][NTT];
+  dbl4 acc64[AWn][NTT];
 #pragma unroll
-  for (int a = 0; a < AW; a++)
+  for (int a = 0; a < AWn; a++)
 #pragma unroll
     for (int n = 0; n < NTT; n++) {
       acc[a][n] = accv{0, 0, 0, 0};
@@ -223,16 +228,16 @@ __global__ __launch_bounds__(256) void contract_v_kernel(
   auto load = [&](long long g0) {
 #pragma unroll
     for (int j = 0; j < PA; j++) {
-      const int idx = tid + 256 * j, plane = idx / (TA * KV), rem = idx % (TA * KV);
+      const int idx = tid + BT * j, plane = idx / (TA * KV), rem = idx % (TA * KV);
       const int row = rem / KV, kv = rem % KV;
       const long long g = g0 + (long long)kv * VW;
       const bool okp = (plane == 0 || has_im) && g < G;
-      ra[j] = (okp && row < t.na) ? *reinterpret_cast<const V *>(pa[plane] + (long long)(t.a0 + row) * G + g) : vz;
-      rb[j] = (okp && row < t.nb) ? *reinterpret_cast<const V *>(pb[plane] + (long long)(t.b0 + row) * G + g) : vz;
+      ra[j] = (okp && row < t.na) ? *reinterpret_cast<const V *>(pa(plane) + (long long)(t.a0 + row) * G + g) : vz;
+      rb[j] = (okp && row < t.nb) ? *reinterpret_cast<const V *>(pb(plane) + (long long)(t.b0 + row) * G + g) : vz;
     }
 #pragma unroll
     for (int j = 0; j < PW; j++) {
-      const int idx = tid + 256 * j, row = idx / KV, kv = idx % KV;
+      const int idx = tid + BT * j, row = idx / KV, kv = idx % KV;
       const long long g = g0 + (long long)kv * VW;
       const int rr = r0 + row;
       rw[j] = (g < G && rr < n_r) ? *reinterpret_cast<const V *>(W + (long long)rr * G + g) : vz;
@@ -241,14 +246,14 @@ __global__ __launch_bounds__(256) void contract_v_kernel(
   auto store = [&](int buf) {
 #pragma unroll
     for (int j = 0; j < PA; j++) {
-      const int idx = tid + 256 * j, plane = idx / (TA * KV), rem = idx % (TA * KV);
+      const int idx = tid + BT * j, plane = idx / (TA * KV), rem = idx % (TA * KV);
       const int row = rem / KV, kv = rem % KV;
       *reinterpret_cast<V *>(&sA[buf][plane][row][kv * VW]) = ra[j];
       *reinterpret_cast<V *>(&sB[buf][plane][row][kv * VW]) = rb[j];
     }
 #pragma unroll
     for (int j = 0; j < PW; j++) {
-      const int idx = tid + 256 * j, row = idx / KV, kv = idx % KV;
+      const int idx = tid + BT * j, row = idx / KV, kv = idx % KV;
       *reinterpret_cast<V *>(&sW[buf][row][kv * VW]) = rw[j];
     }
   };
@@ -270,8 +275,8 @@ __global__ __launch_bounds__(256) void contract_v_kernel(
 #pragma unroll
       for (int n = 0; n < NTT; n++) wv[n] = sW[buf][n * 16 + row][k];
 #pragma unroll
-      for (int a = 0; a < AW; a++) {
-        const int al = wave * AW + a;
+      for (int a = 0; a < AWn; a++) {
+        const int al = wave * AWn + a;
         T av = sA[buf][0][al][k] * br;
         if (has_im) av += sA[buf][1][al][k] * bi;
 #pragma unroll
@@ -285,7 +290,7 @@ __global__ __launch_bounds__(256) void contract_v_kernel(
     }
     if constexpr (sizeof(T) == 4) {
 #pragma unroll
-      for (int a = 0; a < AW; a++)
+      for (int a = 0; a < AWn; a++)
 #pragma unroll
         for (int n = 0; n < NTT; n++)
 #pragma unroll
@@ -298,14 +303,14 @@ __global__ __launch_bounds__(256) void contract_v_kernel(
     __syncthreads();
   }
 #pragma unroll
-  for (int a = 0; a < AW; a++)
+  for (int a = 0; a < AWn; a++)
 #pragma unroll
     for (int n = 0; n < NTT; n++) {
       double v[4];
 #pragma unroll
       for (int i = 0; i < 4; i++)
         v[i] = (sizeof(T) == 8) ? (double)acc[a][n][i] : acc64[a][n][i];
-      store_tile<sizeof(T) == 8>(om, t, wave * AW + a, r0, n, lane, v, out);
+      store_tile<sizeof(T) == 8>(om, t, wave * AWn + a, r0, n, lane, v, out);
     }
 }
 
@@ -434,8 +439,8 @@ __global__ __launch_bounds__(256) void contract_bf16x3_kernel(
 
 // The three-term split-BF16 contraction with 16-byte staging and a
 // double-buffered LDS (needs G to be a multiple of 8).
-template <int KC>
-__global__ __launch_bounds__(256) void contract_bf16x3_v_kernel(
+template <int KC, int NW = WAVES, int NRB = NR>
+__global__ __launch_bounds__(64 * NW) void contract_bf16x3_v_kernel(
     const float *__restrict__ pa_re, const float *__restrict__ pa_im,
     const float *__restrict__ pb_re, const float *__restrict__ pb_im,
     const __bf16 *__restrict__ W0, const __bf16 *__restrict__ W1, const __bf16 *__restrict__ W2,
@@ -445,24 +450,27 @@ __global__ __launch_bounds__(256) void contract_bf16x3_v_kernel(
   constexpr int KPW = KC + 8;            // bf16 rows
   constexpr int KV = KC / 4;             // float4 per fp32 row
   constexpr int KVW = KC / 8;            // 8 x bf16 per weight row
-  constexpr int PA = 2 * TA * KV / 256;  // A (and B) vectors per thread
-  constexpr int PW = 3 * NR * KVW / 256; // weight vectors per thread
-  static_assert(PA * 256 == 2 * TA * KV && PW * 256 == 3 * NR * KVW, "staging must tile the workgroup");
+  constexpr int BT = 64 * NW, AWn = TA / NW;  // threads; a-states per wave
+  constexpr int NTB = NRB / 16;               // R column tiles per workgroup
+  constexpr int PA = 2 * TA * KV / BT;        // A (and B) vectors per thread
+  constexpr int PW = 3 * NRB * KVW / BT;       // weight vectors per thread
+  static_assert(PA * BT == 2 * TA * KV && PW * BT == 3 * NRB * KVW && AWn * NW == TA, "staging must tile the workgroup");
   __shared__ __attribute__((aligned(16))) float sA[2][2][TA][KP];
   __shared__ __attribute__((aligned(16))) float sB[2][2][TB][KP];
-  __shared__ __attribute__((aligned(16))) __bf16 sW[2][3][NR][KPW];
+  __shared__ __attribute__((aligned(16))) __bf16 sW[2][3][NRB][KPW];
   const TileDesc t = tiles[blockIdx.x];
-  const int r0 = blockIdx.y * NR;
+  const int r0 = blockIdx.y * NRB;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const bool has_im = (pa_im != nullptr);
-  const float *pa[2] = {pa_re, pa_im}, *pb[2] = {pb_re, pb_im};
-  const __bf16 *Wp[3] = {W0, W1, W2};
-  flt4 acc[AW][NT];
-  dbl4 acc64[AW][NT];
+  auto pa = [&](int plane) { return plane ? pa_im : pa_re; };
+  auto pb = [&](int plane) { return plane ? pb_im : pb_re; };
+  auto Wp = [&](int q) { return q == 0 ? W0 : (q == 1 ? W1 : W2); };
+  flt4 acc[AWn][NTB];
+  dbl4 acc64[AWn][NTB];
 #pragma unroll
-  for (int a = 0; a < AW; a++)
+  for (int a = 0; a < AWn; a++)
 #pragma unroll
-    for (int n = 0; n < NT; n++) {
+    for (int n = 0; n < NTB; n++) {
       acc[a][n] = flt4{0, 0, 0, 0};
       acc64[a][n] = dbl4{0, 0, 0, 0};
     }
@@ -471,36 +479,36 @@ __global__ __launch_bounds__(256) void contract_bf16x3_v_kernel(
   auto load = [&](long long g0) {
 #pragma unroll
     for (int j = 0; j < PA; j++) {
-      const int idx = tid + 256 * j, plane = idx / (TA * KV), rem = idx % (TA * KV);
+      const int idx = tid + BT * j, plane = idx / (TA * KV), rem = idx % (TA * KV);
       const int row = rem / KV, kv = rem % KV;
       const long long g = g0 + (long long)kv * 4;
       const bool okp = (plane == 0 || has_im) && g < G;
-      ra[j] = (okp && row < t.na) ? *reinterpret_cast<const float4 *>(pa[plane] + (long long)(t.a0 + row) * G + g)
+      ra[j] = (okp && row < t.na) ? *reinterpret_cast<const float4 *>(pa(plane) + (long long)(t.a0 + row) * G + g)
                                   : float4{0, 0, 0, 0};
-      rb[j] = (okp && row < t.nb) ? *reinterpret_cast<const float4 *>(pb[plane] + (long long)(t.b0 + row) * G + g)
+      rb[j] = (okp && row < t.nb) ? *reinterpret_cast<const float4 *>(pb(plane) + (long long)(t.b0 + row) * G + g)
                                   : float4{0, 0, 0, 0};
     }
 #pragma unroll
     for (int j = 0; j < PW; j++) {
-      const int idx = tid + 256 * j, q = idx / (NR * KVW), rem = idx % (NR * KVW);
+      const int idx = tid + BT * j, q = idx / (NRB * KVW), rem = idx % (NRB * KVW);
       const int row = rem / KVW, kv = rem % KVW;
       const long long g = g0 + (long long)kv * 8;
       const int rr = r0 + row;
-      rw[j] = (g < G && rr < n_r) ? *reinterpret_cast<const uint4 *>(Wp[q] + (long long)rr * G + g)
+      rw[j] = (g < G && rr < n_r) ? *reinterpret_cast<const uint4 *>(Wp(q) + (long long)rr * G + g)
                                   : uint4{0, 0, 0, 0};
     }
   };
   auto store = [&](int buf) {
 #pragma unroll
     for (int j = 0; j < PA; j++) {
-      const int idx = tid + 256 * j, plane = idx / (TA * KV), rem = idx % (TA * KV);
+      const int idx = tid + BT * j, plane = idx / (TA * KV), rem = idx % (TA * KV);
       const int row = rem / KV, kv = rem % KV;
       *reinterpret_cast<float4 *>(&sA[buf][plane][row][kv * 4]) = ra[j];
       *reinterpret_cast<float4 *>(&sB[buf][plane][row][kv * 4]) = rb[j];
     }
 #pragma unroll
     for (int j = 0; j < PW; j++) {
-      const int idx = tid + 256 * j, q = idx / (NR * KVW), rem = idx % (NR * KVW);
+      const int idx = tid + BT * j, q = idx / (NRB * KVW), rem = idx % (NRB * KVW);
       const int row = rem / KVW, kv = rem % KVW;
       *reinterpret_cast<uint4 *>(&sW[buf][q][row][kv * 8]) = rw[j];
     }
@@ -523,8 +531,8 @@ __global__ __launch_bounds__(256) void contract_bf16x3_v_kernel(
         bi[j] = has_im ? sB[buf][1][row][k0 + j] : 0.0f;
       }
 #pragma unroll
-      for (int a = 0; a < AW; a++) {
-        const int al = wave * AW + a;
+      for (int a = 0; a < AWn; a++) {
+        const int al = wave * AWn + a;
         bf16x8 p0, p1, p2;
 #pragma unroll
         for (int j = 0; j < 8; j++) {
@@ -537,7 +545,7 @@ __global__ __launch_bounds__(256) void contract_bf16x3_v_kernel(
           p2[j] = h2;
         }
 #pragma unroll
-        for (int n = 0; n < NT; n++) {
+        for (int n = 0; n < NTB; n++) {
           const bf16x8 w0 = *reinterpret_cast<const bf16x8 *>(&sW[buf][0][n * 16 + row][k0]);
           const bf16x8 w1 = *reinterpret_cast<const bf16x8 *>(&sW[buf][1][n * 16 + row][k0]);
           const bf16x8 w2 = *reinterpret_cast<const bf16x8 *>(&sW[buf][2][n * 16 + row][k0]);
@@ -551,9 +559,9 @@ __global__ __launch_bounds__(256) void contract_bf16x3_v_kernel(
       }
     }
 #pragma unroll
-    for (int a = 0; a < AW; a++)
+    for (int a = 0; a < AWn; a++)
 #pragma unroll
-      for (int n = 0; n < NT; n++)
+      for (int n = 0; n < NTB; n++)
 #pragma unroll
         for (int i = 0; i < 4; i++) {
           acc64[a][n][i] += (double)acc[a][n][i];
@@ -563,14 +571,73 @@ __global__ __launch_bounds__(256) void contract_bf16x3_v_kernel(
     __syncthreads();
   }
 #pragma unroll
-  for (int a = 0; a < AW; a++)
+  for (int a = 0; a < AWn; a++)
 #pragma unroll
-    for (int n = 0; n < NT; n++) {
+    for (int n = 0; n < NTB; n++) {
       double v[4];
 #pragma unroll
       for (int i = 0; i < 4; i++) v[i] = acc64[a][n][i];
-      store_tile<false>(om, t, wave * AW + a, r0, n, lane, v, out);
+      store_tile<false>(om, t, wave * AWn + a, r0, n, lane, v, out);
     }
+}
+
+// ---------------------------------------------------------------------------
+// FP64 sums of one tile's pairs at the two cutoff R points on the vector
+// ALU: one thread per (a, b) pair of the 16 x 16 tile.  Basis rows and the
+// two weight rows pass through LDS in chunks of CK grid points, read two
+// points at a time as 16-byte vectors.  Two output columns leave the MFMA
+// tiles almost empty, while here every pair product feeds both sums at once.
+template <int CK>
+__global__ __launch_bounds__(256) void cutoff_fp64_kernel(
+    const double *__restrict__ pa_re, const double *__restrict__ pa_im,
+    const double *__restrict__ pb_re, const double *__restrict__ pb_im,
+    const double *__restrict__ Wc, long long G, const TileDesc *__restrict__ tiles, OutMap om,
+    double *__restrict__ out) {
+  constexpr int CP = CK + 2;  // row stride: 16-byte aligned rows
+  __shared__ __attribute__((aligned(16))) double sA[2][TA][CP], sB[2][TB][CP], sW[2][CK];
+  const TileDesc t = tiles[blockIdx.x];
+  const int tid = threadIdx.x, al = tid >> 4, bl = tid & 15;
+  const bool has_im = pa_im != nullptr;
+  double s0 = 0.0, s1 = 0.0;
+  for (long long g0 = 0; g0 < G; g0 += CK) {
+    for (int e = tid; e < TA * CK; e += 256) {
+      const int r = e / CK, c = e % CK;
+      const long long g = g0 + c;
+      const bool ok = g < G, oka = ok && r < t.na, okb = ok && r < t.nb;
+      const long long ia = (long long)(t.a0 + r) * G + g, ib = (long long)(t.b0 + r) * G + g;
+      sA[0][r][c] = oka ? pa_re[ia] : 0.0;
+      sB[0][r][c] = okb ? pb_re[ib] : 0.0;
+      sA[1][r][c] = (oka && has_im) ? pa_im[ia] : 0.0;
+      sB[1][r][c] = (okb && has_im) ? pb_im[ib] : 0.0;
+    }
+    for (int e = tid; e < 2 * CK; e += 256) {
+      const int q = e / CK, c = e % CK;
+      const long long g = g0 + c;
+      sW[q][c] = g < G ? Wc[q * G + g] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int c = 0; c < CK; c += 2) {
+      const double2 ar = *reinterpret_cast<const double2 *>(&sA[0][al][c]);
+      const double2 ai = *reinterpret_cast<const double2 *>(&sA[1][al][c]);
+      const double2 br = *reinterpret_cast<const double2 *>(&sB[0][bl][c]);
+      const double2 bi = *reinterpret_cast<const double2 *>(&sB[1][bl][c]);
+      const double2 w0 = *reinterpret_cast<const double2 *>(&sW[0][c]);
+      const double2 w1 = *reinterpret_cast<const double2 *>(&sW[1][c]);
+      const double px = ar.x * br.x + ai.x * bi.x, py = ar.y * br.y + ai.y * bi.y;
+      s0 += w0.x * px;
+      s1 += w1.x * px;
+      s0 += w0.y * py;
+      s1 += w1.y * py;
+    }
+    __syncthreads();
+  }
+  if (al >= t.na || bl >= t.nb) return;
+  const int a = t.a0 + al, b = t.b0 + bl;
+  if (b > a) return;  // lower triangle only
+  const long long k = om.kbase[om.pos_state[a]] + (b - om.block_of[a]);
+  out[k * om.ld] = s0 * om.conv;
+  out[k * om.ld + 1] = s1 * om.conv;
 }
 
 // ---------------------------------------------------------------------------
@@ -644,57 +711,121 @@ static OutMap out_map(mqct_hip *h, int n_cols, int ld, double conv) {
   return om;
 }
 
+// Contraction variants per precision form: grid points per LDS stage and
+// waves per workgroup (each wave owns TA / waves a-states of the tile).
+struct CVariant {
+  int kc, nw, nr;  // grid points per stage, waves, R points per workgroup
+};
+constexpr CVariant kCVar[3][4] = {{{32, 4, 64}, {32, 8, 64}, {32, 8, 128}, {0, 0, 0}},  // FP64
+                                  {{64, 4, 64}, {64, 8, 64}, {32, 4, 64}, {64, 8, 128}},  // FP32
+                                  {{64, 4, 64}, {64, 8, 64}, {32, 4, 64}, {0, 0, 0}}};    // split-BF16
+constexpr int kNCVar[3] = {3, 4, 3};
+
+// One launch of variant v of a precision form over tiles [0, ntiles).
+static void launch_contraction(mqct_hip *h, int precision, int v, unsigned ntiles, const OutMap &om, double *out) {
+  const int nrv = (v >= 0 && kCVar[precision][v].nr > 0) ? kCVar[precision][v].nr : NR;
+  const dim3 grid(ntiles, (unsigned)((h->n_r + nrv - 1) / nrv));
+  const dim3 grid64(ntiles, (unsigned)((h->n_r + NR - 1) / NR));  // the fallback kernels take 64 R
+  const bool im = (h->coll_type != 1) || !h->ref_xz;
+  hipStream_t st = h->stream;
+  const TileDesc *tl = h->d_tiles.as<TileDesc>();
+  if (precision == MQCT_PREC_FP64) {
+    const double *ar = h->d_psi_re.as<double>(), *ai = im ? h->d_psi_im.as<double>() : nullptr;
+    const double *br = h->ref_xz ? h->d_psi_re.as<double>() : h->d_psib_re.as<double>();
+    const double *bi = im ? (h->ref_xz ? h->d_psi_im.as<double>() : h->d_psib_im.as<double>()) : nullptr;
+    const double *W = h->d_W64.as<double>();
+    if (h->G % 2 != 0) {
+      contract_k4_kernel<double, 32, 64><<<grid64, 256, 0, st>>>(ar, ai, br, bi, W, h->G, h->n_r, tl, om, out);
+      return;
+    }
+    if (v == 1) contract_v_kernel<double, 32, 64, 8><<<grid, 512, 0, st>>>(ar, ai, br, bi, W, h->G, h->n_r, tl, om, out);
+    else if (v == 2) contract_v_kernel<double, 32, 128, 8><<<grid, 512, 0, st>>>(ar, ai, br, bi, W, h->G, h->n_r, tl, om, out);
+    else contract_v_kernel<double, 32, 64, 4><<<grid, 256, 0, st>>>(ar, ai, br, bi, W, h->G, h->n_r, tl, om, out);
+    return;
+  }
+  const float *ar = h->d_psi32_re.as<float>(), *ai = im ? h->d_psi32_im.as<float>() : nullptr;
+  const float *br = h->ref_xz ? h->d_psi32_re.as<float>() : h->d_psib32_re.as<float>();
+  const float *bi = im ? (h->ref_xz ? h->d_psi32_im.as<float>() : h->d_psib32_im.as<float>()) : nullptr;
+  if (precision == MQCT_PREC_FP32) {
+    const float *W = h->d_W32.as<float>();
+    if (h->G % 4 != 0) {
+      contract_k4_kernel<float, 64, 64><<<grid64, 256, 0, st>>>(ar, ai, br, bi, W, h->G, h->n_r, tl, om, out);
+      return;
+    }
+    if (v == 1) contract_v_kernel<float, 64, 64, 8><<<grid, 512, 0, st>>>(ar, ai, br, bi, W, h->G, h->n_r, tl, om, out);
+    else if (v == 2) contract_v_kernel<float, 32, 64, 4><<<grid, 256, 0, st>>>(ar, ai, br, bi, W, h->G, h->n_r, tl, om, out);
+    else if (v == 3) contract_v_kernel<float, 64, 128, 8><<<grid, 512, 0, st>>>(ar, ai, br, bi, W, h->G, h->n_r, tl, om, out);
+    else contract_v_kernel<float, 64, 64, 4><<<grid, 256, 0, st>>>(ar, ai, br, bi, W, h->G, h->n_r, tl, om, out);
+    return;
+  }
+  const __bf16 *w0 = h->d_Whi.as<__bf16>(), *w1 = h->d_Wlo.as<__bf16>(), *w2 = h->d_Wlo2.as<__bf16>();
+  if (h->G % 8 != 0) {
+    contract_bf16x3_kernel<64><<<grid64, 256, 0, st>>>(ar, ai, br, bi, w0, w1, w2, h->G, h->n_r, tl, om, out);
+    return;
+  }
+  if (v == 1) contract_bf16x3_v_kernel<64, 8><<<grid, 512, 0, st>>>(ar, ai, br, bi, w0, w1, w2, h->G, h->n_r, tl, om, out);
+  else if (v == 2) contract_bf16x3_v_kernel<32, 4><<<grid, 256, 0, st>>>(ar, ai, br, bi, w0, w1, w2, h->G, h->n_r, tl, om, out);
+  else contract_bf16x3_v_kernel<64, 4><<<grid, 256, 0, st>>>(ar, ai, br, bi, w0, w1, w2, h->G, h->n_r, tl, om, out);
+}
+
 int mqct_run_contraction(mqct_hip *h, int precision, double conv) {
   int ntiles = 0, rc;
   if ((rc = ensure_tiles(h, ntiles))) return rc;
   if (ntiles == 0) return MQCT_OK;
+  if (precision < 0 || precision > 2) return mqct_fail(MQCT_ERR_ARG, "contraction: precision");
   OutMap om = out_map(h, h->n_r, h->n_r, conv);
-  const bool im = (h->coll_type != 1) || !h->ref_xz;
   double *out = h->d_mat.as<double>();
-  if (precision == MQCT_PREC_FP64) {
-    dim3 grid((unsigned)ntiles, (unsigned)((h->n_r + NR - 1) / NR));
-    const double *pbr = h->ref_xz ? h->d_psi_re.as<double>() : h->d_psib_re.as<double>();
-    const double *pbi = h->ref_xz ? h->d_psi_im.as<double>() : h->d_psib_im.as<double>();
-    if (h->G % 2 == 0)
-      contract_v_kernel<double, 32, 64><<<grid, 256, 0, h->stream>>>(
-          h->d_psi_re.as<double>(), im ? h->d_psi_im.as<double>() : nullptr,
-          pbr, im ? pbi : nullptr, h->d_W64.as<double>(), h->G, h->n_r,
-          h->d_tiles.as<TileDesc>(), om, out);
-    else
-      contract_k4_kernel<double, 32, 64><<<grid, 256, 0, h->stream>>>(
-          h->d_psi_re.as<double>(), im ? h->d_psi_im.as<double>() : nullptr,
-          pbr, im ? pbi : nullptr, h->d_W64.as<double>(), h->G, h->n_r,
-          h->d_tiles.as<TileDesc>(), om, out);
-  } else if (precision == MQCT_PREC_FP32) {
-    dim3 grid((unsigned)ntiles, (unsigned)((h->n_r + NR - 1) / NR));
-    const float *pbr = h->ref_xz ? h->d_psi32_re.as<float>() : h->d_psib32_re.as<float>();
-    const float *pbi = h->ref_xz ? h->d_psi32_im.as<float>() : h->d_psib32_im.as<float>();
-    if (h->G % 4 == 0)
-      contract_v_kernel<float, 64, 64><<<grid, 256, 0, h->stream>>>(
-          h->d_psi32_re.as<float>(), im ? h->d_psi32_im.as<float>() : nullptr,
-          pbr, im ? pbi : nullptr, h->d_W32.as<float>(), h->G, h->n_r,
-          h->d_tiles.as<TileDesc>(), om, out);
-    else
-      contract_k4_kernel<float, 64, 64><<<grid, 256, 0, h->stream>>>(
-          h->d_psi32_re.as<float>(), im ? h->d_psi32_im.as<float>() : nullptr,
-          pbr, im ? pbi : nullptr, h->d_W32.as<float>(), h->G, h->n_r,
-          h->d_tiles.as<TileDesc>(), om, out);
-  } else {
-    dim3 grid((unsigned)ntiles, (unsigned)((h->n_r + NR - 1) / NR));
-    const float *pbr = h->ref_xz ? h->d_psi32_re.as<float>() : h->d_psib32_re.as<float>();
-    const float *pbi = h->ref_xz ? h->d_psi32_im.as<float>() : h->d_psib32_im.as<float>();
-    if (h->G % 8 == 0)
-      contract_bf16x3_v_kernel<64><<<grid, 256, 0, h->stream>>>(
-          h->d_psi32_re.as<float>(), im ? h->d_psi32_im.as<float>() : nullptr,
-          pbr, im ? pbi : nullptr, h->d_Whi.as<__bf16>(), h->d_Wlo.as<__bf16>(),
-          h->d_Wlo2.as<__bf16>(), h->G, h->n_r, h->d_tiles.as<TileDesc>(), om, out);
-    else
-      contract_bf16x3_kernel<64><<<grid, 256, 0, h->stream>>>(
-          h->d_psi32_re.as<float>(), im ? h->d_psi32_im.as<float>() : nullptr,
-          pbr, im ? pbi : nullptr, h->d_Whi.as<__bf16>(), h->d_Wlo.as<__bf16>(),
-          h->d_Wlo2.as<__bf16>(), h->G, h->n_r, h->d_tiles.as<TileDesc>(), om, out);
+  int v = h->cvar_fixed[precision];
+  if (v < 0) v = h->cvar[precision];
+  h->tune_ms = 0.0;
+  if (v < 0) {
+    const auto t0 = std::chrono::steady_clock::now();
+    // time every candidate twice on a sample of the tiles (their outputs are
+    // written again by the full run below) and keep the fastest
+    const unsigned nsub = (unsigned)(ntiles < 512 ? ntiles : 512);
+    float best = 0.f;
+    for (int c = 0; c < kNCVar[precision]; c++) {
+      float tmin = 0.f;
+      for (int rep = 0; rep < 2; rep++) {
+        HIP_TRY(hipEventRecord(h->ev[6], h->stream));
+        launch_contraction(h, precision, c, nsub, om, out);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipEventRecord(h->ev[7], h->stream));
+        HIP_TRY(hipEventSynchronize(h->ev[7]));
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, h->ev[6], h->ev[7]));
+        if (rep == 0 || ms < tmin) tmin = ms;
+      }
+      if (v < 0 || tmin < best) {
+        v = c;
+        best = tmin;
+      }
+    }
+    h->cvar[precision] = v;
+    h->tune_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   }
+  launch_contraction(h, precision, v, (unsigned)ntiles, om, out);
   HIP_TRY(hipGetLastError());
+  return MQCT_OK;
+}
+
+extern "C" int mqct_hip_set_contract_variant(mqct_hip_t *h, int precision, int v) {
+  if (!h || precision < 0 || precision > 2 || v < -1 || v >= kNCVar[precision < 0 || precision > 2 ? 0 : precision])
+    return mqct_fail(MQCT_ERR_ARG, "set_contract_variant: bad argument");
+  h->cvar_fixed[precision] = v;
+  if (v < 0) h->cvar[precision] = -1;
+  return MQCT_OK;
+}
+
+extern "C" int mqct_hip_contract_variant(mqct_hip_t *h, int precision, int *v, int *kc, int *waves,
+                                         int *n_candidates) {
+  if (!h || precision < 0 || precision > 2 || !v || !kc || !waves || !n_candidates)
+    return mqct_fail(MQCT_ERR_ARG, "contract_variant: bad argument");
+  const int c = h->cvar_fixed[precision] >= 0 ? h->cvar_fixed[precision] : h->cvar[precision];
+  *v = c;
+  *kc = c >= 0 ? kCVar[precision][c].kc : 0;
+  *waves = c >= 0 ? kCVar[precision][c].nw : 0;
+  *n_candidates = kNCVar[precision];
   return MQCT_OK;
 }
 
@@ -716,11 +847,9 @@ int mqct_run_cutoff_fp64(mqct_hip *h, int ir1, int ir2, double conv) {
   const bool im = (h->coll_type != 1) || !h->ref_xz;
   const double *pbr = h->ref_xz ? h->d_psi_re.as<double>() : h->d_psib_re.as<double>();
   const double *pbi = h->ref_xz ? h->d_psi_im.as<double>() : h->d_psib_im.as<double>();
-  dim3 grid((unsigned)ntiles, 1);
-  contract_k4_kernel<double, 32, 16><<<grid, 256, 0, h->stream>>>(
-      h->d_psi_re.as<double>(), im ? h->d_psi_im.as<double>() : nullptr,
-      pbr, im ? pbi : nullptr, h->d_Wcut.as<double>(), h->G, 2,
-      h->d_tiles.as<TileDesc>(), om, h->d_cutv.as<double>());
+  cutoff_fp64_kernel<64><<<dim3((unsigned)ntiles), 256, 0, h->stream>>>(
+      h->d_psi_re.as<double>(), im ? h->d_psi_im.as<double>() : nullptr, pbr, im ? pbi : nullptr,
+      h->d_Wcut.as<double>(), h->G, h->d_tiles.as<TileDesc>(), om, h->d_cutv.as<double>());
   HIP_TRY(hipGetLastError());
   return MQCT_OK;
 }

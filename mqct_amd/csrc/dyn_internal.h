@@ -38,7 +38,7 @@ struct DynState {
   long long prof_launches = 0;
   double prof_ms = 0.0, prof_flops = 0.0, prof_bytes = 0.0;
   std::vector<hipEvent_t> prof_ev;
-  std::vector<double> prof_trace;  // per timed chunk: running trajectories, iterations, GEMM ms, shape
+  std::vector<double> prof_trace;  // per timed chunk: running trajectories, iterations, GEMM ms, shape, step ms
   ~DynState() {
     for (auto e : prof_ev) (void)hipEventDestroy(e);
   }

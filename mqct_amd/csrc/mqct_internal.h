@@ -92,6 +92,10 @@ struct mqct_hip {
   bool built = false;
   bool psi_fresh = false;
   double timing[5] = {0, 0, 0, 0, 0};
+  // contraction variant per precision form: chosen by timing (-1 not yet,
+  // re-timed when the basis or grid changes) or fixed by the caller
+  int cvar[3] = {-1, -1, -1}, cvar_fixed[3] = {-1, -1, -1};
+  double tune_ms = 0.0;  // time the last build spent choosing the variant
 
   MlpState *mlp = nullptr;
   DynState *dyn = nullptr;

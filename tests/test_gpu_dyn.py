@@ -201,7 +201,7 @@ def test_at_batched_fp32_coefficients(dev):
     assert np.all(s32["norm_err"] < 1e-6)
 
 
-N_SHAPES = 15
+N_SHAPES = 17
 
 
 @pytest.mark.parametrize("shape", list(range(N_SHAPES)))
@@ -233,7 +233,7 @@ def test_at_batched_every_gemm_tile(dev, shape):
     assert np.array_equal(s0["n_steps"], s1["n_steps"])
 
 
-@pytest.mark.parametrize("shape", [0, 5, 8, 10, 11, 12, 13, 14])
+@pytest.mark.parametrize("shape", [0, 5, 8, 10, 11, 12, 13, 14, 15, 16])
 def test_at_batched_fp32_knots_every_layout(dev, shape):
     """FP32 knot matrices through the LDS-staged, the per-wave and the
     fragment-ordered GEMM forms: the same trajectories as the FP64
@@ -259,4 +259,35 @@ def test_at_batched_fp32_knots_every_layout(dev, shape):
     N = wl.basis.n_states
     d = np.max(np.abs(q1[:, :2 * N] - q0[:, :2 * N]))
     assert 0.0 < d <= 1e-6, f"shape {shape}: fp32-knot batched vs plain amplitude diff {d}"
+    assert np.array_equal(s0["n_steps"], s1["n_steps"])
+
+
+@pytest.mark.parametrize("shape,knots", [(0, 1), (12, 1), (15, 1), (16, 1), (15, 2), (16, 2)])
+def test_at_batched_large_blocks(dev, shape, knots):
+    """m-blocks of up to 81 states (asymmetric top j <= 8, 969 states): the
+    forms whose waves own several 16-row tiles of a block, and the FP32
+    knot layout with its wider fragments, against the plain kernel's RK4
+    trajectories (1e-9 with FP64 coefficients, 1e-6 with FP32)."""
+    from mqct_amd.matrix import MatrixBuilder
+    from mqct_amd.propagate import at_first_step
+    wl = W.asym_top_atom(jmax=8, n_beta=20, n_gamma=40, n_r=120)
+    bb, E = W.at_batch(24, seed=17)
+    paths, l, _ = at_first_step(dev, W.asym_top_atom(jmax=1, n_beta=12, n_gamma=24, n_r=120), bb, E,
+                                time_step=20.0, rec_cap=20000)
+    MatrixBuilder(dev, wl).build("fp64", copy_out=False)
+    prop = Propagator(dev, wl.basis, wl.R, mass_amu=wl.mass_amu, from_build=True)
+    prop.load_paths(paths)
+    rows = []
+    for i in range(len(bb)):
+        rows += prop.trajectories([l[i]], float(E[i]), 1, time_step=10.0, at_paths=[i])
+    q0, s0 = prop.run(L.MQCT_PROP_AT_RK4, rows, batched=0)
+    L.check(dev.lib.mqct_hip_batched_set_tile(dev.h, shape), "set_tile")
+    try:
+        q1, s1 = prop.run(L.MQCT_PROP_AT_RK4, rows, batched=knots)
+    finally:
+        L.check(dev.lib.mqct_hip_batched_set_tile(dev.h, -1), "set_tile")
+    N = wl.basis.n_states
+    d = np.max(np.abs(q1[:, :2 * N] - q0[:, :2 * N]))
+    tol = 1e-9 if knots == 1 else 1e-6
+    assert d <= tol, f"shape {shape} knots {knots}: batched vs plain amplitude diff {d}"
     assert np.array_equal(s0["n_steps"], s1["n_steps"])

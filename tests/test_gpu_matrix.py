@@ -157,3 +157,32 @@ def test_mixed_precision_two_asym_tops(dev, prec):
     width of every form)."""
     check_mixed(dev, W.two_asym_tops(jmax1=1, jmax2=1, n_ang_each=(2, 4, 3, 2, 4), n_r=8), prec)
 
+
+
+@pytest.mark.parametrize("prec,variant", [("fp64", 0), ("fp64", 1), ("fp64", 2), ("fp32", 0), ("fp32", 1),
+                                          ("fp32", 2), ("fp32", 3), ("bf16x3", 0), ("bf16x3", 1), ("bf16x3", 2)])
+def test_contraction_variants(dev, prec, variant):
+    """Every contraction variant the build may choose (grid points per LDS
+    stage, waves per workgroup) meets its precision's bar against the
+    oracle, on an asymmetric top whose grid (288 points) suits the 16-byte
+    staging of every form, and 150 R points (R blocks of 64 and 128 with a
+    partial last block); the timed choice is restored afterwards."""
+    from mqct_amd import _lib as L
+    from mqct_amd.matrix import PRECISIONS
+    import ctypes
+    p = PRECISIONS[prec]
+    L.check(dev.lib.mqct_hip_set_contract_variant(dev.h, p, variant), "set_contract_variant")
+    try:
+        wl = W.asym_top_atom(jmax=3, n_beta=12, n_gamma=24, n_r=150)
+        if prec == "fp64":
+            (M, D, S), (Mo, Do, So), orc = build_both(dev, wl)
+            assert np.array_equal(S, So)
+            assert fp64_ok(M, Mo, abs_sums(orc, wl)) <= 1.0
+        else:
+            check_mixed(dev, wl, prec)
+        v, kc, nw, nc = (ctypes.c_int() for _ in range(4))
+        L.check(dev.lib.mqct_hip_contract_variant(dev.h, p, ctypes.byref(v), ctypes.byref(kc), ctypes.byref(nw),
+                                                  ctypes.byref(nc)), "contract_variant")
+        assert v.value == variant and kc.value > 0 and nw.value in (4, 8) and nc.value > variant
+    finally:
+        L.check(dev.lib.mqct_hip_set_contract_variant(dev.h, p, -1), "set_contract_variant")
