@@ -74,6 +74,8 @@ def main():
                     help="MIN_TMSTP (a.u.): the step floor of odeint's error control")
     ap.add_argument("--knots", choices=("fp32", "fp64"), default="fp32",
                     help="storage precision of the spline-coefficient matrices of the timed form")
+    ap.add_argument("--step-threads", type=int, default=0, choices=(0, 256, 512),
+                    help="threads per trajectory of the batched step kernel (0: the library's choice)")
     ap.add_argument("--plain-sample", type=int, default=512)
     ap.add_argument("--xsec-sample", type=int, default=512)
     ap.add_argument("--no-graph", action="store_true",
@@ -126,6 +128,7 @@ def main():
     build = {"asym_top_atom": {"fp64": mb.timing()}}
     prop = Propagator(dev, wl.basis, wl.R, mass_amu=wl.mass_amu, from_build=True)
     prop.load_paths(paths)
+    L.check(dev.lib.mqct_hip_batched_set_step_threads(dev.h, args.step_threads), "set_step_threads")
     rows = []
     for i in range(n):
         rows += prop.trajectories([l[i]], float(E[i]), 1, time_step=10.0, eps=1e-8, hmin=args.hmin,

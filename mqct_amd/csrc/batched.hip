@@ -53,6 +53,14 @@ struct Ctl {
   int last, rec;                  // last: rk4 step that ends the path; rec: path-record hint
 };
 
+// A bucket entry: trajectory, knot side (0: the knot below R, 1: above)
+// and its two spline weights for that knot, so the GEMM reads one record
+// per entry.
+struct Entry {
+  int t, side;
+  double wy, wy2;
+};
+
 struct BDev {
   int N, nblk, n_r, n_traj, mode;
   const double *R;            // knots; R[-1] holds the reduced mass
@@ -71,8 +79,8 @@ struct BDev {
   const int *rg_blk;          // [nrg] block of the row group
   const int *rg_row0;         // [nrg] first row of the group in its block
   const double *E;            // per position, Hartree
-  const int *nb_lo, *nb_hi;   // per position: position of state-1 / state+1, -1 none
-  const double *sq1, *sq2;    // per position: Coriolis factors
+  const int2 *nb;             // per position: positions of state-1 / state+1, -1 none
+  const double2 *sq;          // per position: their Coriolis factors
   const double *prec, *pder;  // paths
   const long long *poff;
   double *S;                  // [n_traj][NSLOT][2N]
@@ -80,7 +88,7 @@ struct BDev {
   double2 *W;                 // [n_traj][2][N]
   Ctl *ctl;
   int *cnt;                   // [2][n_r]
-  int2 *bucket;               // [2][n_r][n_traj] (t, side)
+  Entry *bucket;              // [2][n_r][n_traj]
   int *active;                // running trajectories
   unsigned long long *nbucketed;  // right-hand sides with R inside the grid
   unsigned long long *ntouched;   // (GEMM launch, knot) pairs with a non-empty bucket
@@ -96,24 +104,26 @@ __constant__ double CK_B[6][5] = {
     {-11.0 / 54.0, 2.5, -70.0 / 27.0, 35.0 / 27.0, 0},
     {1631.0 / 55296.0, 175.0 / 512.0, 575.0 / 13824.0, 44275.0 / 110592.0, 253.0 / 4096.0}};
 
+template <int TB = BS>
 __device__ inline double block_sum(double x, double *red) {
   for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
   __syncthreads();
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = x;
   __syncthreads();
   double s = 0.0;
-  for (int i = 0; i < BS / 64; i++) s += red[i];
+  for (int i = 0; i < TB / 64; i++) s += red[i];
   __syncthreads();
   return s;
 }
 
+template <int TB = BS>
 __device__ inline double block_max(double x, double *red) {
   for (int o = 32; o > 0; o >>= 1) x = fmax(x, __shfl_xor(x, o, 64));
   __syncthreads();
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = x;
   __syncthreads();
   double s = red[0];
-  for (int i = 1; i < BS / 64; i++) s = fmax(s, red[i]);
+  for (int i = 1; i < TB / 64; i++) s = fmax(s, red[i]);
   __syncthreads();
   return s;
 }
@@ -142,14 +152,14 @@ __device__ inline double stage_time(const Ctl &c) {
 // (dPhi/dt for the Coriolis term), finds the knots and spline weights,
 // records them in its c and appends the trajectory to the buckets of
 // parity `par`.
-template <int PT>
+template <int PT, int TB = BS>
 __device__ void prepare_stage(const BDev &d, int t, Ctl &c, const double *xr, const double *xi, int par) {
   const int tid = threadIdx.x, N = d.N;
   const double tau = stage_time(c);
   double2 *v = d.V + (long long)t * N;
 #pragma unroll
   for (int j = 0; j < PT; j++) {
-    const int i = tid + j * BS;
+    const int i = tid + j * TB;
     if (i < N) {
       double sn, cs;
       sincos(tau * d.E[i], &sn, &cs);
@@ -191,11 +201,11 @@ __device__ void prepare_stage(const BDev &d, int t, Ctl &c, const double *xr, co
     if (!c.zero) {
       if (d.nbucketed) atomicAdd(d.nbucketed, 1ULL);  // counted only in profiled runs
       int *cnt = d.cnt + par * nr;
-      int2 *bk = d.bucket + (long long)par * nr * d.n_traj;
+      Entry *bk = d.bucket + (long long)par * nr * d.n_traj;
       const int p0 = atomicAdd(&cnt[klo], 1);
-      bk[(long long)klo * d.n_traj + p0] = make_int2(t, 0);
+      bk[(long long)klo * d.n_traj + p0] = Entry{t, 0, aa, c.c1};
       const int p1 = atomicAdd(&cnt[khi], 1);
-      bk[(long long)khi * d.n_traj + p1] = make_int2(t, 1);
+      bk[(long long)khi * d.n_traj + p1] = Entry{t, 1, bb, c.c2};
     }
   }
 }
@@ -240,7 +250,7 @@ __device__ void lds_super_tile(const BDev &d, int par, int r, int b, int row0, i
     tile_on[q] = rt0 < nb;
     rr[q] = rt0 + (lane & 15);
   }
-  const int2 *bk = d.bucket + ((long long)par * d.n_r + r) * d.n_traj;
+  const Entry *bk = d.bucket + ((long long)par * d.n_r + r) * d.n_traj;
   const int nct = (2 * nE + 15) / 16;
   const int nEpad = nct * 8;  // entries staged (zero columns up to the tile edge)
   __syncthreads();
@@ -248,12 +258,11 @@ __device__ void lds_super_tile(const BDev &d, int par, int r, int b, int row0, i
     int t = 0, side = 0;
     double wy = 0.0, wy2 = 0.0;
     if (tid < nE) {
-      const int2 ent = bk[e0 + tid];
-      const Ctl &c = d.ctl[ent.x];
-      t = ent.x;
-      side = ent.y;
-      wy = side ? c.bb : c.aa;
-      wy2 = side ? c.c2 : c.c1;
+      const Entry ent = bk[e0 + tid];
+      t = ent.t;
+      side = ent.side;
+      wy = ent.wy;
+      wy2 = ent.wy2;
     }
     sh.t[tid] = t;
     sh.side[tid] = side;
@@ -363,7 +372,7 @@ __global__ __launch_bounds__(BS) void gemm_direct_kernel(BDev d, int par) {
   const int rr = rt0 + (lane & 15);
   const KT *Y = static_cast<const KT *>(d.KM) + ((long long)r * 2) * d.S2 + d.blk_off[b];
   const KT *Y2 = static_cast<const KT *>(d.KM) + ((long long)r * 2 + 1) * d.S2 + d.blk_off[b];
-  const int2 *bk = d.bucket + ((long long)par * d.n_r + r) * d.n_traj;
+  const Entry *bk = d.bucket + ((long long)par * d.n_r + r) * d.n_traj;
   for (int e0 = 0; e0 < ne; e0 += 8 * NCT) {
     const double *vp[NCT];
     double wy[NCT], wy2[NCT];
@@ -373,13 +382,12 @@ __global__ __launch_bounds__(BS) void gemm_direct_kernel(BDev d, int par) {
     for (int ct = 0; ct < NCT; ct++) {
       const int e = e0 + ct * 8 + (col >> 1);
       ok[ct] = e < ne;
-      int2 ent = ok[ct] ? bk[e] : make_int2(0, 0);
-      const Ctl &c = d.ctl[ent.x];
-      tt[ct] = ent.x;
-      sd[ct] = ent.y;
-      wy[ct] = ok[ct] ? (ent.y ? c.bb : c.aa) : 0.0;
-      wy2[ct] = ok[ct] ? (ent.y ? c.c2 : c.c1) : 0.0;
-      vp[ct] = reinterpret_cast<const double *>(d.V + (long long)ent.x * d.N + p0) + comp;
+      const Entry ent = ok[ct] ? bk[e] : Entry{0, 0, 0.0, 0.0};
+      tt[ct] = ent.t;
+      sd[ct] = ent.side;
+      wy[ct] = ent.wy;
+      wy2[ct] = ent.wy2;
+      vp[ct] = reinterpret_cast<const double *>(d.V + (long long)ent.t * d.N + p0) + comp;
     }
     dbl4 acc[NCT];
 #pragma unroll
@@ -458,7 +466,7 @@ __global__ __launch_bounds__(BS) void gemm_stream_kernel(BDev d, int par) {
   const int q = lane >> 4, col = lane & 15, comp = col & 1;
   const FV *A = reinterpret_cast<const FV *>(static_cast<const KT *>(d.KT) + (long long)r * d.KS + d.tile_off[g]) +
                 lane;
-  const int2 *bk = d.bucket + ((long long)par * d.n_r + r) * d.n_traj;
+  const Entry *bk = d.bucket + ((long long)par * d.n_r + r) * d.n_traj;
   for (int e0 = 0; e0 < ne; e0 += 8 * NCT) {
     const double *vp[NCT];
     double wy[NCT], wy2[NCT];
@@ -468,13 +476,12 @@ __global__ __launch_bounds__(BS) void gemm_stream_kernel(BDev d, int par) {
     for (int ct = 0; ct < NCT; ct++) {
       const int e = e0 + ct * 8 + (col >> 1);
       ok[ct] = e < ne;
-      const int2 ent = ok[ct] ? bk[e] : make_int2(0, 0);
-      const Ctl &c = d.ctl[ent.x];
-      tt[ct] = ent.x;
-      sd[ct] = ent.y;
-      wy[ct] = ok[ct] ? (ent.y ? c.bb : c.aa) : 0.0;
-      wy2[ct] = ok[ct] ? (ent.y ? c.c2 : c.c1) : 0.0;
-      vp[ct] = reinterpret_cast<const double *>(d.V + (long long)ent.x * d.N + p0) + comp;
+      const Entry ent = ok[ct] ? bk[e] : Entry{0, 0, 0.0, 0.0};
+      tt[ct] = ent.t;
+      sd[ct] = ent.side;
+      wy[ct] = ent.wy;
+      wy2[ct] = ent.wy2;
+      vp[ct] = reinterpret_cast<const double *>(d.V + (long long)ent.t * d.N + p0) + comp;
     }
     dbl4 acc[NCT];
 #pragma unroll
@@ -598,7 +605,7 @@ __global__ __launch_bounds__(BS) void gemm_queue_kernel(BDev d, int par) {
     const int ns = (nb + KB - 1) / KB;
     const FV *A =
         reinterpret_cast<const FV *>(static_cast<const KT *>(d.KT) + (long long)r * d.KS + d.tile_off[g]) + lane;
-    const int2 *bk = d.bucket + ((long long)par * nr + r) * d.n_traj;
+    const Entry *bk = d.bucket + ((long long)par * nr + r) * d.n_traj;
     const double *vp[NCT];
     double wy[NCT], wy2[NCT];
     int tt[NCT], sd[NCT];
@@ -607,13 +614,12 @@ __global__ __launch_bounds__(BS) void gemm_queue_kernel(BDev d, int par) {
     for (int ct = 0; ct < NCT; ct++) {
       const int e = e0 + ct * 8 + (col >> 1);
       ok[ct] = e < ne;
-      const int2 ent = ok[ct] ? bk[e] : make_int2(0, 0);
-      const Ctl &c = d.ctl[ent.x];
-      tt[ct] = ent.x;
-      sd[ct] = ent.y;
-      wy[ct] = ok[ct] ? (ent.y ? c.bb : c.aa) : 0.0;
-      wy2[ct] = ok[ct] ? (ent.y ? c.c2 : c.c1) : 0.0;
-      vp[ct] = reinterpret_cast<const double *>(d.V + (long long)ent.x * d.N + p0) + comp;
+      const Entry ent = ok[ct] ? bk[e] : Entry{0, 0, 0.0, 0.0};
+      tt[ct] = ent.t;
+      sd[ct] = ent.side;
+      wy[ct] = ent.wy;
+      wy2[ct] = ent.wy2;
+      vp[ct] = reinterpret_cast<const double *>(d.V + (long long)ent.t * d.N + p0) + comp;
     }
     dbl4 acc[NCT];
 #pragma unroll
@@ -689,7 +695,9 @@ __global__ __launch_bounds__(BS, GE >= 64 ? 1 : 2) void gemm_pipe_kernel(BDev d,
   constexpr int U = F::U, KB = 4 * U;
   constexpr int GK = 16, LPS = GK / KB;  // contraction rows per slice, fragment loads per tile per slice
   constexpr int GC = 2 * GE / 16;        // MFMA column tiles
-  constexpr int BW = 2 * GE + 16;        // LDS row stride (doubles)
+  // LDS row stride (doubles): the rows lane groups q and q + 1 read at one
+  // MFMA step are U apart; U BW = 16 (mod 32) puts them 32 banks apart
+  constexpr int BW = 2 * GE + (U == 4 ? 4 : 8);
   constexpr int SPT = GK * GE / BS;      // (row, entry) pairs staged per thread per slice
   static_assert(SPT * BS == GK * GE && LPS * KB == GK, "slice shape");
   extern __shared__ int pre[];  // [n_r + 1] chunk prefix
@@ -711,18 +719,17 @@ __global__ __launch_bounds__(BS, GE >= 64 ? 1 : 2) void gemm_pipe_kernel(BDev d,
     const int p0 = d.blk_start[b], nb = d.blk_start[b + 1] - p0;
     const int nbp = (nb + KB - 1) / KB * KB, nsl = 2 * nbp / GK;
     const int e0 = c * GE, nE = min(GE, cnt[r] - e0), nct = (2 * nE + 15) / 16;
-    const int2 *bk = d.bucket + ((long long)par * nr + r) * d.n_traj;
+    const Entry *bk = d.bucket + ((long long)par * nr + r) * d.n_traj;
     __syncthreads();  // the previous item is done with the entry table and Bs
     if (tid < GE) {
       int t = 0, side = 0;
       double wy = 0.0, wy2 = 0.0;
       if (tid < nE) {
-        const int2 ent = bk[e0 + tid];
-        const Ctl &cc = d.ctl[ent.x];
-        t = ent.x;
-        side = ent.y;
-        wy = side ? cc.bb : cc.aa;
-        wy2 = side ? cc.c2 : cc.c1;
+        const Entry ent = bk[e0 + tid];
+        t = ent.t;
+        side = ent.side;
+        wy = ent.wy;
+        wy2 = ent.wy2;
       }
       s_t[tid] = t;
       s_side[tid] = side;
@@ -885,8 +892,8 @@ void launch_gemm(int shape, dim3 knots_blocks, int maxnb, hipStream_t st, const 
 // phases -- the right-hand side k of the completed stage, the integrator
 // update, the next stage vector -- so k is stored only when a later stage
 // reads it, and a finished step's y feeds the next stage without a reload.
-template <int PT>
-__global__ __launch_bounds__(BS) void step_kernel(BDev d, int par) {
+template <int PT, int TB = BS>
+__global__ __launch_bounds__(TB) void step_kernel(BDev d, int par) {
   extern __shared__ double sys[];  // 2N: the completed stage's vector (Coriolis neighbours)
   __shared__ double red[8];
   const int t = blockIdx.x, tid = threadIdx.x, N = d.N, n2 = 2 * N;
@@ -903,7 +910,7 @@ __global__ __launch_bounds__(BS) void step_kernel(BDev d, int par) {
     const double2 *W0 = d.W + ((long long)t * 2) * N, *W1 = W0 + N;
 #pragma unroll
     for (int j = 0; j < PT; j++) {
-      const int i = tid + j * BS;
+      const int i = tid + j * TB;
       kr[j] = ki[j] = 0.0;
       if (i < N) {
         double sn, cs;
@@ -922,17 +929,18 @@ __global__ __launch_bounds__(BS) void step_kernel(BDev d, int par) {
     const double hd = c.dphidt / 2.0;
 #pragma unroll
     for (int j = 0; j < PT; j++) {
-      const int i = tid + j * BS;
+      const int i = tid + j * TB;
       if (i < N) {
         // state-diagonal Coriolis coupling (mqct_eqtns.f:845-877)
-        const int lo = d.nb_lo[i], hi = d.nb_hi[i];
-        if (lo >= 0) {
-          kr[j] += hd * d.sq1[i] * sys[lo + N];
-          ki[j] -= hd * d.sq1[i] * sys[lo];
+        const int2 nbr = d.nb[i];
+        const double2 f = d.sq[i];
+        if (nbr.x >= 0) {
+          kr[j] += hd * f.x * sys[nbr.x + N];
+          ki[j] -= hd * f.x * sys[nbr.x];
         }
-        if (hi >= 0) {
-          kr[j] += hd * d.sq2[i] * sys[hi + N];
-          ki[j] -= hd * d.sq2[i] * sys[hi];
+        if (nbr.y >= 0) {
+          kr[j] += hd * f.y * sys[nbr.y + N];
+          ki[j] -= hd * f.y * sys[nbr.y];
         }
       }
     }
@@ -945,7 +953,7 @@ __global__ __launch_bounds__(BS) void step_kernel(BDev d, int par) {
     double *k = S + (1 + s) * n2;
 #pragma unroll
     for (int j = 0; j < PT; j++) {
-      const int i = tid + j * BS;
+      const int i = tid + j * TB;
       if (i < N) {
         k[i] = kr[j];
         k[i + N] = ki[j];
@@ -956,7 +964,7 @@ __global__ __launch_bounds__(BS) void step_kernel(BDev d, int par) {
   auto stage_from_k = [&](double f) {
 #pragma unroll
     for (int j = 0; j < PT; j++) {
-      const int i = tid + j * BS;
+      const int i = tid + j * TB;
       if (i < N) {
         xr[j] = y[i] + f * kr[j];
         xi[j] = y[i + N] + f * ki[j];
@@ -974,7 +982,7 @@ __global__ __launch_bounds__(BS) void step_kernel(BDev d, int par) {
       double nrm = 0.0;
 #pragma unroll
       for (int j = 0; j < PT; j++) {
-        const int i = tid + j * BS;
+        const int i = tid + j * TB;
         if (i < N) {
           xr[j] = y[i] + h6 * (k1[i] + kr[j] + 2.0 * (k2[i] + k3[i]));
           xi[j] = y[i + N] + h6 * (k1[i + N] + ki[j] + 2.0 * (k2[i + N] + k3[i + N]));
@@ -996,7 +1004,7 @@ __global__ __launch_bounds__(BS) void step_kernel(BDev d, int par) {
         if (c.last) c.status = 1;
         else if (c.x + c.h >= c.x2) { c.h = c.x2 - c.x; c.last = 1; }
       }
-      nrm = block_sum(nrm, red);
+      nrm = block_sum<TB>(nrm, red);
       c.norm_err = fmax(c.norm_err, fabs(1.0 - nrm));
     }
   } else if (s == 0) {
@@ -1022,7 +1030,7 @@ __global__ __launch_bounds__(BS) void step_kernel(BDev d, int par) {
     c.stage = s + 1;
 #pragma unroll
     for (int j = 0; j < PT; j++) {
-      const int i = tid + j * BS;
+      const int i = tid + j * TB;
       if (i < N) {
         double ar = 0.0, ai = 0.0;
         for (int q = 0; q < s; q++) {
@@ -1045,7 +1053,7 @@ __global__ __launch_bounds__(BS) void step_kernel(BDev d, int par) {
     double em = 0.0;
 #pragma unroll
     for (int j = 0; j < PT; j++) {
-      const int i = tid + j * BS;
+      const int i = tid + j * TB;
       if (i < N) {
 #pragma unroll
         for (int p = 0; p < 2; p++) {
@@ -1060,7 +1068,7 @@ __global__ __launch_bounds__(BS) void step_kernel(BDev d, int par) {
         }
       }
     }
-    em = block_max(em, red) / c.eps;
+    em = block_max<TB>(em, red) / c.eps;
     // a step at or below hmin is taken as is
     const bool accept = (c.h <= c.hmin) || !(em > 1.0);
     if (!accept) {
@@ -1070,7 +1078,7 @@ __global__ __launch_bounds__(BS) void step_kernel(BDev d, int par) {
       c.stage = 1;  // retry with the same k1
 #pragma unroll
       for (int j = 0; j < PT; j++) {
-        const int i = tid + j * BS;
+        const int i = tid + j * TB;
         if (i < N) {
           xr[j] = y[i] + CK_B[1][0] * c.h * k1[i];
           xi[j] = y[i + N] + CK_B[1][0] * c.h * k1[i + N];
@@ -1080,7 +1088,7 @@ __global__ __launch_bounds__(BS) void step_kernel(BDev d, int par) {
       if (c.h > c.hmin) c.hnext = (em > 1.89e-4) ? 0.9 * c.h * pow(em, -.2) : 5.0 * c.h;
 #pragma unroll
       for (int j = 0; j < PT; j++) {
-        const int i = tid + j * BS;
+        const int i = tid + j * TB;
         if (i < N) {
           y[i] = xr[j];
           y[i + N] = xi[j];
@@ -1104,7 +1112,7 @@ __global__ __launch_bounds__(BS) void step_kernel(BDev d, int par) {
     }
     return;
   }
-  prepare_stage<PT>(d, t, c, xr, xi, par);
+  prepare_stage<PT, TB>(d, t, c, xr, xi, par);
   if (tid == 0) d.ctl[t] = c;
 }
 
@@ -1267,20 +1275,20 @@ int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *t
     return mqct_fail(MQCT_ERR_ARG,
                      "propagate: the batched form keeps a stage vector of 2N doubles in LDS; N too large");
   // per position: energies, Coriolis neighbours (state-1 / state+1) and factors
-  std::vector<double> E(N), sq1(N), sq2(N);
-  std::vector<int> nlo(N), nhi(N);
+  std::vector<double> E(N), sqv(2 * N);
+  std::vector<int> nbv(2 * N);
   for (int p = 0; p < N; p++) {
     const int s = pos_state[p];
     E[p] = ds->E[s];
     const double j = ds->j12[s], m = ds->m12[s];
     const double m1 = m + 1.0, m0 = m - 1.0;
-    sq1[p] = std::sqrt(j * (j + 1.0) - m0 * (m0 + 1.0));
-    sq2[p] = std::sqrt(j * (j + 1.0) - m1 * (m1 - 1.0));
-    nlo[p] = s > 0 ? state_pos[s - 1] : -1;
-    nhi[p] = s < N - 1 ? state_pos[s + 1] : -1;
+    sqv[2 * p] = std::sqrt(j * (j + 1.0) - m0 * (m0 + 1.0));
+    sqv[2 * p + 1] = std::sqrt(j * (j + 1.0) - m1 * (m1 - 1.0));
+    nbv[2 * p] = s > 0 ? state_pos[s - 1] : -1;
+    nbv[2 * p + 1] = s < N - 1 ? state_pos[s + 1] : -1;
   }
   const int n2 = 2 * N;
-  DevBuf d_ps, d_sp, d_bs, d_bo, d_pb, d_E, d_nlo, d_nhi, d_sq1, d_sq2, d_R, d_tr;
+  DevBuf d_ps, d_sp, d_bs, d_bo, d_pb, d_E, d_nb, d_sq, d_R, d_tr;
   auto up = [&](DevBuf &b, const void *src, size_t bytes) -> int {
     if (b.ensure(bytes)) return 1;
     return hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, h->stream) != hipSuccess;
@@ -1290,8 +1298,8 @@ int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *t
   for (int i = 0; i < n_r; i++) Rmu[i + 1] = ds->R[i];
   if (up(d_ps, pos_state.data(), N * 4) || up(d_sp, state_pos.data(), N * 4) ||
       up(d_bs, blk_start.data(), blk_start.size() * 4) || up(d_bo, blk_off.data(), blk_off.size() * 8) ||
-      up(d_pb, pos_blk.data(), N * 4) || up(d_E, E.data(), N * 8) || up(d_nlo, nlo.data(), N * 4) ||
-      up(d_nhi, nhi.data(), N * 4) || up(d_sq1, sq1.data(), N * 8) || up(d_sq2, sq2.data(), N * 8) ||
+      up(d_pb, pos_blk.data(), N * 4) || up(d_E, E.data(), N * 8) || up(d_nb, nbv.data(), N * 8) ||
+      up(d_sq, sqv.data(), N * 16) ||
       up(d_R, Rmu.data(), (n_r + 1) * 8) || up(d_tr, traj, (size_t)n_traj * sizeof(mqct_traj_t)))
     return mqct_fail(MQCT_ERR_NOMEM, "batched: tables");
   // knot matrices: made once per matrix and storage precision
@@ -1375,7 +1383,7 @@ int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *t
   const size_t off_C = off_W + al((size_t)n_traj * 2 * N * 16);
   const size_t off_cnt = off_C + al((size_t)n_traj * sizeof(Ctl));
   const size_t off_bk = off_cnt + al((size_t)2 * n_r * 4);
-  const size_t off_act = off_bk + al((size_t)2 * n_r * n_traj * 8);
+  const size_t off_act = off_bk + al((size_t)2 * n_r * n_traj * sizeof(Entry));
   const size_t total = off_act + 256;
   if (ds->d_bat.ensure(total)) return mqct_fail(MQCT_ERR_NOMEM, "batched: trajectory state");
   char *base = ds->d_bat.as<char>();
@@ -1401,10 +1409,8 @@ int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *t
   d.rg_blk = d_rgb.as<int>();
   d.rg_row0 = d_rgr.as<int>();
   d.E = d_E.as<double>();
-  d.nb_lo = d_nlo.as<int>();
-  d.nb_hi = d_nhi.as<int>();
-  d.sq1 = d_sq1.as<double>();
-  d.sq2 = d_sq2.as<double>();
+  d.nb = d_nb.as<int2>();
+  d.sq = d_sq.as<double2>();
   d.prec = ds->d_prec.as<double>();
   d.pder = ds->d_pder.as<double>();
   d.poff = ds->d_poff.as<long long>();
@@ -1413,7 +1419,7 @@ int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *t
   d.W = reinterpret_cast<double2 *>(base + off_W);
   d.ctl = reinterpret_cast<Ctl *>(base + off_C);
   d.cnt = reinterpret_cast<int *>(base + off_cnt);
-  d.bucket = reinterpret_cast<int2 *>(base + off_bk);
+  d.bucket = reinterpret_cast<Entry *>(base + off_bk);
   d.active = reinterpret_cast<int *>(base + off_act);
   d.nbucketed = ds->prof_on ? reinterpret_cast<unsigned long long *>(base + off_act + 8) : nullptr;
   d.ntouched = reinterpret_cast<unsigned long long *>(base + off_act + 16);
@@ -1422,7 +1428,16 @@ int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *t
   HIP_TRY(hipMemcpyAsync(d.active, &n_traj, 4, hipMemcpyHostToDevice, h->stream));
   // states per thread of the step kernel
   const int PT = N <= 4 * BS ? 4 : (N <= 8 * BS ? 8 : MAXPT);
+  // step kernel: 256 threads per trajectory, or 512 (half the states per
+  // thread, twice the loads in flight) when the states fill them
+  const int stb = ds->step_threads > 0 ? ds->step_threads : (N > 4 * BS ? 512 : 256);
+  if (stb != 256 && stb != 512) return mqct_fail(MQCT_ERR_ARG, "batched: step threads must be 256 or 512");
   auto step = [&](int par) {
+    if (stb == 512) {
+      if (N <= 4 * 512) step_kernel<4, 512><<<n_traj, 512, lds_step, h->stream>>>(d, par);
+      else step_kernel<8, 512><<<n_traj, 512, lds_step, h->stream>>>(d, par);
+      return;
+    }
     if (PT == 4) step_kernel<4><<<n_traj, BS, lds_step, h->stream>>>(d, par);
     else if (PT == 8) step_kernel<8><<<n_traj, BS, lds_step, h->stream>>>(d, par);
     else step_kernel<MAXPT><<<n_traj, BS, lds_step, h->stream>>>(d, par);
@@ -1631,6 +1646,14 @@ extern "C" int mqct_hip_batched_prof_trace(mqct_hip_t *h, int cap, double *rows,
   *n = (int)(tr.size() / 5);
   const size_t m = std::min((size_t)cap, tr.size() / 5) * 5;
   std::copy(tr.begin(), tr.begin() + m, rows);
+  return MQCT_OK;
+}
+
+extern "C" int mqct_hip_batched_set_step_threads(mqct_hip_t *h, int threads) {
+  if (!h || !h->dyn) return mqct_fail(MQCT_ERR_STATE, "batched_set_step_threads: no matrix loaded");
+  if (threads != 0 && threads != 256 && threads != 512)
+    return mqct_fail(MQCT_ERR_ARG, "batched_set_step_threads: 0, 256 or 512");
+  h->dyn->step_threads = threads;
   return MQCT_OK;
 }
 

@@ -28,6 +28,7 @@ struct DynState {
   static constexpr int kLevels = 24;
   int gemm_shape[2][kLevels];
   int gemm_fixed = -1;
+  int step_threads = 0;  // batched step kernel: 0 automatic, 256 or 512
   int gemm_first[2] = {-1, -1};  // shape the last run started with
   DynState() {
     for (auto &p : gemm_shape)

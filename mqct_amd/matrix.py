@@ -151,3 +151,38 @@ def read_mtrx_uf(path):
             "read_mtrx_uf")
     return {"coll_type": coll, "n_channels": nch, "n_states": ns, "total_size": T, "n_r": nr,
             "indx_chann": ich, "j12": j12, "m12": m12, "chan_qn": qn, "ind_mat": ind, "R": R, "mat_el": M}
+
+
+def _main(argv=None):
+    """Times the matrix build of a named workload on GPU 0 in each precision
+    form (twice each: the first build of a form also times its contraction
+    variants, which the reported times leave out):
+
+        python -m mqct_amd.matrix two_asym_tops fp64,fp32,bf16x3
+    """
+    import argparse
+    from . import workloads as W
+    ap = argparse.ArgumentParser(description=_main.__doc__)
+    ap.add_argument("workload", choices=("diatom_atom", "asym_top_atom", "two_asym_tops"))
+    ap.add_argument("precisions", nargs="?", default="fp64,fp32,bf16x3")
+    ap.add_argument("--reps", type=int, default=2)
+    a = ap.parse_args(argv)
+    wl = getattr(W, a.workload)()
+    with Device(0) as dev:
+        mb = MatrixBuilder(dev, wl)
+        for prec in a.precisions.split(","):
+            for rep in range(a.reps):
+                mb.build(prec, copy_out=False)
+                v, kc, nw, nc = (ctypes.c_int() for _ in range(4))
+                L.check(dev.lib.mqct_hip_contract_variant(dev.h, PRECISIONS[prec], ctypes.byref(v), ctypes.byref(kc),
+                                                          ctypes.byref(nw), ctypes.byref(nc)), "contract_variant")
+                t = mb.timing()
+                print("%s %s build %d: contraction variant %d (%d grid points per stage, %d waves); %s" % (
+                    a.workload, prec, rep, v.value, kc.value, nw.value,
+                    ", ".join("%s %.2f" % (k, float(x)) for k, x in t.items())), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    import sys
+    sys.exit(_main())

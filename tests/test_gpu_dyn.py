@@ -291,3 +291,34 @@ def test_at_batched_large_blocks(dev, shape, knots):
     tol = 1e-9 if knots == 1 else 1e-6
     assert d <= tol, f"shape {shape} knots {knots}: batched vs plain amplitude diff {d}"
     assert np.array_equal(s0["n_steps"], s1["n_steps"])
+
+
+@pytest.mark.parametrize("mode", [L.MQCT_PROP_AT_RK4, L.MQCT_PROP_AT_ODEINT])
+def test_at_batched_step_threads(dev, mode):
+    """The step kernel with 512 threads per trajectory (two states per
+    thread fewer, block reductions over eight waves) against the plain
+    kernel, on 969 states where the automatic choice would take 256."""
+    from mqct_amd.matrix import MatrixBuilder
+    from mqct_amd.propagate import at_first_step
+    wl = W.asym_top_atom(jmax=8, n_beta=20, n_gamma=40, n_r=120)
+    bb, E = W.at_batch(16, seed=19)
+    paths, l, _ = at_first_step(dev, W.asym_top_atom(jmax=1, n_beta=12, n_gamma=24, n_r=120), bb, E,
+                                time_step=20.0, rec_cap=20000)
+    MatrixBuilder(dev, wl).build("fp64", copy_out=False)
+    prop = Propagator(dev, wl.basis, wl.R, mass_amu=wl.mass_amu, from_build=True)
+    prop.load_paths(paths)
+    rows = []
+    for i in range(len(bb)):
+        rows += prop.trajectories([l[i]], float(E[i]), 1, time_step=10.0, eps=1e-8, hmin=20.0, at_paths=[i])
+    q0, s0 = prop.run(mode, rows, batched=0)
+    L.check(dev.lib.mqct_hip_batched_set_step_threads(dev.h, 512), "set_step_threads")
+    try:
+        q1, s1 = prop.run(mode, rows, batched=1)
+    finally:
+        L.check(dev.lib.mqct_hip_batched_set_step_threads(dev.h, 0), "set_step_threads")
+    N = wl.basis.n_states
+    d = np.max(np.abs(q1[:, :2 * N] - q0[:, :2 * N]))
+    tol = 1e-9 if mode == L.MQCT_PROP_AT_RK4 else 1e-6
+    assert d <= tol, f"512-thread step kernel vs plain amplitude diff {d}"
+    if mode == L.MQCT_PROP_AT_RK4:
+        assert np.array_equal(s0["n_steps"], s1["n_steps"])
