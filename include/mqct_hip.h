@@ -208,14 +208,15 @@ int mqct_hip_batched_profile(mqct_hip_t *h, int on);
  * ms, shape index, summed step-kernel ms); copies min(cap, n) rows of 5
  * doubles into rows and returns their number in n. */
 int mqct_hip_batched_prof_trace(mqct_hip_t *h, int cap, double *rows, int *n);
-/* The interval-GEMM candidates: shape index 0..16 (shape = -1: the shape
+/* The interval-GEMM candidates: shape index 0..18 (shape = -1: the shape
  * the last batched run started with, for the coefficient precision of that
  * run).  form: 0 LDS-staged per knot, 1 per-wave loads per knot, 2 the same
  * over the fragment-ordered knot tiles, 3 fragment-ordered work queue of
  * (knot, tile group, entry chunk) items over persistent workgroups, 4
  * LDS-staged work queue of (knot, 64-row group, entry chunk) items, 5
  * software-pipelined work queue of (knot, m-block, entry chunk) items
- * (m-blocks of at most 128 states);
+ * (m-blocks of at most 128 states), 6 the same with the items handed out
+ * one at a time from a device counter, largest m-blocks first;
  * entries: bucket entries per super-tile (per wave for forms 1-3); k_rows:
  * K rows per LDS slice (0 without LDS); row_tiles: 16-row tiles per wave.
  * The runner chooses a shape per level floor(log2(running trajectories)) by
@@ -226,7 +227,7 @@ int mqct_hip_batched_tile(mqct_hip_t *h, int shape, int *form, int *entries,
  * levels 0..n-1 for the FP64 (fp32=0) or FP32 (fp32=1) knot matrices. */
 int mqct_hip_batched_tile_levels(mqct_hip_t *h, int fp32, int n, int *shapes);
 /* Fixes the interval-GEMM tile shape (index into the candidate list,
- * 0..16) for the following batched runs, or -1 to choose by timing again. */
+ * 0..18) for the following batched runs, or -1 to choose by timing again. */
 int mqct_hip_batched_set_tile(mqct_hip_t *h, int shape);
 /* Threads per trajectory of the batched step kernel: 256 or 512, or 0 for
  * the automatic choice (512 when the states exceed 1024). */

@@ -74,6 +74,8 @@ struct BDev {
   const int *tile_blk;        // [ntile] block of the tile
   const int *tile_r0;         // [ntile] first row of the tile in its block
   const int *blk_tile0;       // [nblk] first tile of the block
+  const int *blk_order;       // [nblk] blocks by decreasing size
+  int *qctr;                  // item counter of the dynamically scheduled GEMM
   const long long *tile_off;  // [ntile] offset of the tile in a knot's KT
   int nrg;                    // 64-row groups over all blocks
   const int *rg_blk;          // [nrg] block of the row group
@@ -528,6 +530,7 @@ __global__ __launch_bounds__(BS) void gemm_stream_kernel(BDev d, int par) {
 // workgroup forms the chunk prefix over the knots from the bucket counts in
 // LDS (n_r + 1 ints).
 constexpr int QGRID = 2048;  // persistent workgroups (a multiple of 8)
+constexpr int DGRID = 1024;  // workgroups of the dynamically scheduled form (four per CU)
 
 // The work-queue preamble (whole workgroup): workgroup 0 clears the other
 // parity's bucket counts and counts the touched knots; every workgroup forms
@@ -678,37 +681,171 @@ __global__ __launch_bounds__(BS) void gemm_lds_queue_kernel(BDev d, int par) {
   }
 }
 
-// The software-pipelined product as a work queue: items (knot r, m-block b,
-// chunk of GE bucket entries), in that order, so the chunks of one block at
-// one knot follow each other and meet its matrix in L2.  The whole
-// workgroup takes an item: wave w owns the block's 16-row tiles w and w + 4
-// (up to 128 rows) and keeps their fragment-ordered [Y | Y2] operands in
-// registers, one slice of GK = 16 contraction rows at a time; the weighted
-// amplitudes of the chunk pass through a double-buffered LDS slice.  The
-// next slice's matrix fragments and amplitudes are loaded while the MFMAs
-// of the current one run, so there is one barrier per slice and the loads
-// overlap the math.
+// The software-pipelined product: the whole workgroup takes an item (knot
+// r, m-block b, chunk c of GE bucket entries).  Wave w owns the block's
+// 16-row tiles w and w + 4 (up to 128 rows) and keeps their fragment-ordered
+// [Y | Y2] operands in registers, one slice of GK = 16 contraction rows at
+// a time; the weighted amplitudes of the chunk pass through a
+// double-buffered LDS slice.  The next slice's matrix fragments and
+// amplitudes are loaded while the MFMAs of the current one run, so there is
+// one barrier per slice and the loads overlap the math.
 template <typename KT, int GE>
-__global__ __launch_bounds__(BS, GE >= 64 ? 1 : 2) void gemm_pipe_kernel(BDev d, int par) {
-  using F = Frag<KT>;
-  using FV = typename F::V;
-  constexpr int U = F::U, KB = 4 * U;
-  constexpr int GK = 16, LPS = GK / KB;  // contraction rows per slice, fragment loads per tile per slice
-  constexpr int GC = 2 * GE / 16;        // MFMA column tiles
+struct PipeShape {
+  static constexpr int U = Frag<KT>::U, KB = 4 * U;
+  static constexpr int GK = 16, LPS = GK / KB;  // contraction rows per slice, fragment loads per tile per slice
+  static constexpr int GC = 2 * GE / 16;        // MFMA column tiles
   // LDS row stride (doubles): the rows lane groups q and q + 1 read at one
   // MFMA step are U apart; U BW = 16 (mod 32) puts them 32 banks apart
-  constexpr int BW = 2 * GE + (U == 4 ? 4 : 8);
-  constexpr int SPT = GK * GE / BS;      // (row, entry) pairs staged per thread per slice
+  static constexpr int BW = 2 * GE + (U == 4 ? 4 : 8);
+  static constexpr int SPT = GK * GE / BS;      // (row, entry) pairs staged per thread per slice
   static_assert(SPT * BS == GK * GE && LPS * KB == GK, "slice shape");
-  extern __shared__ int pre[];  // [n_r + 1] chunk prefix
-  __shared__ int wtot[BS / 64];
-  __shared__ double Bs[2][GK][BW];
-  __shared__ int s_t[GE], s_side[GE];
-  __shared__ double s_wy[GE], s_wy2[GE];
+};
+
+template <typename KT, int GE>
+struct PipeShared {
+  double Bs[2][PipeShape<KT, GE>::GK][PipeShape<KT, GE>::BW];
+  int t[GE], side[GE];
+  double wy[GE], wy2[GE];
+};
+
+template <typename KT, int GE>
+__device__ void pipe_item(const BDev &d, int par, int r, int b, int c, PipeShared<KT, GE> &sh) {
+  using F = Frag<KT>;
+  using FV = typename F::V;
+  using P = PipeShape<KT, GE>;
+  constexpr int U = P::U, KB = P::KB, GK = P::GK, LPS = P::LPS, GC = P::GC, SPT = P::SPT;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int q = lane >> 4, col = lane & 15;
-  const int nr = d.n_r, nblk = d.nblk;
+  const int nr = d.n_r;
   const int *cnt = d.cnt + par * nr;
+  const int p0 = d.blk_start[b], nb = d.blk_start[b + 1] - p0;
+  const int nbp = (nb + KB - 1) / KB * KB, nsl = 2 * nbp / GK;
+  const int e0 = c * GE, nE = min(GE, cnt[r] - e0), nct = (2 * nE + 15) / 16;
+  const Entry *bk = d.bucket + ((long long)par * nr + r) * d.n_traj;
+  __syncthreads();  // the previous item is done with the entry table and Bs
+  if (tid < GE) {
+    int t = 0, side = 0;
+    double wy = 0.0, wy2 = 0.0;
+    if (tid < nE) {
+      const Entry ent = bk[e0 + tid];
+      t = ent.t;
+      side = ent.side;
+      wy = ent.wy;
+      wy2 = ent.wy2;
+    }
+    sh.t[tid] = t;
+    sh.side[tid] = side;
+    sh.wy[tid] = wy;
+    sh.wy2[tid] = wy2;
+  }
+  __syncthreads();
+  // the wave's tiles and their fragment streams
+  const int ntl = (nb + 15) / 16;
+  bool on[2];
+  const FV *A[2];
+#pragma unroll
+  for (int h = 0; h < 2; h++) {
+    const int tl = wave + 4 * h;
+    on[h] = tl < ntl;
+    const int g = d.blk_tile0[b] + (on[h] ? tl : 0);
+    A[h] = reinterpret_cast<const FV *>(static_cast<const KT *>(d.KT) + (long long)r * d.KS + d.tile_off[g]) +
+           lane;
+  }
+  FV an[2][LPS];
+  double2 bn[SPT];
+  double bw[SPT];
+  // loads of slice s into registers: fragments of the owned tiles and the
+  // chunk's amplitudes for rows s GK .. s GK + GK - 1 of [Y | Y2]
+  auto load = [&](int s) {
+#pragma unroll
+    for (int h = 0; h < 2; h++)
+#pragma unroll
+      for (int l = 0; l < LPS; l++) an[h][l] = on[h] ? A[h][(s * LPS + l) * 64] : FV{};
+#pragma unroll
+    for (int j = 0; j < SPT; j++) {
+      const int idx = tid + j * BS, kk = idx % GK, e = idx / GK, k = s * GK + kk;
+      const bool y2 = k >= nbp;
+      const int kv = y2 ? k - nbp : k;
+      const bool ok = e < nE && kv < nb;
+      bn[j] = ok ? d.V[(long long)sh.t[e] * d.N + p0 + kv] : make_double2(0.0, 0.0);
+      bw[j] = y2 ? sh.wy2[e] : sh.wy[e];
+    }
+  };
+  auto stage = [&](int buf) {
+#pragma unroll
+    for (int j = 0; j < SPT; j++) {
+      const int idx = tid + j * BS, kk = idx % GK, e = idx / GK;
+      sh.Bs[buf][kk][2 * e] = bw[j] * bn[j].x;
+      sh.Bs[buf][kk][2 * e + 1] = bw[j] * bn[j].y;
+    }
+  };
+  dbl4 acc[2][GC];
+#pragma unroll
+  for (int h = 0; h < 2; h++)
+#pragma unroll
+    for (int ct = 0; ct < GC; ct++) acc[h][ct] = dbl4{0.0, 0.0, 0.0, 0.0};
+  load(0);
+  stage(0);
+  __syncthreads();
+  for (int s = 0; s < nsl; s++) {
+    const int buf = s & 1;
+    FV ac[2][LPS];
+#pragma unroll
+    for (int h = 0; h < 2; h++)
+#pragma unroll
+      for (int l = 0; l < LPS; l++) ac[h][l] = an[h][l];
+    const bool more = s + 1 < nsl;
+    if (more) load(s + 1);
+#pragma unroll
+    for (int l = 0; l < LPS; l++)
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        // fragment order: lane group q holds row KB l + U q + u at step u
+        const double *brow = &sh.Bs[buf][KB * l + U * q + u][col];
+        double bf[GC];
+#pragma unroll
+        for (int ct = 0; ct < GC; ct++) bf[ct] = (ct < nct) ? brow[16 * ct] : 0.0;
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+          if (!on[h]) continue;
+          const double a = F::get(ac[h][l], u);
+#pragma unroll
+          for (int ct = 0; ct < GC; ct++)
+            if (ct < nct) acc[h][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bf[ct], acc[h][ct], 0, 0, 0);
+        }
+      }
+    if (more) stage(buf ^ 1);
+    __syncthreads();
+  }
+  // f64 16x16x4 output: row lane/16 + 4i, column lane%16 of each tile
+#pragma unroll
+  for (int h = 0; h < 2; h++) {
+    if (!on[h]) continue;
+    const int rt0 = (wave + 4 * h) * 16;
+#pragma unroll
+    for (int ct = 0; ct < GC; ct++) {
+      if (ct >= nct) break;
+      const int cc = 16 * ct + col, e = cc >> 1;
+      if (e >= nE) continue;
+      double *wb = reinterpret_cast<double *>(d.W + ((long long)sh.t[e] * 2 + sh.side[e]) * d.N + p0) + (cc & 1);
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int orow = rt0 + q + 4 * i;
+        if (orow < nb) wb[2 * orow] = acc[h][ct][i];
+      }
+    }
+  }
+}
+
+// The pipelined product over a static share of the items: (knot, block,
+// chunk) in that order, so the chunks of one block at one knot follow each
+// other and meet its matrix in L2, with a contiguous item range per XCD.
+template <typename KT, int GE>
+__global__ __launch_bounds__(BS, GE >= 64 ? 1 : 2) void gemm_pipe_kernel(BDev d, int par) {
+  extern __shared__ int pre[];  // [n_r + 1] chunk prefix
+  __shared__ int wtot[BS / 64];
+  __shared__ PipeShared<KT, GE> sh;
+  const int nr = d.n_r, nblk = d.nblk;
   const long long M = (long long)chunk_prefix(d, par, GE, pre, wtot) * nblk;
   const int X = 8, xcd = blockIdx.x % X, slot = blockIdx.x / X, Gx = gridDim.x / X;
   const long long lo = M * xcd / X, hi = M * (xcd + 1) / X;
@@ -716,130 +853,46 @@ __global__ __launch_bounds__(BS, GE >= 64 ? 1 : 2) void gemm_pipe_kernel(BDev d,
     // item -> (knot r, block b, chunk c)
     const int r = chunk_knot(pre, nr, (int)(it / nblk)), cr = pre[r + 1] - pre[r];
     const int loc = (int)(it - (long long)pre[r] * nblk), b = loc / cr, c = loc % cr;
-    const int p0 = d.blk_start[b], nb = d.blk_start[b + 1] - p0;
-    const int nbp = (nb + KB - 1) / KB * KB, nsl = 2 * nbp / GK;
-    const int e0 = c * GE, nE = min(GE, cnt[r] - e0), nct = (2 * nE + 15) / 16;
-    const Entry *bk = d.bucket + ((long long)par * nr + r) * d.n_traj;
-    __syncthreads();  // the previous item is done with the entry table and Bs
-    if (tid < GE) {
-      int t = 0, side = 0;
-      double wy = 0.0, wy2 = 0.0;
-      if (tid < nE) {
-        const Entry ent = bk[e0 + tid];
-        t = ent.t;
-        side = ent.side;
-        wy = ent.wy;
-        wy2 = ent.wy2;
-      }
-      s_t[tid] = t;
-      s_side[tid] = side;
-      s_wy[tid] = wy;
-      s_wy2[tid] = wy2;
-    }
+    pipe_item<KT, GE>(d, par, r, b, c, sh);
+  }
+}
+
+// The pipelined product with the items handed out one at a time from a
+// device counter (cleared by the step kernel before every GEMM), the
+// largest m-blocks first: the block rank in d.blk_order, then the chunk
+// index over all knots.  Item costs differ by the square of the block size,
+// so a static share leaves a long tail; taken largest first, the small
+// items fill in at the end.  Each workgroup fetches its next item while it
+// works on the current one.
+template <typename KT, int GE>
+__global__ __launch_bounds__(BS, GE >= 64 ? 1 : 2) void gemm_pipe_dyn_kernel(BDev d, int par) {
+  extern __shared__ int pre[];  // [n_r + 1] chunk prefix
+  __shared__ int wtot[BS / 64];
+  __shared__ PipeShared<KT, GE> sh;
+  __shared__ int s_next;
+  const int tid = threadIdx.x, nr = d.n_r;
+  const int C = chunk_prefix(d, par, GE, pre, wtot);
+  const int M = C * d.nblk;
+  if (tid == 0) s_next = atomicAdd(d.qctr, 1);
+  __syncthreads();
+  int it = s_next;
+  while (it < M) {
+    int nxt = 0;
+    if (tid == 0) nxt = atomicAdd(d.qctr, 1);
+    const int b = d.blk_order[it / C], qc = it % C;
+    const int r = chunk_knot(pre, nr, qc), c = qc - pre[r];
+    pipe_item<KT, GE>(d, par, r, b, c, sh);
+    __syncthreads();  // everyone has read s_next
+    if (tid == 0) s_next = nxt;
     __syncthreads();
-    // the wave's tiles and their fragment streams
-    const int ntl = (nb + 15) / 16;
-    bool on[2];
-    const FV *A[2];
-#pragma unroll
-    for (int h = 0; h < 2; h++) {
-      const int tl = wave + 4 * h;
-      on[h] = tl < ntl;
-      const int g = d.blk_tile0[b] + (on[h] ? tl : 0);
-      A[h] = reinterpret_cast<const FV *>(static_cast<const KT *>(d.KT) + (long long)r * d.KS + d.tile_off[g]) +
-             lane;
-    }
-    FV an[2][LPS];
-    double2 bn[SPT];
-    double bw[SPT];
-    // loads of slice s into registers: fragments of the owned tiles and the
-    // chunk's amplitudes for rows s GK .. s GK + GK - 1 of [Y | Y2]
-    auto load = [&](int s) {
-#pragma unroll
-      for (int h = 0; h < 2; h++)
-#pragma unroll
-        for (int l = 0; l < LPS; l++) an[h][l] = on[h] ? A[h][(s * LPS + l) * 64] : FV{};
-#pragma unroll
-      for (int j = 0; j < SPT; j++) {
-        const int idx = tid + j * BS, kk = idx % GK, e = idx / GK, k = s * GK + kk;
-        const bool y2 = k >= nbp;
-        const int kv = y2 ? k - nbp : k;
-        const bool ok = e < nE && kv < nb;
-        bn[j] = ok ? d.V[(long long)s_t[e] * d.N + p0 + kv] : make_double2(0.0, 0.0);
-        bw[j] = y2 ? s_wy2[e] : s_wy[e];
-      }
-    };
-    auto stage = [&](int buf) {
-#pragma unroll
-      for (int j = 0; j < SPT; j++) {
-        const int idx = tid + j * BS, kk = idx % GK, e = idx / GK;
-        Bs[buf][kk][2 * e] = bw[j] * bn[j].x;
-        Bs[buf][kk][2 * e + 1] = bw[j] * bn[j].y;
-      }
-    };
-    dbl4 acc[2][GC];
-#pragma unroll
-    for (int h = 0; h < 2; h++)
-#pragma unroll
-      for (int ct = 0; ct < GC; ct++) acc[h][ct] = dbl4{0.0, 0.0, 0.0, 0.0};
-    load(0);
-    stage(0);
-    __syncthreads();
-    for (int s = 0; s < nsl; s++) {
-      const int buf = s & 1;
-      FV ac[2][LPS];
-#pragma unroll
-      for (int h = 0; h < 2; h++)
-#pragma unroll
-        for (int l = 0; l < LPS; l++) ac[h][l] = an[h][l];
-      const bool more = s + 1 < nsl;
-      if (more) load(s + 1);
-#pragma unroll
-      for (int l = 0; l < LPS; l++)
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-          // fragment order: lane group q holds row KB l + U q + u at step u
-          const double *brow = &Bs[buf][KB * l + U * q + u][col];
-          double bf[GC];
-#pragma unroll
-          for (int ct = 0; ct < GC; ct++) bf[ct] = (ct < nct) ? brow[16 * ct] : 0.0;
-#pragma unroll
-          for (int h = 0; h < 2; h++) {
-            if (!on[h]) continue;
-            const double a = F::get(ac[h][l], u);
-#pragma unroll
-            for (int ct = 0; ct < GC; ct++)
-              if (ct < nct) acc[h][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bf[ct], acc[h][ct], 0, 0, 0);
-          }
-        }
-      if (more) stage(buf ^ 1);
-      __syncthreads();
-    }
-    // f64 16x16x4 output: row lane/16 + 4i, column lane%16 of each tile
-#pragma unroll
-    for (int h = 0; h < 2; h++) {
-      if (!on[h]) continue;
-      const int rt0 = (wave + 4 * h) * 16;
-#pragma unroll
-      for (int ct = 0; ct < GC; ct++) {
-        if (ct >= nct) break;
-        const int cc = 16 * ct + col, e = cc >> 1;
-        if (e >= nE) continue;
-        double *wb = reinterpret_cast<double *>(d.W + ((long long)s_t[e] * 2 + s_side[e]) * d.N + p0) + (cc & 1);
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-          const int orow = rt0 + q + 4 * i;
-          if (orow < nb) wb[2 * orow] = acc[h][ct][i];
-        }
-      }
-    }
+    it = s_next;
   }
 }
 
 // The GEMM forms and tile shapes the runner chooses from: form, bucket
 // entries per super-tile (per wave for the LDS-free forms), K rows per LDS
 // slice, 16-row tiles per wave.
-enum GemmForm { kLds = 0, kDirect = 1, kStream = 2, kQueue = 3, kLdsQueue = 4, kPipe = 5 };
+enum GemmForm { kLds = 0, kDirect = 1, kStream = 2, kQueue = 3, kLdsQueue = 4, kPipe = 5, kPipeDyn = 6 };
 struct GemmShape {
   int form, ge, gk, grt;
 };
@@ -848,7 +901,7 @@ constexpr GemmShape kShapes[] = {
     {kLds, 64, 16, 2},      {kDirect, 32, 0, 1},  {kDirect, 16, 0, 1},  {kDirect, 8, 0, 1},
     {kStream, 8, 0, 1},     {kStream, 16, 0, 1},  {kStream, 32, 0, 1},  {kQueue, 16, 0, 1},
     {kQueue, 32, 0, 1},     {kLdsQueue, 64, 16, 1}, {kLdsQueue, 32, 16, 1}, {kPipe, 64, 16, 2},
-    {kPipe, 32, 16, 2}};
+    {kPipe, 32, 16, 2},     {kPipeDyn, 64, 16, 2},  {kPipeDyn, 32, 16, 2}};
 constexpr int kNShapes = sizeof(kShapes) / sizeof(kShapes[0]);
 // the work-queue forms keep the chunk prefix of all knots in LDS
 constexpr bool uses_queue(int shape) { return kShapes[shape].form >= kQueue; }
@@ -867,6 +920,8 @@ void launch_gemm(int shape, dim3 knots_blocks, int maxnb, hipStream_t st, const 
     case 14: gemm_lds_queue_kernel<KT, 32, 16><<<QGRID, BS, shq, st>>>(d, par); return;
     case 15: gemm_pipe_kernel<KT, 64><<<QGRID, BS, shq, st>>>(d, par); return;
     case 16: gemm_pipe_kernel<KT, 32><<<QGRID, BS, shq, st>>>(d, par); return;
+    case 17: gemm_pipe_dyn_kernel<KT, 64><<<DGRID, BS, shq, st>>>(d, par); return;
+    case 18: gemm_pipe_dyn_kernel<KT, 32><<<DGRID, BS, shq, st>>>(d, par); return;
     default: break;
   }
   auto go = [&](auto kern, int grt) {
@@ -897,6 +952,7 @@ __global__ __launch_bounds__(TB) void step_kernel(BDev d, int par) {
   extern __shared__ double sys[];  // 2N: the completed stage's vector (Coriolis neighbours)
   __shared__ double red[8];
   const int t = blockIdx.x, tid = threadIdx.x, N = d.N, n2 = 2 * N;
+  if (t == 0 && tid == 0) *d.qctr = 0;  // item counter of the next GEMM
   Ctl c = d.ctl[t];
   if (c.status != 0) return;
   double *S = d.S + (long long)t * NSLOT * n2;
@@ -1338,6 +1394,11 @@ int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *t
     }
   }
   const int ntile = (int)tile_blk.size();
+  std::vector<int> blk_order(nblk);
+  for (int b = 0; b < nblk; b++) blk_order[b] = b;
+  std::stable_sort(blk_order.begin(), blk_order.end(), [&](int x, int y) {
+    return blk_start[x + 1] - blk_start[x] > blk_start[y + 1] - blk_start[y];
+  });
   // 64-row groups of the blocks (the LDS-staged work queue)
   std::vector<int> rg_blk, rg_row0;
   for (int b = 0; b < nblk; b++)
@@ -1348,13 +1409,14 @@ int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *t
   const int nrg = (int)rg_blk.size();
   if (maxnb > 128) {
     // the pipelined form holds at most two 16-row tiles per wave (128 rows)
-    if (ds->gemm_fixed >= 0 && kShapes[ds->gemm_fixed].form == kPipe)
+    if (ds->gemm_fixed >= 0 && kShapes[ds->gemm_fixed].form >= kPipe)
       return mqct_fail(MQCT_ERR_ARG, "batched: the pipelined GEMM form takes m-blocks of at most 128 states");
   }
-  DevBuf d_tb, d_tr0, d_to, d_bt0, d_rgb, d_rgr;
+  DevBuf d_tb, d_tr0, d_to, d_bt0, d_rgb, d_rgr, d_bord;
   if (up(d_tb, tile_blk.data(), ntile * 4) || up(d_tr0, tile_r0.data(), ntile * 4) ||
       up(d_to, tile_off.data(), ntile * 8) || up(d_bt0, blk_tile0.data(), nblk * 4) ||
-      up(d_rgb, rg_blk.data(), nrg * 4) || up(d_rgr, rg_row0.data(), nrg * 4))
+      up(d_rgb, rg_blk.data(), nrg * 4) || up(d_rgr, rg_row0.data(), nrg * 4) ||
+      up(d_bord, blk_order.data(), nblk * 4))
     return mqct_fail(MQCT_ERR_NOMEM, "batched: tile tables");
   if (!ds->kt_valid || ds->kt_fp32 != knots_fp32) {
     const size_t need_kt = (size_t)n_r * KS * (knots_fp32 ? 4 : 8);
@@ -1404,6 +1466,7 @@ int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *t
   d.tile_blk = d_tb.as<int>();
   d.tile_r0 = d_tr0.as<int>();
   d.blk_tile0 = d_bt0.as<int>();
+  d.blk_order = d_bord.as<int>();
   d.tile_off = d_to.as<long long>();
   d.nrg = nrg;
   d.rg_blk = d_rgb.as<int>();
@@ -1423,6 +1486,7 @@ int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *t
   d.active = reinterpret_cast<int *>(base + off_act);
   d.nbucketed = ds->prof_on ? reinterpret_cast<unsigned long long *>(base + off_act + 8) : nullptr;
   d.ntouched = reinterpret_cast<unsigned long long *>(base + off_act + 16);
+  d.qctr = reinterpret_cast<int *>(base + off_act + 24);
   HIP_TRY(hipMemsetAsync(d.cnt, 0, (size_t)2 * n_r * 4, h->stream));
   HIP_TRY(hipMemsetAsync(d.active, 0, 32, h->stream));
   HIP_TRY(hipMemcpyAsync(d.active, &n_traj, 4, hipMemcpyHostToDevice, h->stream));
@@ -1527,7 +1591,7 @@ int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *t
       double tot[kNShapes] = {};
       bool keep[kNShapes];
       for (int s = 0; s < kNShapes; s++)
-        keep[s] = (queue_ok || !uses_queue(s)) && (kShapes[s].form != kPipe || maxnb <= 128);
+        keep[s] = (queue_ok || !uses_queue(s)) && (kShapes[s].form < kPipe || maxnb <= 128);
       for (int round = 0; round < 3 && active > 0; round++) {
         for (int s = 0; s < ncand && active > 0; s++) {
           if (!keep[s]) continue;
