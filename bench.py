@@ -94,7 +94,8 @@ def main():
                     help="TIME_STEP (a.u.) of the two-tops rk4 propagation")
     ap.add_argument("--c3-plain", type=int, default=32, help="two-tops trajectories timed in the plain form")
     ap.add_argument("--sur-samples", type=int, default=4_000_000)
-    ap.add_argument("--sur-steps", type=int, default=20000)
+    ap.add_argument("--sur-steps", type=int, default=50000)
+    ap.add_argument("--sur-batch", type=int, default=8192)
     ap.add_argument("--sur-rmin", type=float, default=5.0,
                     help="the surrogate covers R >= this (bohr); the plain PES fills the grid below")
     args = ap.parse_args()
@@ -391,8 +392,8 @@ def two_tops_section(dev, args):
             "" if prec == "fp64" else ", xsec err %.2e, elem err max %.2e" % (rec["xsec_rel_err"],
                                                                              rec["elem_err"]["max"])))
     # the learned potential: trained on the GPU, fills the V grid, split-BF16 build
-    sur, stats = train_surrogate(dev, wl, n_samples=args.sur_samples, steps=args.sur_steps, r_lo=args.sur_rmin,
-                                 log=log)
+    sur, stats = train_surrogate(dev, wl, n_samples=args.sur_samples, steps=args.sur_steps, batch=args.sur_batch,
+                                 r_lo=args.sur_rmin, log=log)
     mb2 = MatrixBuilder(dev, wl)  # plain V grid: the fallback below r_min
     par = L.as_f64(wl.pes_par)
     R = L.as_f64(wl.R)

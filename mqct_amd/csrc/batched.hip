@@ -865,7 +865,7 @@ __global__ __launch_bounds__(BS, GE >= 64 ? 1 : 2) void gemm_pipe_kernel(BDev d,
 // items fill in at the end.  Each workgroup fetches its next item while it
 // works on the current one.
 template <typename KT, int GE>
-__global__ __launch_bounds__(BS, GE >= 64 ? 1 : 2) void gemm_pipe_dyn_kernel(BDev d, int par) {
+__global__ __launch_bounds__(BS, GE >= 64 ? 1 : (GE >= 32 ? 2 : 3)) void gemm_pipe_dyn_kernel(BDev d, int par) {
   extern __shared__ int pre[];  // [n_r + 1] chunk prefix
   __shared__ int wtot[BS / 64];
   __shared__ PipeShared<KT, GE> sh;
@@ -901,7 +901,7 @@ constexpr GemmShape kShapes[] = {
     {kLds, 64, 16, 2},      {kDirect, 32, 0, 1},  {kDirect, 16, 0, 1},  {kDirect, 8, 0, 1},
     {kStream, 8, 0, 1},     {kStream, 16, 0, 1},  {kStream, 32, 0, 1},  {kQueue, 16, 0, 1},
     {kQueue, 32, 0, 1},     {kLdsQueue, 64, 16, 1}, {kLdsQueue, 32, 16, 1}, {kPipe, 64, 16, 2},
-    {kPipe, 32, 16, 2},     {kPipeDyn, 64, 16, 2},  {kPipeDyn, 32, 16, 2}};
+    {kPipe, 32, 16, 2},     {kPipeDyn, 64, 16, 2},  {kPipeDyn, 32, 16, 2},  {kPipeDyn, 16, 16, 2}};
 constexpr int kNShapes = sizeof(kShapes) / sizeof(kShapes[0]);
 // the work-queue forms keep the chunk prefix of all knots in LDS
 constexpr bool uses_queue(int shape) { return kShapes[shape].form >= kQueue; }
@@ -922,6 +922,7 @@ void launch_gemm(int shape, dim3 knots_blocks, int maxnb, hipStream_t st, const 
     case 16: gemm_pipe_kernel<KT, 32><<<QGRID, BS, shq, st>>>(d, par); return;
     case 17: gemm_pipe_dyn_kernel<KT, 64><<<DGRID, BS, shq, st>>>(d, par); return;
     case 18: gemm_pipe_dyn_kernel<KT, 32><<<DGRID, BS, shq, st>>>(d, par); return;
+    case 19: gemm_pipe_dyn_kernel<KT, 16><<<DGRID, BS, shq, st>>>(d, par); return;
     default: break;
   }
   auto go = [&](auto kern, int grt) {

@@ -20,6 +20,7 @@
 //          sums per chunk, FP64 accumulators.
 #include "mqct_internal.h"
 
+#include <algorithm>
 #include <chrono>
 #include <type_traits>
 
@@ -315,6 +316,39 @@ __global__ __launch_bounds__(64 * NW) void contract_v_kernel(
 }
 
 // ---------------------------------------------------------------------------
+// Eight FP32 values split by truncation into three BF16 terms each,
+// x = x0 + x1 + x2 exactly: x0 keeps the top 8 significant bits of x (the
+// upper half of its encoding), x1 the top 8 of the remainder, and x2 the
+// rest, which has at most 8 significant bits left.  Bit masks and
+// subtractions replace round-trip conversions, and one byte permute packs
+// two upper halves into a dword of the MFMA operand.
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+__device__ inline void split3_pack8(const float (&x)[8], bf16x8 &t0, bf16x8 &t1, bf16x8 &t2) {
+  u32x4 w0, w1, w2;
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    unsigned h[2], m[2], l[2];
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      const float v = x[2 * j + q];
+      const unsigned hb = __float_as_uint(v) & 0xffff0000u;
+      const float r1 = v - __uint_as_float(hb);
+      const unsigned mb = __float_as_uint(r1) & 0xffff0000u;
+      h[q] = hb;
+      m[q] = mb;
+      l[q] = __float_as_uint(r1 - __uint_as_float(mb));
+    }
+    // bytes 2-3 of the first operand's pair member, then of the second
+    w0[j] = __builtin_amdgcn_perm(h[1], h[0], 0x07060302u);
+    w1[j] = __builtin_amdgcn_perm(m[1], m[0], 0x07060302u);
+    w2[j] = __builtin_amdgcn_perm(l[1], l[0], 0x07060302u);
+  }
+  t0 = __builtin_bit_cast(bf16x8, w0);
+  t1 = __builtin_bit_cast(bf16x8, w1);
+  t2 = __builtin_bit_cast(bf16x8, w2);
+}
+
+// ---------------------------------------------------------------------------
 // Three-term split-BF16 kernel.  Each operand is carried as three BF16
 // terms, x = x0 + x1 + x2 (about 24 significant bits): the weights come
 // split from HBM, the pair product p = Re_a Re_b + Im_a Im_b (FP32) is split
@@ -322,13 +356,8 @@ __global__ __launch_bounds__(64 * NW) void contract_v_kernel(
 //   p2 w0 + p1 w1 + p0 w2 + p1 w0 + p0 w1 + p0 w0,
 // run on v_mfma_f32_16x16x32_bf16 into FP32 partial sums over each staged
 // chunk of KC grid points, added into FP64 accumulators; the omitted terms
-// are below 2^-24 relative.
-__device__ inline void split3_bf16(float x, __bf16 &b0, __bf16 &b1, __bf16 &b2) {
-  b0 = (__bf16)x;
-  const float r1 = x - (float)b0;
-  b1 = (__bf16)r1;
-  b2 = (__bf16)(r1 - (float)b1);
-}
+// are below 2^-24 relative.  (The weights are split by rounding to nearest
+// where they are made; the pair products by truncation, split3_pack8.)
 
 template <int KC>
 __global__ __launch_bounds__(256) void contract_bf16x3_kernel(
@@ -391,16 +420,13 @@ __global__ __launch_bounds__(256) void contract_bf16x3_kernel(
       for (int a = 0; a < AW; a++) {
         int al = wave * AW + a;
         bf16x8 p0, p1, p2;
+        float pv[8];
 #pragma unroll
         for (int j = 0; j < 8; j++) {
-          float p = sAr[al][k0 + j] * br[j];
-          if (has_im) p += sAi[al][k0 + j] * bi[j];
-          __bf16 h0, h1, h2;
-          split3_bf16(p, h0, h1, h2);
-          p0[j] = h0;
-          p1[j] = h1;
-          p2[j] = h2;
+          pv[j] = sAr[al][k0 + j] * br[j];
+          if (has_im) pv[j] += sAi[al][k0 + j] * bi[j];
         }
+        split3_pack8(pv, p0, p1, p2);
 #pragma unroll
         for (int n = 0; n < NT; n++) {
           const bf16x8 w0 = *reinterpret_cast<const bf16x8 *>(&sW[0][n * 16 + row][k0]);
@@ -534,16 +560,13 @@ __global__ __launch_bounds__(64 * NW) void contract_bf16x3_v_kernel(
       for (int a = 0; a < AWn; a++) {
         const int al = wave * AWn + a;
         bf16x8 p0, p1, p2;
+        float pv[8];
 #pragma unroll
         for (int j = 0; j < 8; j++) {
-          float p = sA[buf][0][al][k0 + j] * br[j];
-          if (has_im) p += sA[buf][1][al][k0 + j] * bi[j];
-          __bf16 h0, h1, h2;
-          split3_bf16(p, h0, h1, h2);
-          p0[j] = h0;
-          p1[j] = h1;
-          p2[j] = h2;
+          pv[j] = sA[buf][0][al][k0 + j] * br[j];
+          if (has_im) pv[j] += sA[buf][1][al][k0 + j] * bi[j];
         }
+        split3_pack8(pv, p0, p1, p2);
 #pragma unroll
         for (int n = 0; n < NTB; n++) {
           const bf16x8 w0 = *reinterpret_cast<const bf16x8 *>(&sW[buf][0][n * 16 + row][k0]);
@@ -582,28 +605,36 @@ __global__ __launch_bounds__(64 * NW) void contract_bf16x3_v_kernel(
 }
 
 // ---------------------------------------------------------------------------
-// FP64 sums of one tile's pairs at the two cutoff R points on the vector
-// ALU: one thread per (a, b) pair of the 16 x 16 tile.  Basis rows and the
-// two weight rows pass through LDS in chunks of CK grid points, read two
-// points at a time as 16-byte vectors.  Two output columns leave the MFMA
-// tiles almost empty, while here every pair product feeds both sums at once.
+// FP64 sums of every pair at the two cutoff R points, on the vector ALU (a
+// 16-column MFMA tile would be 7/8 empty).  With two output columns the
+// work is bound by reading the basis rows, so the tiles here are 64 x 64
+// pairs (CT): 128 rows read per 4096 pairs.  Each thread owns the 4 x 4
+// pairs (ta + 16 i, tb + 16 j) of the tile, so one pass over LDS feeds
+// sixteen pair products; basis rows and the two weight rows pass through
+// LDS in chunks of CK grid points, two points per 16-byte read.  The grid
+// is split into CS slices for enough workgroups; per-slice partial sums go
+// to part[slice][pair][2] and cutoff_reduce_kernel adds them in slice
+// order, so the result does not depend on scheduling.
+constexpr int CS = 16;  // grid slices of the cutoff sums
+constexpr int CT = 64;  // states per side of a cutoff tile
 template <int CK>
 __global__ __launch_bounds__(256) void cutoff_fp64_kernel(
     const double *__restrict__ pa_re, const double *__restrict__ pa_im,
     const double *__restrict__ pb_re, const double *__restrict__ pb_im,
-    const double *__restrict__ Wc, long long G, const TileDesc *__restrict__ tiles, OutMap om,
-    double *__restrict__ out) {
+    const double *__restrict__ Wc, long long G, const TileDesc *__restrict__ tiles, OutMap om, long long T,
+    double *__restrict__ part) {
   constexpr int CP = CK + 2;  // row stride: 16-byte aligned rows
-  __shared__ __attribute__((aligned(16))) double sA[2][TA][CP], sB[2][TB][CP], sW[2][CK];
+  __shared__ __attribute__((aligned(16))) double sA[2][CT][CP], sB[2][CT][CP], sW[2][CK];
   const TileDesc t = tiles[blockIdx.x];
-  const int tid = threadIdx.x, al = tid >> 4, bl = tid & 15;
+  const int tid = threadIdx.x, ta = tid >> 4, tb = tid & 15;
   const bool has_im = pa_im != nullptr;
-  double s0 = 0.0, s1 = 0.0;
-  for (long long g0 = 0; g0 < G; g0 += CK) {
-    for (int e = tid; e < TA * CK; e += 256) {
+  const long long gs = (G + CS - 1) / CS, g_lo = blockIdx.y * gs, g_hi = min(G, g_lo + gs);
+  double s[4][4][2] = {};  // [a][b][R point]
+  for (long long g0 = g_lo; g0 < g_hi; g0 += CK) {
+    for (int e = tid; e < CT * CK; e += 256) {
       const int r = e / CK, c = e % CK;
       const long long g = g0 + c;
-      const bool ok = g < G, oka = ok && r < t.na, okb = ok && r < t.nb;
+      const bool ok = g < g_hi, oka = ok && r < t.na, okb = ok && r < t.nb;
       const long long ia = (long long)(t.a0 + r) * G + g, ib = (long long)(t.b0 + r) * G + g;
       sA[0][r][c] = oka ? pa_re[ia] : 0.0;
       sB[0][r][c] = okb ? pb_re[ib] : 0.0;
@@ -613,31 +644,57 @@ __global__ __launch_bounds__(256) void cutoff_fp64_kernel(
     for (int e = tid; e < 2 * CK; e += 256) {
       const int q = e / CK, c = e % CK;
       const long long g = g0 + c;
-      sW[q][c] = g < G ? Wc[q * G + g] : 0.0;
+      sW[q][c] = g < g_hi ? Wc[q * G + g] : 0.0;
     }
     __syncthreads();
-#pragma unroll 4
     for (int c = 0; c < CK; c += 2) {
-      const double2 ar = *reinterpret_cast<const double2 *>(&sA[0][al][c]);
-      const double2 ai = *reinterpret_cast<const double2 *>(&sA[1][al][c]);
-      const double2 br = *reinterpret_cast<const double2 *>(&sB[0][bl][c]);
-      const double2 bi = *reinterpret_cast<const double2 *>(&sB[1][bl][c]);
+      double2 ar[4], ai[4], br[4], bi[4];
+#pragma unroll
+      for (int h = 0; h < 4; h++) {
+        ar[h] = *reinterpret_cast<const double2 *>(&sA[0][ta + 16 * h][c]);
+        ai[h] = *reinterpret_cast<const double2 *>(&sA[1][ta + 16 * h][c]);
+        br[h] = *reinterpret_cast<const double2 *>(&sB[0][tb + 16 * h][c]);
+        bi[h] = *reinterpret_cast<const double2 *>(&sB[1][tb + 16 * h][c]);
+      }
       const double2 w0 = *reinterpret_cast<const double2 *>(&sW[0][c]);
       const double2 w1 = *reinterpret_cast<const double2 *>(&sW[1][c]);
-      const double px = ar.x * br.x + ai.x * bi.x, py = ar.y * br.y + ai.y * bi.y;
-      s0 += w0.x * px;
-      s1 += w1.x * px;
-      s0 += w0.y * py;
-      s1 += w1.y * py;
+#pragma unroll
+      for (int x = 0; x < 4; x++)
+#pragma unroll
+        for (int y = 0; y < 4; y++) {
+          const double px = ar[x].x * br[y].x + ai[x].x * bi[y].x;
+          const double py = ar[x].y * br[y].y + ai[x].y * bi[y].y;
+          s[x][y][0] += w0.x * px;
+          s[x][y][1] += w1.x * px;
+          s[x][y][0] += w0.y * py;
+          s[x][y][1] += w1.y * py;
+        }
     }
     __syncthreads();
   }
-  if (al >= t.na || bl >= t.nb) return;
-  const int a = t.a0 + al, b = t.b0 + bl;
-  if (b > a) return;  // lower triangle only
-  const long long k = om.kbase[om.pos_state[a]] + (b - om.block_of[a]);
-  out[k * om.ld] = s0 * om.conv;
-  out[k * om.ld + 1] = s1 * om.conv;
+#pragma unroll
+  for (int x = 0; x < 4; x++)
+#pragma unroll
+    for (int y = 0; y < 4; y++) {
+      const int la = ta + 16 * x, lb = tb + 16 * y;
+      if (la >= t.na || lb >= t.nb) continue;
+      const int a = t.a0 + la, b = t.b0 + lb;
+      if (b > a) continue;  // lower triangle only
+      const long long k = om.kbase[om.pos_state[a]] + (b - om.block_of[a]);
+      double *o = part + ((long long)blockIdx.y * T + k) * 2;
+      o[0] = s[x][y][0];
+      o[1] = s[x][y][1];
+    }
+}
+
+// cutv[k][j] = conv * sum over slices of part[slice][k][j], slices in order
+__global__ void cutoff_reduce_kernel(const double *__restrict__ part, long long T, double conv,
+                                     double *__restrict__ cutv) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 2 * T) return;
+  double acc = 0.0;
+  for (int sl = 0; sl < CS; sl++) acc += part[(long long)sl * 2 * T + i];
+  cutv[i] = acc * conv;
 }
 
 // ---------------------------------------------------------------------------
@@ -834,9 +891,18 @@ extern "C" int mqct_hip_contract_variant(mqct_hip_t *h, int precision, int *v, i
 // is then taken from the same FP64 sums the FP64 build forms.  Output
 // [T][2] in d_cutv; W64 rows of the two R points in d_Wcut.
 int mqct_run_cutoff_fp64(mqct_hip *h, int ir1, int ir2, double conv) {
-  int ntiles = 0, rc;
-  if ((rc = ensure_tiles(h, ntiles))) return rc;
+  // 64 x 64 tiles (b-tile <= a-tile) of every m-block
+  std::vector<TileDesc> ct;
+  for (size_t blk = 0; blk + 1 < h->block_start.size(); blk++) {
+    const int s0 = h->block_start[blk], s1 = h->block_start[blk + 1];
+    for (int a0 = s0; a0 < s1; a0 += CT)
+      for (int b0 = s0; b0 <= a0; b0 += CT)
+        ct.push_back(TileDesc{a0, std::min(CT, s1 - a0), b0, std::min(CT, s1 - b0)});
+  }
+  const int ntiles = (int)ct.size();
   if (ntiles == 0) return MQCT_OK;
+  if (h->d_ctiles.ensure(ct.size() * sizeof(TileDesc))) return mqct_fail(MQCT_ERR_NOMEM, "cutoff tiles");
+  HIP_TRY(hipMemcpyAsync(h->d_ctiles.p, ct.data(), ct.size() * sizeof(TileDesc), hipMemcpyHostToDevice, h->stream));
   if (h->d_Wcut.ensure((size_t)2 * h->G * 8) ||
       h->d_cutv.ensure((size_t)h->built_T * 2 * 8))
     return mqct_fail(MQCT_ERR_NOMEM, "cutoff columns");
@@ -847,9 +913,14 @@ int mqct_run_cutoff_fp64(mqct_hip *h, int ir1, int ir2, double conv) {
   const bool im = (h->coll_type != 1) || !h->ref_xz;
   const double *pbr = h->ref_xz ? h->d_psi_re.as<double>() : h->d_psib_re.as<double>();
   const double *pbi = h->ref_xz ? h->d_psi_im.as<double>() : h->d_psib_im.as<double>();
-  cutoff_fp64_kernel<64><<<dim3((unsigned)ntiles), 256, 0, h->stream>>>(
+  const long long T = h->built_T;
+  if (h->d_cutp.ensure((size_t)CS * T * 2 * 8)) return mqct_fail(MQCT_ERR_NOMEM, "cutoff partial sums");
+  cutoff_fp64_kernel<16><<<dim3((unsigned)ntiles, CS), 256, 0, h->stream>>>(
       h->d_psi_re.as<double>(), im ? h->d_psi_im.as<double>() : nullptr, pbr, im ? pbi : nullptr,
-      h->d_Wcut.as<double>(), h->G, h->d_tiles.as<TileDesc>(), om, h->d_cutv.as<double>());
+      h->d_Wcut.as<double>(), h->G, h->d_ctiles.as<TileDesc>(), om, T, h->d_cutp.as<double>());
+  HIP_TRY(hipGetLastError());
+  cutoff_reduce_kernel<<<(unsigned)((2 * T + 255) / 256), 256, 0, h->stream>>>(h->d_cutp.as<double>(), T, conv,
+                                                                                h->d_cutv.as<double>());
   HIP_TRY(hipGetLastError());
   return MQCT_OK;
 }

@@ -85,9 +85,9 @@ struct mqct_hip {
   DevBuf d_psi32_re, d_psi32_im, d_psib32_re, d_psib32_im;
   DevBuf d_W64, d_W32, d_Whi, d_Wlo, d_Wlo2;  // [n_r][G]; BF16 terms of W
   DevBuf d_tab1, d_tab2;             // type 0 per-level one-molecule tables
-  DevBuf d_tiles;
+  DevBuf d_tiles, d_ctiles;          // contraction tiles; 64 x 64 tiles of the cutoff sums
   DevBuf d_mat, d_der, d_skip, d_spl;  // outputs [T][n_r]
-  DevBuf d_Wcut, d_cutv;             // FP64 cutoff columns (mixed precision)
+  DevBuf d_Wcut, d_cutv, d_cutp;     // FP64 cutoff columns (mixed precision), per-slice partial sums
   int64_t built_T = 0;
   bool built = false;
   bool psi_fresh = false;

@@ -186,3 +186,10 @@ def test_contraction_variants(dev, prec, variant):
         assert v.value == variant and kc.value > 0 and nw.value in (4, 8) and nc.value > variant
     finally:
         L.check(dev.lib.mqct_hip_set_contract_variant(dev.h, p, -1), "set_contract_variant")
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+def test_mixed_precision_unaligned_grid(dev, prec):
+    """A grid of 45 points (5 x 9), not a multiple of the 16-byte staging
+    width: the builds take the scalar-staging kernels."""
+    check_mixed(dev, W.asym_top_atom(jmax=2, n_beta=5, n_gamma=9, n_r=20), prec)

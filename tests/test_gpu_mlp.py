@@ -95,3 +95,33 @@ def test_surrogate_vgrid_is_the_network_on_the_grid(dev):
     assert np.array_equal(S1, S2)
     scale = np.abs(M1).max()
     assert np.max(np.abs(M1 - M2)) <= 1e-12 * scale
+
+
+def test_surrogate_gain_over_host_pes(dev, data):
+    """What the network buys: the potential on two-tops geometries from the
+    trained network on the GPU, against the host's PES routine (the
+    oracle's C site-site potential filling an 8^5 x 8 grid on one core, as
+    the reference fills its grid point by point).  The network must be at
+    least 10x faster per point (measured: about 80x); its error on random
+    points is printed beside the timing."""
+    import time
+    import oracle.oracle as O
+    wl, geo, V = data
+    sur = Surrogate(dev, seed=15, scale=20.0)
+    sur.set_data(geo, V)
+    sur.fit(steps=400, batch=512, lr0=2e-3, lr1=1e-4)
+    pts = sample_geometries(262144, 5.0, 25.0, seed=23)
+    sur.predict(pts[:4096])  # first call: kernels loaded
+    t0 = time.perf_counter()
+    Vn = sur.predict(pts)
+    t_net = (time.perf_counter() - t0) / len(pts)
+    ws = W.two_asym_tops(jmax1=0, jmax2=0, n_ang=8, n_r=8)
+    t0 = time.perf_counter()
+    O.oracle_for_workload(ws)
+    t_host = (time.perf_counter() - t0) / (ws.grid.G * len(ws.R))
+    Vp = pes_points(dev, wl, pts[:20000])
+    err = np.abs(Vn[:20000] - Vp)
+    print(f"network {t_net * 1e9:.1f} ns/point, host PES {t_host * 1e9:.1f} ns/point "
+          f"(x{t_host / t_net:.0f}); |V_net - V| rms {np.sqrt(np.mean(err ** 2)):.1f} cm^-1")
+    assert np.all(np.isfinite(Vn))
+    assert t_host >= 10.0 * t_net
