@@ -149,13 +149,15 @@ __device__ inline double stage_time(const Ctl &c) {
 }
 
 // Prepares stage c.stage from its vector x (the thread's states in
-// registers: position i = tid + j BS, Re at i, Im at i + N): writes
-// v = exp(-i E tau) x to V; thread 0 evaluates the recorded path at tau
+// registers: position i = tid + j BS, Re at i, Im at i + N; Ej their
+// energies): writes v = exp(-i E tau) x to V; thread 0 evaluates the
+// recorded path at tau
 // (dPhi/dt for the Coriolis term), finds the knots and spline weights,
 // records them in its c and appends the trajectory to the buckets of
 // parity `par`.
 template <int PT, int TB = BS>
-__device__ void prepare_stage(const BDev &d, int t, Ctl &c, const double *xr, const double *xi, int par) {
+__device__ void prepare_stage(const BDev &d, int t, Ctl &c, const double *xr, const double *xi, int par,
+                              const double *Ej) {
   const int tid = threadIdx.x, N = d.N;
   const double tau = stage_time(c);
   double2 *v = d.V + (long long)t * N;
@@ -164,7 +166,7 @@ __device__ void prepare_stage(const BDev &d, int t, Ctl &c, const double *xr, co
     const int i = tid + j * TB;
     if (i < N) {
       double sn, cs;
-      sincos(tau * d.E[i], &sn, &cs);
+      sincos(tau * Ej[j], &sn, &cs);
       v[i] = make_double2(xr[j] * cs + xi[j] * sn, xi[j] * cs - xr[j] * sn);
     }
   }
@@ -962,21 +964,46 @@ __global__ __launch_bounds__(TB) void step_kernel(BDev d, int par) {
   // -- k = -i exp(i E tau) (W_lo + W_hi) + Coriolis(stage vector), the stage
   //    vector formed again from its rotated amplitudes
   double kr[PT], ki[PT];
+  // every load of the thread's states first (energies, rotated stage vector,
+  // both knot sides' products, y and the Coriolis tables), so their
+  // latencies overlap instead of following one another state by state
+  double Ej[PT], yr[PT], yi[PT];
   {
     const double2 *vv = d.V + (long long)t * N;
     const double2 *W0 = d.W + ((long long)t * 2) * N, *W1 = W0 + N;
+    double2 vj[PT], w0j[PT], w1j[PT], fj[PT];
+    int2 nj[PT];
+#pragma unroll
+    for (int j = 0; j < PT; j++) {
+      const int i = tid + j * TB;
+      Ej[j] = yr[j] = yi[j] = 0.0;
+      vj[j] = w0j[j] = w1j[j] = fj[j] = make_double2(0.0, 0.0);
+      nj[j] = make_int2(-1, -1);
+      if (i < N) {
+        Ej[j] = d.E[i];
+        vj[j] = vv[i];
+        if (!c.zero) {
+          w0j[j] = W0[i];
+          w1j[j] = W1[i];
+        }
+        yr[j] = y[i];
+        yi[j] = y[i + N];
+        nj[j] = d.nb[i];
+        fj[j] = d.sq[i];
+      }
+    }
 #pragma unroll
     for (int j = 0; j < PT; j++) {
       const int i = tid + j * TB;
       kr[j] = ki[j] = 0.0;
       if (i < N) {
         double sn, cs;
-        sincos(c.tau * d.E[i], &sn, &cs);
-        const double2 vi = vv[i];
+        sincos(c.tau * Ej[j], &sn, &cs);
+        const double2 vi = vj[j];
         sys[i] = vi.x * cs - vi.y * sn;
         sys[i + N] = vi.y * cs + vi.x * sn;
         if (!c.zero) {
-          const double wx = W0[i].x + W1[i].x, wy = W0[i].y + W1[i].y;
+          const double wx = w0j[j].x + w1j[j].x, wy = w0j[j].y + w1j[j].y;
           kr[j] = sn * wx + cs * wy;
           ki[j] = -(cs * wx - sn * wy);
         }
@@ -989,8 +1016,8 @@ __global__ __launch_bounds__(TB) void step_kernel(BDev d, int par) {
       const int i = tid + j * TB;
       if (i < N) {
         // state-diagonal Coriolis coupling (mqct_eqtns.f:845-877)
-        const int2 nbr = d.nb[i];
-        const double2 f = d.sq[i];
+        const int2 nbr = nj[j];
+        const double2 f = fj[j];
         if (nbr.x >= 0) {
           kr[j] += hd * f.x * sys[nbr.x + N];
           ki[j] -= hd * f.x * sys[nbr.x];
@@ -1023,8 +1050,8 @@ __global__ __launch_bounds__(TB) void step_kernel(BDev d, int par) {
     for (int j = 0; j < PT; j++) {
       const int i = tid + j * TB;
       if (i < N) {
-        xr[j] = y[i] + f * kr[j];
-        xi[j] = y[i + N] + f * ki[j];
+        xr[j] = yr[j] + f * kr[j];
+        xi[j] = yi[j] + f * ki[j];
       }
     }
   };
@@ -1041,8 +1068,8 @@ __global__ __launch_bounds__(TB) void step_kernel(BDev d, int par) {
       for (int j = 0; j < PT; j++) {
         const int i = tid + j * TB;
         if (i < N) {
-          xr[j] = y[i] + h6 * (k1[i] + kr[j] + 2.0 * (k2[i] + k3[i]));
-          xi[j] = y[i + N] + h6 * (k1[i + N] + ki[j] + 2.0 * (k2[i + N] + k3[i + N]));
+          xr[j] = yr[j] + h6 * (k1[i] + kr[j] + 2.0 * (k2[i] + k3[i]));
+          xi[j] = yi[j] + h6 * (k1[i + N] + ki[j] + 2.0 * (k2[i + N] + k3[i + N]));
           y[i] = xr[j];
           y[i + N] = xi[j];
           nrm += xr[j] * xr[j] + xi[j] * xi[j];
@@ -1096,8 +1123,8 @@ __global__ __launch_bounds__(TB) void step_kernel(BDev d, int par) {
         }
         ar += CK_B[s + 1][s] * kr[j];
         ai += CK_B[s + 1][s] * ki[j];
-        xr[j] = y[i] + c.h * ar;
-        xi[j] = y[i + N] + c.h * ai;
+        xr[j] = yr[j] + c.h * ar;
+        xi[j] = yi[j] + c.h * ai;
       }
     }
   } else {
@@ -1117,9 +1144,10 @@ __global__ __launch_bounds__(TB) void step_kernel(BDev d, int par) {
           const int ii = i + p * N;
           const double k6 = p ? ki[j] : kr[j];
           const double yerr = c.h * (DC1 * k1[ii] + DC3 * k3[ii] + DC4 * k4[ii] + DC5 * k5[ii] + DC6 * k6);
-          const double ysc = fabs(y[ii]) + fabs(c.h0 * k1[ii]) + TINY;
+          const double yv = p ? yi[j] : yr[j];
+          const double ysc = fabs(yv) + fabs(c.h0 * k1[ii]) + TINY;
           em = fmax(em, fabs(yerr / ysc));
-          const double yn = y[ii] + c.h * (C1 * k1[ii] + C3 * k3[ii] + C4 * k4[ii] + C6 * k6);
+          const double yn = yv + c.h * (C1 * k1[ii] + C3 * k3[ii] + C4 * k4[ii] + C6 * k6);
           if (p) xi[j] = yn;
           else xr[j] = yn;
         }
@@ -1137,8 +1165,8 @@ __global__ __launch_bounds__(TB) void step_kernel(BDev d, int par) {
       for (int j = 0; j < PT; j++) {
         const int i = tid + j * TB;
         if (i < N) {
-          xr[j] = y[i] + CK_B[1][0] * c.h * k1[i];
-          xi[j] = y[i + N] + CK_B[1][0] * c.h * k1[i + N];
+          xr[j] = yr[j] + CK_B[1][0] * c.h * k1[i];
+          xi[j] = yi[j] + CK_B[1][0] * c.h * k1[i + N];
         }
       }
     } else {
@@ -1169,7 +1197,7 @@ __global__ __launch_bounds__(TB) void step_kernel(BDev d, int par) {
     }
     return;
   }
-  prepare_stage<PT, TB>(d, t, c, xr, xi, par);
+  prepare_stage<PT, TB>(d, t, c, xr, xi, par, Ej);
   if (tid == 0) d.ctl[t] = c;
 }
 
@@ -1238,12 +1266,13 @@ __global__ __launch_bounds__(BS) void init_kernel(BDev d, const mqct_traj_t *__r
   double *S = d.S + (long long)t * NSLOT * n2;
   const mqct_traj_t p = tr[t];
   const int s0 = state_pos[p.s_st - 1];
-  double xr[PT], xi[PT];
+  double xr[PT], xi[PT], Ej[PT];
 #pragma unroll
   for (int j = 0; j < PT; j++) {
     const int i = tid + j * BS;
-    xr[j] = xi[j] = 0.0;
+    xr[j] = xi[j] = Ej[j] = 0.0;
     if (i < N) {
+      Ej[j] = d.E[i];
       xr[j] = (i == s0) ? 1.0 : 0.0;
       S[i] = xr[j];
       S[i + N] = 0.0;
@@ -1264,7 +1293,7 @@ __global__ __launch_bounds__(BS) void init_kernel(BDev d, const mqct_traj_t *__r
   c.hmin = p.hmin;
   c.eps = p.eps;
   c.path = p.at_path;
-  prepare_stage<PT>(d, t, c, xr, xi, 0);
+  prepare_stage<PT>(d, t, c, xr, xi, 0, Ej);
   if (tid == 0) d.ctl[t] = c;
 }
 
