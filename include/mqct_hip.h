@@ -208,7 +208,7 @@ int mqct_hip_batched_profile(mqct_hip_t *h, int on);
  * ms, shape index, summed step-kernel ms); copies min(cap, n) rows of 5
  * doubles into rows and returns their number in n. */
 int mqct_hip_batched_prof_trace(mqct_hip_t *h, int cap, double *rows, int *n);
-/* The interval-GEMM candidates: shape index 0..19 (shape = -1: the shape
+/* The interval-GEMM candidates: shape index 0..21 (shape = -1: the shape
  * the last batched run started with, for the coefficient precision of that
  * run).  form: 0 LDS-staged per knot, 1 per-wave loads per knot, 2 the same
  * over the fragment-ordered knot tiles, 3 fragment-ordered work queue of
@@ -227,7 +227,7 @@ int mqct_hip_batched_tile(mqct_hip_t *h, int shape, int *form, int *entries,
  * levels 0..n-1 for the FP64 (fp32=0) or FP32 (fp32=1) knot matrices. */
 int mqct_hip_batched_tile_levels(mqct_hip_t *h, int fp32, int n, int *shapes);
 /* Fixes the interval-GEMM tile shape (index into the candidate list,
- * 0..19) for the following batched runs, or -1 to choose by timing again. */
+ * 0..21) for the following batched runs, or -1 to choose by timing again. */
 int mqct_hip_batched_set_tile(mqct_hip_t *h, int shape);
 /* Threads per trajectory of the batched step kernel: 256 or 512, or 0 for
  * the automatic choice (512 when the states exceed 1024). */

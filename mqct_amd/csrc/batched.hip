@@ -691,10 +691,10 @@ __global__ __launch_bounds__(BS) void gemm_lds_queue_kernel(BDev d, int par) {
 // double-buffered LDS slice.  The next slice's matrix fragments and
 // amplitudes are loaded while the MFMAs of the current one run, so there is
 // one barrier per slice and the loads overlap the math.
-template <typename KT, int GE>
+template <typename KT, int GE, int GKS = 16>
 struct PipeShape {
   static constexpr int U = Frag<KT>::U, KB = 4 * U;
-  static constexpr int GK = 16, LPS = GK / KB;  // contraction rows per slice, fragment loads per tile per slice
+  static constexpr int GK = GKS, LPS = GK / KB;  // contraction rows per slice, fragment loads per tile per slice
   static constexpr int GC = 2 * GE / 16;        // MFMA column tiles
   // LDS row stride (doubles): the rows lane groups q and q + 1 read at one
   // MFMA step are U apart; U BW = 16 (mod 32) puts them 32 banks apart
@@ -703,25 +703,28 @@ struct PipeShape {
   static_assert(SPT * BS == GK * GE && LPS * KB == GK, "slice shape");
 };
 
-template <typename KT, int GE>
+template <typename KT, int GE, int GKS = 16>
 struct PipeShared {
-  double Bs[2][PipeShape<KT, GE>::GK][PipeShape<KT, GE>::BW];
+  double Bs[2][PipeShape<KT, GE, GKS>::GK][PipeShape<KT, GE, GKS>::BW];
   int t[GE], side[GE];
   double wy[GE], wy2[GE];
 };
 
-template <typename KT, int GE>
-__device__ void pipe_item(const BDev &d, int par, int r, int b, int c, PipeShared<KT, GE> &sh) {
+template <typename KT, int GE, int GKS = 16>
+__device__ void pipe_item(const BDev &d, int par, int r, int b, int c, PipeShared<KT, GE, GKS> &sh) {
   using F = Frag<KT>;
   using FV = typename F::V;
-  using P = PipeShape<KT, GE>;
+  using P = PipeShape<KT, GE, GKS>;
   constexpr int U = P::U, KB = P::KB, GK = P::GK, LPS = P::LPS, GC = P::GC, SPT = P::SPT;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int q = lane >> 4, col = lane & 15;
   const int nr = d.n_r;
   const int *cnt = d.cnt + par * nr;
   const int p0 = d.blk_start[b], nb = d.blk_start[b + 1] - p0;
-  const int nbp = (nb + KB - 1) / KB * KB, nsl = 2 * nbp / GK;
+  // [Y | Y2] rows of the tile, in fragment loads of KB rows; the last
+  // slice may run past them (GK = 32 with 8-row FP64 fragments): its
+  // fragments and amplitudes are zero there
+  const int nbp = (nb + KB - 1) / KB * KB, nfr = 2 * nbp / KB, nsl = (2 * nbp + GK - 1) / GK;
   const int e0 = c * GE, nE = min(GE, cnt[r] - e0), nct = (2 * nE + 15) / 16;
   const Entry *bk = d.bucket + ((long long)par * nr + r) * d.n_traj;
   __syncthreads();  // the previous item is done with the entry table and Bs
@@ -762,7 +765,7 @@ __device__ void pipe_item(const BDev &d, int par, int r, int b, int c, PipeShare
 #pragma unroll
     for (int h = 0; h < 2; h++)
 #pragma unroll
-      for (int l = 0; l < LPS; l++) an[h][l] = on[h] ? A[h][(s * LPS + l) * 64] : FV{};
+      for (int l = 0; l < LPS; l++) an[h][l] = (on[h] && s * LPS + l < nfr) ? A[h][(s * LPS + l) * 64] : FV{};
 #pragma unroll
     for (int j = 0; j < SPT; j++) {
       const int idx = tid + j * BS, kk = idx % GK, e = idx / GK, k = s * GK + kk;
@@ -866,11 +869,11 @@ __global__ __launch_bounds__(BS, GE >= 64 ? 1 : 2) void gemm_pipe_kernel(BDev d,
 // so a static share leaves a long tail; taken largest first, the small
 // items fill in at the end.  Each workgroup fetches its next item while it
 // works on the current one.
-template <typename KT, int GE>
+template <typename KT, int GE, int GKS = 16>
 __global__ __launch_bounds__(BS, GE >= 64 ? 1 : (GE >= 32 ? 2 : 3)) void gemm_pipe_dyn_kernel(BDev d, int par) {
   extern __shared__ int pre[];  // [n_r + 1] chunk prefix
   __shared__ int wtot[BS / 64];
-  __shared__ PipeShared<KT, GE> sh;
+  __shared__ PipeShared<KT, GE, GKS> sh;
   __shared__ int s_next;
   const int tid = threadIdx.x, nr = d.n_r;
   const int C = chunk_prefix(d, par, GE, pre, wtot);
@@ -883,7 +886,7 @@ __global__ __launch_bounds__(BS, GE >= 64 ? 1 : (GE >= 32 ? 2 : 3)) void gemm_pi
     if (tid == 0) nxt = atomicAdd(d.qctr, 1);
     const int b = d.blk_order[it / C], qc = it % C;
     const int r = chunk_knot(pre, nr, qc), c = qc - pre[r];
-    pipe_item<KT, GE>(d, par, r, b, c, sh);
+    pipe_item<KT, GE, GKS>(d, par, r, b, c, sh);
     __syncthreads();  // everyone has read s_next
     if (tid == 0) s_next = nxt;
     __syncthreads();
@@ -903,7 +906,8 @@ constexpr GemmShape kShapes[] = {
     {kLds, 64, 16, 2},      {kDirect, 32, 0, 1},  {kDirect, 16, 0, 1},  {kDirect, 8, 0, 1},
     {kStream, 8, 0, 1},     {kStream, 16, 0, 1},  {kStream, 32, 0, 1},  {kQueue, 16, 0, 1},
     {kQueue, 32, 0, 1},     {kLdsQueue, 64, 16, 1}, {kLdsQueue, 32, 16, 1}, {kPipe, 64, 16, 2},
-    {kPipe, 32, 16, 2},     {kPipeDyn, 64, 16, 2},  {kPipeDyn, 32, 16, 2},  {kPipeDyn, 16, 16, 2}};
+    {kPipe, 32, 16, 2},     {kPipeDyn, 64, 16, 2},  {kPipeDyn, 32, 16, 2},  {kPipeDyn, 16, 16, 2},
+    {kPipeDyn, 32, 32, 2},  {kPipeDyn, 16, 32, 2}};
 constexpr int kNShapes = sizeof(kShapes) / sizeof(kShapes[0]);
 // the work-queue forms keep the chunk prefix of all knots in LDS
 constexpr bool uses_queue(int shape) { return kShapes[shape].form >= kQueue; }
@@ -925,6 +929,8 @@ void launch_gemm(int shape, dim3 knots_blocks, int maxnb, hipStream_t st, const 
     case 17: gemm_pipe_dyn_kernel<KT, 64><<<DGRID, BS, shq, st>>>(d, par); return;
     case 18: gemm_pipe_dyn_kernel<KT, 32><<<DGRID, BS, shq, st>>>(d, par); return;
     case 19: gemm_pipe_dyn_kernel<KT, 16><<<DGRID, BS, shq, st>>>(d, par); return;
+    case 20: gemm_pipe_dyn_kernel<KT, 32, 32><<<DGRID, BS, shq, st>>>(d, par); return;
+    case 21: gemm_pipe_dyn_kernel<KT, 16, 32><<<DGRID, BS, shq, st>>>(d, par); return;
     default: break;
   }
   auto go = [&](auto kern, int grt) {

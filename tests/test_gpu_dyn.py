@@ -201,7 +201,7 @@ def test_at_batched_fp32_coefficients(dev):
     assert np.all(s32["norm_err"] < 1e-6)
 
 
-N_SHAPES = 20
+N_SHAPES = 22
 
 
 @pytest.mark.parametrize("shape", list(range(N_SHAPES)))
@@ -233,7 +233,7 @@ def test_at_batched_every_gemm_tile(dev, shape):
     assert np.array_equal(s0["n_steps"], s1["n_steps"])
 
 
-@pytest.mark.parametrize("shape", [0, 5, 8, 10, 11, 12, 13, 14, 15, 16, 17, 18, 19])
+@pytest.mark.parametrize("shape", [0, 5, 8, 10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20, 21])
 def test_at_batched_fp32_knots_every_layout(dev, shape):
     """FP32 knot matrices through the LDS-staged, the per-wave and the
     fragment-ordered GEMM forms: the same trajectories as the FP64
@@ -263,7 +263,8 @@ def test_at_batched_fp32_knots_every_layout(dev, shape):
 
 
 @pytest.mark.parametrize("shape,knots", [(0, 1), (12, 1), (15, 1), (16, 1), (15, 2), (16, 2), (17, 1), (18, 1),
-                                         (17, 2), (18, 2), (19, 1), (19, 2)])
+                                         (17, 2), (18, 2), (19, 1), (19, 2), (20, 1), (20, 2), (21, 1),
+                                         (21, 2)])
 def test_at_batched_large_blocks(dev, shape, knots):
     """m-blocks of up to 81 states (asymmetric top j <= 8, 969 states): the
     forms whose waves own several 16-row tiles of a block, and the FP32
