@@ -295,6 +295,39 @@ def test_at_batched_large_blocks(dev, shape, knots):
     assert np.array_equal(s0["n_steps"], s1["n_steps"])
 
 
+@pytest.mark.parametrize("shape,knots", [(16, 2), (18, 1), (18, 2), (19, 2), (21, 2)])
+def test_at_batched_full_blocks(dev, shape, knots):
+    """The benchmark's basis (asymmetric top j <= 10 + atom, 1771 states,
+    m-blocks of up to 121 states: all eight 16-row tiles of the pipelined
+    forms in use), 16 trajectories: the pipelined forms the tuner picks for
+    this basis against the plain kernel's RK4 trajectories (1e-9 with FP64
+    coefficients, 1e-6 with FP32)."""
+    from mqct_amd.matrix import MatrixBuilder
+    from mqct_amd.propagate import at_first_step
+    wl = W.asym_top_atom(n_r=120)
+    assert wl.basis.n_states == 1771
+    bb, E = W.at_batch(16, seed=29)
+    paths, l, _ = at_first_step(dev, W.asym_top_atom(jmax=1, n_beta=12, n_gamma=24, n_r=120), bb, E,
+                                time_step=20.0, rec_cap=20000)
+    MatrixBuilder(dev, wl).build("fp64", copy_out=False)
+    prop = Propagator(dev, wl.basis, wl.R, mass_amu=wl.mass_amu, from_build=True)
+    prop.load_paths(paths)
+    rows = []
+    for i in range(len(bb)):
+        rows += prop.trajectories([l[i]], float(E[i]), 1, time_step=10.0, at_paths=[i])
+    q0, s0 = prop.run(L.MQCT_PROP_AT_RK4, rows, batched=0)
+    L.check(dev.lib.mqct_hip_batched_set_tile(dev.h, shape), "set_tile")
+    try:
+        q1, s1 = prop.run(L.MQCT_PROP_AT_RK4, rows, batched=knots)
+    finally:
+        L.check(dev.lib.mqct_hip_batched_set_tile(dev.h, -1), "set_tile")
+    N = wl.basis.n_states
+    d = np.max(np.abs(q1[:, :2 * N] - q0[:, :2 * N]))
+    tol = 1e-9 if knots == 1 else 1e-6
+    assert d <= tol, f"shape {shape} knots {knots}: batched vs plain amplitude diff {d}"
+    assert np.array_equal(s0["n_steps"], s1["n_steps"])
+
+
 @pytest.mark.parametrize("mode", [L.MQCT_PROP_AT_RK4, L.MQCT_PROP_AT_ODEINT])
 def test_at_batched_step_threads(dev, mode):
     """The step kernel with 512 threads per trajectory (two states per
