@@ -76,6 +76,10 @@ def main():
                     help="storage precision of the spline-coefficient matrices of the timed form")
     ap.add_argument("--step-threads", type=int, default=0, choices=(0, 256, 512),
                     help="threads per trajectory of the batched step kernel (0: the library's choice)")
+    ap.add_argument("--tile-levels", default="",
+                    help="comma-separated interval-GEMM shape per level floor(log2(running trajectories)), "
+                         "from level 0 up (-1: chosen by timing), for the timed form's knot precision: "
+                         "repeats a measured run's choices in a profiled one")
     ap.add_argument("--plain-sample", type=int, default=512)
     ap.add_argument("--xsec-sample", type=int, default=512)
     ap.add_argument("--no-graph", action="store_true",
@@ -130,6 +134,11 @@ def main():
     prop = Propagator(dev, wl.basis, wl.R, mass_amu=wl.mass_amu, from_build=True)
     prop.load_paths(paths)
     L.check(dev.lib.mqct_hip_batched_set_step_threads(dev.h, args.step_threads), "set_step_threads")
+    if args.tile_levels:
+        lv_set = [int(x) for x in args.tile_levels.split(",")]
+        arr = (ctypes.c_int * len(lv_set))(*lv_set)
+        L.check(dev.lib.mqct_hip_batched_set_tile_levels(dev.h, 1 if args.knots == "fp32" else 0, len(lv_set), arr),
+                "set_tile_levels")
     rows = []
     for i in range(n):
         rows += prop.trajectories([l[i]], float(E[i]), 1, time_step=10.0, eps=1e-8, hmin=args.hmin,

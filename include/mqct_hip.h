@@ -226,6 +226,12 @@ int mqct_hip_batched_tile(mqct_hip_t *h, int shape, int *form, int *entries,
 /* The chosen shapes (indices into the candidate list, -1 none yet) of
  * levels 0..n-1 for the FP64 (fp32=0) or FP32 (fp32=1) knot matrices. */
 int mqct_hip_batched_tile_levels(mqct_hip_t *h, int fp32, int n, int *shapes);
+/* Sets the shapes of levels 0..n-1 (n <= 24; -1: chosen by timing) for the
+ * FP64 or FP32 knot matrices, e.g. to repeat a measured run's choices in a
+ * profiled one.  A batched run refuses a set shape that does not fit its
+ * matrix (pipelined forms: m-blocks of at most 128 states; work-queue forms:
+ * at most 16383 knots). */
+int mqct_hip_batched_set_tile_levels(mqct_hip_t *h, int fp32, int n, const int *shapes);
 /* Fixes the interval-GEMM tile shape (index into the candidate list,
  * 0..21) for the following batched runs, or -1 to choose by timing again. */
 int mqct_hip_batched_set_tile(mqct_hip_t *h, int shape);

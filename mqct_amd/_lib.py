@@ -91,6 +91,7 @@ SIGNATURES = {
     "mqct_hip_batched_profile": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "mqct_hip_batched_tile": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, c_int_p, c_int_p, c_int_p, c_int_p]),
     "mqct_hip_batched_set_tile": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "mqct_hip_batched_set_tile_levels": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, c_int_p]),
     "mqct_hip_batched_set_step_threads": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "mqct_hip_set_contract_variant": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
     "mqct_hip_contract_variant": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, c_int_p, c_int_p, c_int_p, c_int_p]),

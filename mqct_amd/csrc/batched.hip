@@ -1610,6 +1610,12 @@ int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *t
   const bool queue_ok = (size_t)(n_r + 1) * 4 <= 64 * 1024;
   if (ds->gemm_fixed >= 0 && uses_queue(ds->gemm_fixed) && !queue_ok)
     return mqct_fail(MQCT_ERR_ARG, "batched: the work-queue GEMM shapes take at most 16383 knots");
+  // shapes set per level (the tuner's own choices always fit)
+  for (int l = 0; l < DynState::kLevels; l++) {
+    const int sh = ds->gemm_shape[pi_][l];
+    if (sh >= 0 && ((uses_queue(sh) && !queue_ok) || (kShapes[sh].form >= kPipe && maxnb > 128)))
+      return mqct_fail(MQCT_ERR_ARG, "batched: a GEMM shape set for a level does not fit this matrix");
+  }
   auto shape_for = [&](int &sh) -> int {
     for (;;) {
       if (ds->gemm_fixed >= 0) {
@@ -1788,6 +1794,15 @@ extern "C" int mqct_hip_batched_tile(mqct_hip_t *h, int shape, int *form, int *e
 extern "C" int mqct_hip_batched_tile_levels(mqct_hip_t *h, int fp32, int n, int *shapes) {
   if (!h || !h->dyn || !shapes || n < 0) return mqct_fail(MQCT_ERR_ARG, "batched_tile_levels: bad argument");
   for (int l = 0; l < n; l++) shapes[l] = l < DynState::kLevels ? h->dyn->gemm_shape[fp32 ? 1 : 0][l] : -1;
+  return MQCT_OK;
+}
+
+extern "C" int mqct_hip_batched_set_tile_levels(mqct_hip_t *h, int fp32, int n, const int *shapes) {
+  if (!h || !h->dyn) return mqct_fail(MQCT_ERR_STATE, "batched_set_tile_levels: no matrix loaded");
+  if (!shapes || n < 0 || n > DynState::kLevels) return mqct_fail(MQCT_ERR_ARG, "batched_set_tile_levels: bad argument");
+  for (int l = 0; l < n; l++)
+    if (shapes[l] < -1 || shapes[l] >= kNShapes) return mqct_fail(MQCT_ERR_ARG, "batched_set_tile_levels: unknown shape");
+  for (int l = 0; l < n; l++) h->dyn->gemm_shape[fp32 ? 1 : 0][l] = shapes[l];
   return MQCT_OK;
 }
 

@@ -357,3 +357,37 @@ def test_at_batched_step_threads(dev, mode):
     assert d <= tol, f"512-thread step kernel vs plain amplitude diff {d}"
     if mode == L.MQCT_PROP_AT_RK4:
         assert np.array_equal(s0["n_steps"], s1["n_steps"])
+
+
+def test_at_batched_tile_levels(dev):
+    """Shapes set per level are used as set (read back after the run, the
+    tuner leaving them alone) and give the plain kernel's RK4 trajectories;
+    an unknown shape is refused."""
+    import ctypes
+    from mqct_amd.matrix import MatrixBuilder
+    from mqct_amd.propagate import at_first_step
+    wl = W.asym_top_atom(jmax=3, n_beta=12, n_gamma=24, n_r=120)
+    bb, E = W.at_batch(24, seed=31)
+    paths, l, _ = at_first_step(dev, W.asym_top_atom(jmax=1, n_beta=12, n_gamma=24, n_r=120), bb, E,
+                                time_step=20.0, rec_cap=20000)
+    MatrixBuilder(dev, wl).build("fp64", copy_out=False)
+    prop = Propagator(dev, wl.basis, wl.R, mass_amu=wl.mass_amu, from_build=True)
+    prop.load_paths(paths)
+    rows = []
+    for i in range(len(bb)):
+        rows += prop.trajectories([l[i]], float(E[i]), 1, time_step=10.0, at_paths=[i])
+    q0, s0 = prop.run(L.MQCT_PROP_AT_RK4, rows, batched=0)
+    want = [8, 11, 19, 18, 21]  # levels 0..4 (up to 31 running trajectories)
+    arr = (ctypes.c_int * len(want))(*want)
+    L.check(dev.lib.mqct_hip_batched_set_tile_levels(dev.h, 0, len(want), arr), "set_tile_levels")
+    bad = (ctypes.c_int * 1)(99)
+    assert dev.lib.mqct_hip_batched_set_tile_levels(dev.h, 0, 1, bad) != 0
+    q1, s1 = prop.run(L.MQCT_PROP_AT_RK4, rows, batched=1)
+    got = (ctypes.c_int * len(want))()
+    L.check(dev.lib.mqct_hip_batched_tile_levels(dev.h, 0, len(want), got), "tile_levels")
+    L.check(dev.lib.mqct_hip_batched_set_tile(dev.h, -1), "set_tile")  # back to timing every level
+    assert list(got) == want
+    N = wl.basis.n_states
+    d = np.max(np.abs(q1[:, :2 * N] - q0[:, :2 * N]))
+    assert d <= 1e-9, f"batched with set levels vs plain amplitude diff {d}"
+    assert np.array_equal(s0["n_steps"], s1["n_steps"])
