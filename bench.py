@@ -4,15 +4,17 @@ Headline (`value`): AT-MQCT trajectories per second for the batched
 propagation of the asymmetric top + atom system (1771 states, 86 779
 coupling pairs, 300 spline knots; 4096 trajectories with b uniform in
 [0, 18] bohr and E uniform in [200, 2000] cm^-1; amplitudes complex FP64,
-Cash-Karp RK45 with tolerance 1e-8 and MIN_TMSTP 20 a.u.; spline
-coefficients stored in FP32 by default).  One step = one propagation of the
+Cash-Karp RK45 with tolerance 1e-8 and MIN_TMSTP 20 a.u.; by default the
+spline coefficients are stored in FP32 and the pipelined interval-GEMM forms
+form their products in FP32 on f32 MFMA).  One step = one propagation of the
 whole batch, inputs (matrix splines and recorded classical paths) resident
 in HBM.
 
 Reported beside it:
   * the same batch with FP64 coefficients, and the cross-section error of
-    the FP32-coefficient form against it (FP32 is the storage the workload
-    names for the spline coefficients: "complex FP32 1771x1771");
+    the timed form against it (FP32 is the storage the workload names for
+    the spline coefficients: "complex FP32 1771x1771"; the metric admits
+    1e-5 relative cross-section error);
   * the plain FP64 per-trajectory kernel on a sample of the batch (the
     comparison the metric names);
   * the roofline of the dominant kernel (the interval GEMM) from HIP events
@@ -39,6 +41,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "traj/s + matrix-build time vs plain FP64 GPU at ≤1e-5 rel. cross-section error"
 PEAK_FP64_TFLOPS = 78.6
+PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
 
 
@@ -72,8 +75,9 @@ def main():
     ap.add_argument("--n-traj", type=int, default=4096)
     ap.add_argument("--hmin", type=float, default=20.0,
                     help="MIN_TMSTP (a.u.): the step floor of odeint's error control")
-    ap.add_argument("--knots", choices=("fp32", "fp64"), default="fp32",
-                    help="storage precision of the spline-coefficient matrices of the timed form")
+    ap.add_argument("--knots", choices=("fp32", "fp64", "fp32p"), default="fp32p",
+                    help="storage precision of the spline-coefficient matrices of the timed form "
+                         "(fp32p: FP32 storage and FP32 MFMA products in the pipelined GEMM forms)")
     ap.add_argument("--step-threads", type=int, default=0, choices=(0, 256, 512),
                     help="threads per trajectory of the batched step kernel (0: the library's choice)")
     ap.add_argument("--tile-levels", default="",
@@ -116,7 +120,7 @@ def main():
     dev = Device(0)
     n = args.n_traj
     bb, E = W.at_batch(n)
-    form = 2 if args.knots == "fp32" else 1
+    form = {"fp64": 1, "fp32": 2, "fp32p": 3}[args.knots]
 
     # -- setup: first AT step (classical paths on the j <= 2 basis) ----------
     t0 = time.time()
@@ -137,7 +141,7 @@ def main():
     if args.tile_levels:
         lv_set = [int(x) for x in args.tile_levels.split(",")]
         arr = (ctypes.c_int * len(lv_set))(*lv_set)
-        L.check(dev.lib.mqct_hip_batched_set_tile_levels(dev.h, 1 if args.knots == "fp32" else 0, len(lv_set), arr),
+        L.check(dev.lib.mqct_hip_batched_set_tile_levels(dev.h, form - 1, len(lv_set), arr),
                 "set_tile_levels")
     rows = []
     for i in range(n):
@@ -203,7 +207,7 @@ def main():
                                               ctypes.byref(grt)), "tile")
         return "%s e%d k%d r%d" % (FORMS[fo.value], ge.value, gk.value, grt.value)
     lv = (ctypes.c_int * 24)()
-    L.check(dev.lib.mqct_hip_batched_tile_levels(dev.h, 1 if form == 2 else 0, 24, lv), "tile_levels")
+    L.check(dev.lib.mqct_hip_batched_tile_levels(dev.h, form - 1, 24, lv), "tile_levels")
     tile = {"first": shape_desc(-1),
             "per_level": {"2^%d" % i: shape_desc(v) for i, v in enumerate(lv) if v >= 0}}
     roof = None
@@ -211,42 +215,51 @@ def main():
         sec = ms.value * 1e-3 / la.value
         tflops = fl.value / la.value / sec / 1e12
         gbs = by.value / la.value / sec / 1e9
-        f_mfma, f_hbm = tflops / PEAK_FP64_TFLOPS, gbs / PEAK_HBM_GBS
+        # the pipelined GEMM forms (the full and middle levels) run f32 MFMA
+        # with FP32 products, f64 MFMA otherwise
+        peak_mfma = PEAK_FP32_TFLOPS if form == 3 else PEAK_FP64_TFLOPS
+        f_mfma, f_hbm = tflops / peak_mfma, gbs / PEAK_HBM_GBS
         if f_hbm >= f_mfma:
             roof = {"bound": "hbm", "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                     "frac": round(f_hbm, 4)}
         else:
-            roof = {"bound": "mfma", "achieved": round(tflops, 3), "peak": PEAK_FP64_TFLOPS,
+            roof = {"bound": "mfma", "achieved": round(tflops, 3), "peak": peak_mfma,
                     "unit": "TFLOP/s", "frac": round(f_mfma, 4)}
         roof.update({"traffic": traffic_from_profiles(args.knots),
-                     "kernel": "interval GEMM over %s knot matrices, FP64 MFMA 16x16x4 (gemm_*_kernel: "
-                               "form and tile shape chosen per running-trajectory level)"
-                               % ("FP32" if form == 2 else "FP64"),
+                     "kernel": "interval GEMM over %s knot matrices, %s (gemm_*_kernel: form and tile shape "
+                               "chosen per running-trajectory level)"
+                               % ("FP64" if form == 1 else "FP32",
+                                  "f32 MFMA 16x16x4 with FP32 products in the pipelined forms, f64 MFMA 16x16x4 "
+                                  "in the others" if form == 3 else "f64 MFMA 16x16x4"),
+                     "mfma_peak_tflops": peak_mfma,
                      "tile": tile, "launches": la.value, "avg_launch_us": round(sec * 1e6, 2),
                      "flops_per_launch": fl.value / la.value, "bytes_per_launch": by.value / la.value,
-                     "fp64_mfma_tflops": round(tflops, 3), "mfma_frac": round(f_mfma, 4),
+                     "mfma_tflops": round(tflops, 3), "mfma_frac": round(f_mfma, 4),
                      "hbm_gbs": round(gbs, 1), "hbm_frac": round(f_hbm, 4),
                      "gemm_share_of_step": round(ms.value / (pel * 1e3), 3), "levels": levels})
-        log("gemm: %d launches, avg %.1f us, %.2f TFLOP/s (%.1f%% of FP64 peak), %.0f GB/s (%.1f%% of HBM), "
-            "%.0f%% of a %.2f s event-timed step" % (la.value, sec * 1e6, tflops, 100 * f_mfma, gbs,
-                                                    100 * f_hbm, 100 * ms.value / (pel * 1e3), pel))
+        log("gemm: %d launches, avg %.1f us, %.2f TFLOP/s (%.1f%% of the %.1f TFLOP/s MFMA peak), %.0f GB/s "
+            "(%.1f%% of HBM), %.0f%% of a %.2f s event-timed step"
+            % (la.value, sec * 1e6, tflops, 100 * f_mfma, peak_mfma, gbs, 100 * f_hbm,
+               100 * ms.value / (pel * 1e3), pel))
 
     # -- the other coefficient precision and the cross-section difference -----
     other = None
     if not args.no_other:
-        oform = 1 if form == 2 else 2
+        oform = 2 if form == 1 else 1
         t0 = time.perf_counter()
         if args.no_graph:
             L.check(dev.lib.mqct_hip_batched_profile(dev.h, 1), "batched_profile")
         qo, _ = prop.run(L.MQCT_PROP_AT_ODEINT, rows, batched=oform)
         oel = time.perf_counter() - t0
-        q32, q64 = (q, qo) if form == 2 else (qo, q)
-        e, nch = rel_err(xsec_functional(prop, q32, l, E), xsec_functional(prop, q64, l, E))
-        other = {"knots": "fp64" if oform == 1 else "fp32", "traj_per_s": round(n / oel, 3),
-                 "xsec_rel_err_fp32_vs_fp64_coefficients": e, "channels": nch,
-                 "max_amp_diff": float(np.max(np.abs(q32[:, :2 * prop.N] - q64[:, :2 * prop.N])))}
-        log("batched AT (%s coefficients): %.2f s -> %.1f traj/s; xsec fp32 vs fp64 coefficients %.2e over %d "
-            "channels" % (other["knots"], oel, n / oel, e, nch))
+        # the reduced-precision form against the FP64 coefficients
+        qx, q64 = (qo, q) if form == 1 else (q, qo)
+        xname = "fp32" if form == 1 else args.knots
+        e, nch = rel_err(xsec_functional(prop, qx, l, E), xsec_functional(prop, q64, l, E))
+        other = {"knots": "fp32" if oform == 2 else "fp64", "traj_per_s": round(n / oel, 3),
+                 "xsec_rel_err_vs_fp64_coefficients": e, "compared_form": xname, "channels": nch,
+                 "max_amp_diff": float(np.max(np.abs(qx[:, :2 * prop.N] - q64[:, :2 * prop.N])))}
+        log("batched AT (%s coefficients): %.2f s -> %.1f traj/s; xsec %s vs fp64 coefficients %.2e over %d "
+            "channels" % (other["knots"], oel, n / oel, xname, e, nch))
     else:
         q64 = q
 
@@ -309,11 +322,14 @@ def main():
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "traj/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(el * 1e3 / args.steps, 2), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64/f32" if form == 3 else "f64", "data": "synthetic",
         "config": {"workload": "AT-MQCT batched propagation: asymmetric top + atom, 1771 states, "
                                "86779 pairs, 300 knots, 4096 trajectories (b 0-18 bohr, E 200-2000 cm^-1), "
-                               "RK45 tol 1e-8, MIN_TMSTP %g a.u., FP64 amplitudes, %s spline coefficients"
-                               % (args.hmin, args.knots.upper()),
+                               "RK45 tol 1e-8, MIN_TMSTP %g a.u., FP64 amplitudes, %s"
+                               % (args.hmin, {"fp64": "FP64 spline coefficients",
+                                              "fp32": "FP32 spline coefficients, FP64 products",
+                                              "fp32p": "FP32 spline coefficients, FP32 products in the "
+                                                       "pipelined GEMM forms"}[args.knots]),
                    "n_traj": n, "states": int(wl.basis.n_states), "pairs": int(wl.basis.total_size),
                    "knots": int(len(wl.R)), "form": "batched", "graph": not args.no_graph},
         "roofline": roof,

@@ -185,7 +185,11 @@ int mqct_hip_load_paths(mqct_hip_t *h, int n_paths, const int64_t *path_off,
  * batched=0: the one-trajectory-per-workgroup plain FP64 form;
  * batched=1: the interval-bucketed batched form (AT modes) with FP64
  * spline coefficients; batched=2: the same with the coefficients stored
- * in FP32 (amplitudes, products and time stepping stay FP64). */
+ * in FP32 (amplitudes, products and time stepping stay FP64); batched=3:
+ * FP32 coefficients and FP32 products -- the pipelined interval-GEMM forms
+ * multiply them by the weighted amplitudes rounded to FP32 on f32 MFMA
+ * with FP32 accumulation (the other GEMM forms keep FP64 products;
+ * amplitudes and time stepping stay FP64). */
 int mqct_hip_propagate(mqct_hip_t *h, int mode, int n_traj,
                        const mqct_traj_t *traj, int batched, double *q_out,
                        mqct_traj_stat_t *stat_out);
@@ -224,7 +228,8 @@ int mqct_hip_batched_prof_trace(mqct_hip_t *h, int cap, double *rows, int *n);
 int mqct_hip_batched_tile(mqct_hip_t *h, int shape, int *form, int *entries,
                           int *k_rows, int *row_tiles);
 /* The chosen shapes (indices into the candidate list, -1 none yet) of
- * levels 0..n-1 for the FP64 (fp32=0) or FP32 (fp32=1) knot matrices. */
+ * levels 0..n-1 for the FP64 (fp32=0) or FP32 (fp32=1) knot matrices, or
+ * the FP32 knot matrices with FP32 products (fp32=2). */
 int mqct_hip_batched_tile_levels(mqct_hip_t *h, int fp32, int n, int *shapes);
 /* Sets the shapes of levels 0..n-1 (n <= 24; -1: chosen by timing) for the
  * FP64 or FP32 knot matrices, e.g. to repeat a measured run's choices in a

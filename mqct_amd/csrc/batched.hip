@@ -31,11 +31,13 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 namespace {
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
+typedef float flt4 __attribute__((ext_vector_type(4)));
 constexpr int BS = 256;
 constexpr int NSLOT = 6;       // y, k1..k5 (k6 and rk4's k4 stay in registers)
 constexpr int MAXSTP = 10000;  // odeint (suppl_routines.f:1362)
@@ -440,6 +442,7 @@ struct Frag<float> {
   __device__ static double get(const V &v, int u) {
     return (double)(u == 0 ? v.x : (u == 1 ? v.y : (u == 2 ? v.z : v.w)));
   }
+  __device__ static float getf(const V &v, int u) { return u == 0 ? v.x : (u == 1 ? v.y : (u == 2 ? v.z : v.w)); }
 };
 
 // The same product over the fragment-ordered tiles: each wave owns one
@@ -699,22 +702,31 @@ struct PipeShape {
   // LDS row stride (doubles): the rows lane groups q and q + 1 read at one
   // MFMA step are U apart; U BW = 16 (mod 32) puts them 32 banks apart
   static constexpr int BW = 2 * GE + (U == 4 ? 4 : 8);
+  // the same with FP32 staging (FP32 products): lane groups q read rows
+  // 4 apart, 4 BW32 = 16 (mod 64) puts them 16 banks apart
+  static constexpr int BW32 = 2 * GE + 4;
   static constexpr int SPT = GK * GE / BS;      // (row, entry) pairs staged per thread per slice
   static_assert(SPT * BS == GK * GE && LPS * KB == GK, "slice shape");
 };
 
-template <typename KT, int GE, int GKS = 16>
+template <typename KT, int GE, int GKS = 16, bool M32 = false>
 struct PipeShared {
-  double Bs[2][PipeShape<KT, GE, GKS>::GK][PipeShape<KT, GE, GKS>::BW];
+  using BT = typename std::conditional<M32, float, double>::type;
+  BT Bs[2][PipeShape<KT, GE, GKS>::GK][M32 ? PipeShape<KT, GE, GKS>::BW32 : PipeShape<KT, GE, GKS>::BW];
   int t[GE], side[GE];
   double wy[GE], wy2[GE];
 };
 
-template <typename KT, int GE, int GKS = 16>
-__device__ void pipe_item(const BDev &d, int par, int r, int b, int c, PipeShared<KT, GE, GKS> &sh) {
+template <typename KT, int GE, int GKS = 16, bool M32 = false>
+__device__ void pipe_item(const BDev &d, int par, int r, int b, int c, PipeShared<KT, GE, GKS, M32> &sh) {
   using F = Frag<KT>;
   using FV = typename F::V;
   using P = PipeShape<KT, GE, GKS>;
+  // M32: FP32 products -- the FP32 knot fragments times the weighted
+  // amplitudes rounded to FP32 on f32 16x16x4 MFMA with FP32 accumulation
+  static_assert(!M32 || std::is_same<KT, float>::value, "FP32 products need FP32 knot matrices");
+  using BT = typename PipeShared<KT, GE, GKS, M32>::BT;
+  using AccT = typename std::conditional<M32, flt4, dbl4>::type;
   constexpr int U = P::U, KB = P::KB, GK = P::GK, LPS = P::LPS, GC = P::GC, SPT = P::SPT;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int q = lane >> 4, col = lane & 15;
@@ -780,15 +792,15 @@ __device__ void pipe_item(const BDev &d, int par, int r, int b, int c, PipeShare
 #pragma unroll
     for (int j = 0; j < SPT; j++) {
       const int idx = tid + j * BS, kk = idx % GK, e = idx / GK;
-      sh.Bs[buf][kk][2 * e] = bw[j] * bn[j].x;
-      sh.Bs[buf][kk][2 * e + 1] = bw[j] * bn[j].y;
+      sh.Bs[buf][kk][2 * e] = (BT)(bw[j] * bn[j].x);
+      sh.Bs[buf][kk][2 * e + 1] = (BT)(bw[j] * bn[j].y);
     }
   };
-  dbl4 acc[2][GC];
+  AccT acc[2][GC];
 #pragma unroll
   for (int h = 0; h < 2; h++)
 #pragma unroll
-    for (int ct = 0; ct < GC; ct++) acc[h][ct] = dbl4{0.0, 0.0, 0.0, 0.0};
+    for (int ct = 0; ct < GC; ct++) acc[h][ct] = AccT{0, 0, 0, 0};
   load(0);
   stage(0);
   __syncthreads();
@@ -806,23 +818,31 @@ __device__ void pipe_item(const BDev &d, int par, int r, int b, int c, PipeShare
 #pragma unroll
       for (int u = 0; u < U; u++) {
         // fragment order: lane group q holds row KB l + U q + u at step u
-        const double *brow = &sh.Bs[buf][KB * l + U * q + u][col];
-        double bf[GC];
+        const BT *brow = &sh.Bs[buf][KB * l + U * q + u][col];
+        BT bf[GC];
 #pragma unroll
-        for (int ct = 0; ct < GC; ct++) bf[ct] = (ct < nct) ? brow[16 * ct] : 0.0;
+        for (int ct = 0; ct < GC; ct++) bf[ct] = (ct < nct) ? brow[16 * ct] : (BT)0;
 #pragma unroll
         for (int h = 0; h < 2; h++) {
           if (!on[h]) continue;
-          const double a = F::get(ac[h][l], u);
+          if constexpr (M32) {
+            const float a = F::getf(ac[h][l], u);
 #pragma unroll
-          for (int ct = 0; ct < GC; ct++)
-            if (ct < nct) acc[h][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bf[ct], acc[h][ct], 0, 0, 0);
+            for (int ct = 0; ct < GC; ct++)
+              if (ct < nct) acc[h][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bf[ct], acc[h][ct], 0, 0, 0);
+          } else {
+            const double a = F::get(ac[h][l], u);
+#pragma unroll
+            for (int ct = 0; ct < GC; ct++)
+              if (ct < nct) acc[h][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bf[ct], acc[h][ct], 0, 0, 0);
+          }
         }
       }
     if (more) stage(buf ^ 1);
     __syncthreads();
   }
-  // f64 16x16x4 output: row lane/16 + 4i, column lane%16 of each tile
+  // 16x16x4 output, column lane%16 of each tile: row lane/16 + 4i (f64),
+  // 4 (lane/16) + i (f32)
 #pragma unroll
   for (int h = 0; h < 2; h++) {
     if (!on[h]) continue;
@@ -835,8 +855,8 @@ __device__ void pipe_item(const BDev &d, int par, int r, int b, int c, PipeShare
       double *wb = reinterpret_cast<double *>(d.W + ((long long)sh.t[e] * 2 + sh.side[e]) * d.N + p0) + (cc & 1);
 #pragma unroll
       for (int i = 0; i < 4; i++) {
-        const int orow = rt0 + q + 4 * i;
-        if (orow < nb) wb[2 * orow] = acc[h][ct][i];
+        const int orow = rt0 + (M32 ? 4 * q + i : q + 4 * i);
+        if (orow < nb) wb[2 * orow] = (double)acc[h][ct][i];
       }
     }
   }
@@ -845,11 +865,11 @@ __device__ void pipe_item(const BDev &d, int par, int r, int b, int c, PipeShare
 // The pipelined product over a static share of the items: (knot, block,
 // chunk) in that order, so the chunks of one block at one knot follow each
 // other and meet its matrix in L2, with a contiguous item range per XCD.
-template <typename KT, int GE>
+template <typename KT, int GE, bool M32 = false>
 __global__ __launch_bounds__(BS, GE >= 64 ? 1 : 2) void gemm_pipe_kernel(BDev d, int par) {
   extern __shared__ int pre[];  // [n_r + 1] chunk prefix
   __shared__ int wtot[BS / 64];
-  __shared__ PipeShared<KT, GE> sh;
+  __shared__ PipeShared<KT, GE, 16, M32> sh;
   const int nr = d.n_r, nblk = d.nblk;
   const long long M = (long long)chunk_prefix(d, par, GE, pre, wtot) * nblk;
   const int X = 8, xcd = blockIdx.x % X, slot = blockIdx.x / X, Gx = gridDim.x / X;
@@ -858,7 +878,7 @@ __global__ __launch_bounds__(BS, GE >= 64 ? 1 : 2) void gemm_pipe_kernel(BDev d,
     // item -> (knot r, block b, chunk c)
     const int r = chunk_knot(pre, nr, (int)(it / nblk)), cr = pre[r + 1] - pre[r];
     const int loc = (int)(it - (long long)pre[r] * nblk), b = loc / cr, c = loc % cr;
-    pipe_item<KT, GE>(d, par, r, b, c, sh);
+    pipe_item<KT, GE, 16, M32>(d, par, r, b, c, sh);
   }
 }
 
@@ -869,11 +889,11 @@ __global__ __launch_bounds__(BS, GE >= 64 ? 1 : 2) void gemm_pipe_kernel(BDev d,
 // so a static share leaves a long tail; taken largest first, the small
 // items fill in at the end.  Each workgroup fetches its next item while it
 // works on the current one.
-template <typename KT, int GE, int GKS = 16>
+template <typename KT, int GE, int GKS = 16, bool M32 = false>
 __global__ __launch_bounds__(BS, GE >= 64 ? 1 : (GE >= 32 ? 2 : 3)) void gemm_pipe_dyn_kernel(BDev d, int par) {
   extern __shared__ int pre[];  // [n_r + 1] chunk prefix
   __shared__ int wtot[BS / 64];
-  __shared__ PipeShared<KT, GE, GKS> sh;
+  __shared__ PipeShared<KT, GE, GKS, M32> sh;
   __shared__ int s_next;
   const int tid = threadIdx.x, nr = d.n_r;
   const int C = chunk_prefix(d, par, GE, pre, wtot);
@@ -886,7 +906,7 @@ __global__ __launch_bounds__(BS, GE >= 64 ? 1 : (GE >= 32 ? 2 : 3)) void gemm_pi
     if (tid == 0) nxt = atomicAdd(d.qctr, 1);
     const int b = d.blk_order[it / C], qc = it % C;
     const int r = chunk_knot(pre, nr, qc), c = qc - pre[r];
-    pipe_item<KT, GE, GKS>(d, par, r, b, c, sh);
+    pipe_item<KT, GE, GKS, M32>(d, par, r, b, c, sh);
     __syncthreads();  // everyone has read s_next
     if (tid == 0) s_next = nxt;
     __syncthreads();
@@ -913,9 +933,24 @@ constexpr int kNShapes = sizeof(kShapes) / sizeof(kShapes[0]);
 constexpr bool uses_queue(int shape) { return kShapes[shape].form >= kQueue; }
 
 template <typename KT>
-void launch_gemm(int shape, dim3 knots_blocks, int maxnb, hipStream_t st, const BDev &d, int par) {
+void launch_gemm(int shape, dim3 knots_blocks, int maxnb, hipStream_t st, const BDev &d, int par,
+                 bool m32 = false) {
   const size_t shq = (size_t)(d.n_r + 1) * 4;
   const dim3 gs(knots_blocks.x, (d.ntile + 3) / 4);
+  // FP32 products: the pipelined forms on f32 MFMA (the other forms keep
+  // FP64 products)
+  if constexpr (std::is_same<KT, float>::value) {
+    if (m32) switch (shape) {
+        case 15: gemm_pipe_kernel<KT, 64, true><<<QGRID, BS, shq, st>>>(d, par); return;
+        case 16: gemm_pipe_kernel<KT, 32, true><<<QGRID, BS, shq, st>>>(d, par); return;
+        case 17: gemm_pipe_dyn_kernel<KT, 64, 16, true><<<DGRID, BS, shq, st>>>(d, par); return;
+        case 18: gemm_pipe_dyn_kernel<KT, 32, 16, true><<<DGRID, BS, shq, st>>>(d, par); return;
+        case 19: gemm_pipe_dyn_kernel<KT, 16, 16, true><<<DGRID, BS, shq, st>>>(d, par); return;
+        case 20: gemm_pipe_dyn_kernel<KT, 32, 32, true><<<DGRID, BS, shq, st>>>(d, par); return;
+        case 21: gemm_pipe_dyn_kernel<KT, 16, 32, true><<<DGRID, BS, shq, st>>>(d, par); return;
+        default: break;
+      }
+  }
   switch (shape) {
     case 8: gemm_stream_kernel<KT, 1><<<gs, BS, 0, st>>>(d, par); return;
     case 9: gemm_stream_kernel<KT, 2><<<gs, BS, 0, st>>>(d, par); return;
@@ -1333,7 +1368,7 @@ __global__ __launch_bounds__(BS) void out_kernel(BDev d, const int *__restrict__
 }  // namespace
 
 int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *traj,
-                         double *q_out, mqct_traj_stat_t *stat_out, bool knots_fp32) {
+                         double *q_out, mqct_traj_stat_t *stat_out, bool knots_fp32, bool prod_fp32) {
   DynState *ds = h->dyn;
   const int N = ds->N, n_r = ds->n_r;
   const long long T = ds->T;
@@ -1548,9 +1583,10 @@ int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *t
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(h->stream));
   const dim3 kb(n_r, nblk);
-  const int pi_ = knots_fp32 ? 1 : 0;
+  const int pi_ = prod_fp32 ? 2 : (knots_fp32 ? 1 : 0);
+  ds->last_form = pi_;
   auto gemm = [&](int sh, int par) {
-    if (knots_fp32) launch_gemm<float>(sh, kb, maxnb, h->stream, d, par);
+    if (knots_fp32) launch_gemm<float>(sh, kb, maxnb, h->stream, d, par, prod_fp32);
     else launch_gemm<double>(sh, kb, maxnb, h->stream, d, par);
   };
   const int ITER = 16;  // even: the bucket parity pattern repeats per replay
@@ -1781,7 +1817,7 @@ extern "C" int mqct_hip_batched_tile(mqct_hip_t *h, int shape, int *form, int *e
   int sh = shape;
   if (sh < 0) {
     if (!h->dyn) return mqct_fail(MQCT_ERR_STATE, "batched_tile: no matrix loaded");
-    sh = h->dyn->gemm_first[h->dyn->km_fp32 ? 1 : 0];
+    sh = h->dyn->gemm_first[h->dyn->last_form];
     if (sh < 0) return mqct_fail(MQCT_ERR_STATE, "batched_tile: no batched run yet");
   }
   *form = kShapes[sh].form;
@@ -1792,17 +1828,19 @@ extern "C" int mqct_hip_batched_tile(mqct_hip_t *h, int shape, int *form, int *e
 }
 
 extern "C" int mqct_hip_batched_tile_levels(mqct_hip_t *h, int fp32, int n, int *shapes) {
-  if (!h || !h->dyn || !shapes || n < 0) return mqct_fail(MQCT_ERR_ARG, "batched_tile_levels: bad argument");
-  for (int l = 0; l < n; l++) shapes[l] = l < DynState::kLevels ? h->dyn->gemm_shape[fp32 ? 1 : 0][l] : -1;
+  if (!h || !h->dyn || !shapes || n < 0 || fp32 < 0 || fp32 > 2)
+    return mqct_fail(MQCT_ERR_ARG, "batched_tile_levels: bad argument");
+  for (int l = 0; l < n; l++) shapes[l] = l < DynState::kLevels ? h->dyn->gemm_shape[fp32][l] : -1;
   return MQCT_OK;
 }
 
 extern "C" int mqct_hip_batched_set_tile_levels(mqct_hip_t *h, int fp32, int n, const int *shapes) {
   if (!h || !h->dyn) return mqct_fail(MQCT_ERR_STATE, "batched_set_tile_levels: no matrix loaded");
-  if (!shapes || n < 0 || n > DynState::kLevels) return mqct_fail(MQCT_ERR_ARG, "batched_set_tile_levels: bad argument");
+  if (!shapes || n < 0 || n > DynState::kLevels || fp32 < 0 || fp32 > 2)
+    return mqct_fail(MQCT_ERR_ARG, "batched_set_tile_levels: bad argument");
   for (int l = 0; l < n; l++)
     if (shapes[l] < -1 || shapes[l] >= kNShapes) return mqct_fail(MQCT_ERR_ARG, "batched_set_tile_levels: unknown shape");
-  for (int l = 0; l < n; l++) h->dyn->gemm_shape[fp32 ? 1 : 0][l] = shapes[l];
+  for (int l = 0; l < n; l++) h->dyn->gemm_shape[fp32][l] = shapes[l];
   return MQCT_OK;
 }
 

@@ -22,14 +22,16 @@ struct DynState {
   bool km_fp32 = false;   // ... stored in FP32
   DevBuf d_KT;       // the same matrices in MFMA-fragment order (16-row tiles)
   bool kt_valid = false, kt_fp32 = false;
-  // interval-GEMM tile shape per coefficient precision (FP64, FP32) and
+  // interval-GEMM tile shape per product form (FP64 knots, FP32 knots, FP32
+  // knots with FP32 products) and
   // level of the running-trajectory count (floor(log2(active))), -1 not yet
   // chosen; a shape fixed by the caller (>= 0) overrides them
   static constexpr int kLevels = 24;
-  int gemm_shape[2][kLevels];
+  int gemm_shape[3][kLevels];
   int gemm_fixed = -1;
   int step_threads = 0;  // batched step kernel: 0 automatic, 256 or 512
-  int gemm_first[2] = {-1, -1};  // shape the last run started with
+  int gemm_first[3] = {-1, -1, -1};  // shape the last run started with
+  int last_form = 0;                  // product form of the last run
   DynState() {
     for (auto &p : gemm_shape)
       for (int &v : p) v = -1;
@@ -52,4 +54,4 @@ int mqct_dyn_run_plain(mqct_hip *h, int mode, int n_traj,
                        double *rec_out = nullptr, int *rec_n_out = nullptr);
 int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj,
                          const mqct_traj_t *traj, double *q_out,
-                         mqct_traj_stat_t *stat_out, bool knots_fp32);
+                         mqct_traj_stat_t *stat_out, bool knots_fp32, bool prod_fp32 = false);

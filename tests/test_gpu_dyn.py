@@ -391,3 +391,37 @@ def test_at_batched_tile_levels(dev):
     d = np.max(np.abs(q1[:, :2 * N] - q0[:, :2 * N]))
     assert d <= 1e-9, f"batched with set levels vs plain amplitude diff {d}"
     assert np.array_equal(s0["n_steps"], s1["n_steps"])
+
+
+@pytest.mark.parametrize("shape", [15, 16, 17, 18, 19, 20, 21, -1])
+def test_at_batched_fp32_products(dev, shape):
+    """FP32 products (batched=3): the pipelined GEMM forms multiply the FP32
+    knot matrices by the weighted amplitudes rounded to FP32 on f32 MFMA
+    with FP32 accumulation; the other forms, the step kernel and the
+    amplitudes stay FP64.  Against the plain kernel's RK4 trajectories on
+    the benchmark basis (m-blocks of up to 121 states): amplitudes within
+    1e-6 (1.2e-7 measured) and the same step counts.  shape -1: the tuner's
+    choices."""
+    from mqct_amd.matrix import MatrixBuilder
+    from mqct_amd.propagate import at_first_step
+    wl = W.asym_top_atom(n_r=120)
+    bb, E = W.at_batch(16, seed=37)
+    paths, l, _ = at_first_step(dev, W.asym_top_atom(jmax=1, n_beta=12, n_gamma=24, n_r=120), bb, E,
+                                time_step=20.0, rec_cap=20000)
+    MatrixBuilder(dev, wl).build("fp64", copy_out=False)
+    prop = Propagator(dev, wl.basis, wl.R, mass_amu=wl.mass_amu, from_build=True)
+    prop.load_paths(paths)
+    rows = []
+    for i in range(len(bb)):
+        rows += prop.trajectories([l[i]], float(E[i]), 1, time_step=10.0, at_paths=[i])
+    q0, s0 = prop.run(L.MQCT_PROP_AT_RK4, rows, batched=0)
+    L.check(dev.lib.mqct_hip_batched_set_tile(dev.h, shape), "set_tile")
+    try:
+        q1, s1 = prop.run(L.MQCT_PROP_AT_RK4, rows, batched=3)
+    finally:
+        L.check(dev.lib.mqct_hip_batched_set_tile(dev.h, -1), "set_tile")
+    N = wl.basis.n_states
+    d = np.max(np.abs(q1[:, :2 * N] - q0[:, :2 * N]))
+    print(f"shape {shape}: fp32-product amplitude diff {d:.3e}")
+    assert 0.0 < d <= 1e-6, f"shape {shape}: fp32-product batched vs plain amplitude diff {d}"
+    assert np.array_equal(s0["n_steps"], s1["n_steps"])
