@@ -204,7 +204,7 @@ int mqct_hip_propagate_timing(mqct_hip_t *h, double *ms);
  * right-hand side with R inside the grid, S2 = sum over m-blocks of nb^2)
  * and their algorithmic bytes (the knot matrices of every non-empty bucket,
  * 2*S2 values each, plus 64*N bytes of amplitudes in and products out per
- * right-hand side). */
+ * right-hand side, 48*N where the products are stored in FP32). */
 int mqct_hip_batched_profile(mqct_hip_t *h, int on);
 /* The GEMM and step timing of the profiled runs since the last
  * batched_profile call, per group of event-timed iterations: rows of

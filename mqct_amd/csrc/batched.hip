@@ -89,7 +89,7 @@ struct BDev {
   const long long *poff;
   double *S;                  // [n_traj][NSLOT][2N]
   double2 *V;                 // [n_traj][N]
-  double2 *W;                 // [n_traj][2][N]
+  double2 *W;                 // [n_traj][2][N] (float2 [n_traj][2][N] after FP32 products)
   Ctl *ctl;
   int *cnt;                   // [2][n_r]
   Entry *bucket;              // [2][n_r][n_traj]
@@ -852,11 +852,22 @@ __device__ void pipe_item(const BDev &d, int par, int r, int b, int c, PipeShare
       if (ct >= nct) break;
       const int cc = 16 * ct + col, e = cc >> 1;
       if (e >= nE) continue;
-      double *wb = reinterpret_cast<double *>(d.W + ((long long)sh.t[e] * 2 + sh.side[e]) * d.N + p0) + (cc & 1);
+      const long long wo = ((long long)sh.t[e] * 2 + sh.side[e]) * d.N + p0;
+      // FP32 products are stored as they were accumulated, in FP32
+      if constexpr (M32) {
+        float *wb = reinterpret_cast<float *>(reinterpret_cast<float2 *>(d.W) + wo) + (cc & 1);
 #pragma unroll
-      for (int i = 0; i < 4; i++) {
-        const int orow = rt0 + (M32 ? 4 * q + i : q + 4 * i);
-        if (orow < nb) wb[2 * orow] = (double)acc[h][ct][i];
+        for (int i = 0; i < 4; i++) {
+          const int orow = rt0 + 4 * q + i;
+          if (orow < nb) wb[2 * orow] = acc[h][ct][i];
+        }
+      } else {
+        double *wb = reinterpret_cast<double *>(d.W + wo) + (cc & 1);
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          const int orow = rt0 + q + 4 * i;
+          if (orow < nb) wb[2 * orow] = acc[h][ct][i];
+        }
       }
     }
   }
@@ -992,7 +1003,7 @@ void launch_gemm(int shape, dim3 knots_blocks, int maxnb, hipStream_t st, const 
 // update, the next stage vector -- so k is stored only when a later stage
 // reads it, and a finished step's y feeds the next stage without a reload.
 template <int PT, int TB = BS>
-__global__ __launch_bounds__(TB) void step_kernel(BDev d, int par) {
+__global__ __launch_bounds__(TB) void step_kernel(BDev d, int par, int w32) {
   extern __shared__ double sys[];  // 2N: the completed stage's vector (Coriolis neighbours)
   __shared__ double red[8];
   const int t = blockIdx.x, tid = threadIdx.x, N = d.N, n2 = 2 * N;
@@ -1011,7 +1022,9 @@ __global__ __launch_bounds__(TB) void step_kernel(BDev d, int par) {
   double Ej[PT], yr[PT], yi[PT];
   {
     const double2 *vv = d.V + (long long)t * N;
+    // the GEMM's products: FP64, or FP32 after the FP32-product forms (w32)
     const double2 *W0 = d.W + ((long long)t * 2) * N, *W1 = W0 + N;
+    const float2 *W0f = reinterpret_cast<const float2 *>(d.W) + ((long long)t * 2) * N, *W1f = W0f + N;
     double2 vj[PT], w0j[PT], w1j[PT], fj[PT];
     int2 nj[PT];
 #pragma unroll
@@ -1024,8 +1037,14 @@ __global__ __launch_bounds__(TB) void step_kernel(BDev d, int par) {
         Ej[j] = d.E[i];
         vj[j] = vv[i];
         if (!c.zero) {
-          w0j[j] = W0[i];
-          w1j[j] = W1[i];
+          if (w32) {
+            const float2 a = W0f[i], b = W1f[i];
+            w0j[j] = make_double2(a.x, a.y);
+            w1j[j] = make_double2(b.x, b.y);
+          } else {
+            w0j[j] = W0[i];
+            w1j[j] = W1[i];
+          }
         }
         yr[j] = y[i];
         yi[j] = y[i + N];
@@ -1567,15 +1586,17 @@ int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *t
   // thread, twice the loads in flight) when the states fill them
   const int stb = ds->step_threads > 0 ? ds->step_threads : (N > 4 * BS ? 512 : 256);
   if (stb != 256 && stb != 512) return mqct_fail(MQCT_ERR_ARG, "batched: step threads must be 256 or 512");
-  auto step = [&](int par) {
+  // the step kernel reads FP32 products after the FP32-product GEMM forms
+  auto w32_of = [&](int sh) -> int { return prod_fp32 && sh >= 0 && kShapes[sh].form >= kPipe ? 1 : 0; };
+  auto step = [&](int par, int w32) {
     if (stb == 512) {
-      if (N <= 4 * 512) step_kernel<4, 512><<<n_traj, 512, lds_step, h->stream>>>(d, par);
-      else step_kernel<8, 512><<<n_traj, 512, lds_step, h->stream>>>(d, par);
+      if (N <= 4 * 512) step_kernel<4, 512><<<n_traj, 512, lds_step, h->stream>>>(d, par, w32);
+      else step_kernel<8, 512><<<n_traj, 512, lds_step, h->stream>>>(d, par, w32);
       return;
     }
-    if (PT == 4) step_kernel<4><<<n_traj, BS, lds_step, h->stream>>>(d, par);
-    else if (PT == 8) step_kernel<8><<<n_traj, BS, lds_step, h->stream>>>(d, par);
-    else step_kernel<MAXPT><<<n_traj, BS, lds_step, h->stream>>>(d, par);
+    if (PT == 4) step_kernel<4><<<n_traj, BS, lds_step, h->stream>>>(d, par, w32);
+    else if (PT == 8) step_kernel<8><<<n_traj, BS, lds_step, h->stream>>>(d, par, w32);
+    else step_kernel<MAXPT><<<n_traj, BS, lds_step, h->stream>>>(d, par, w32);
   };
   if (PT == 4) init_kernel<4><<<n_traj, BS, 0, h->stream>>>(d, d_tr.as<mqct_traj_t>(), d_sp.as<int>());
   else if (PT == 8) init_kernel<8><<<n_traj, BS, 0, h->stream>>>(d, d_tr.as<mqct_traj_t>(), d_sp.as<int>());
@@ -1598,19 +1619,29 @@ int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *t
     for (auto &e : ds->prof_ev) HIP_TRY(hipEventCreate(&e));
   }
   // n_it iterations launched one by one with HIP events on this stream
-  // around every GEMM and step launch; returns the summed GEMM time
+  // around every GEMM and step launch; returns the summed GEMM time.  In
+  // profiled runs the algorithmic work of the chunk is added up from the
+  // device counts of bucketed right-hand sides and touched knots:
+  // per bucketed RHS 2 sides x 2 matrices x nb x nb x (Re, Im)
+  // multiply-adds summed over the blocks = 16 S2 flops, its rotated
+  // amplitudes read for both entries (2 x 16 N bytes) and both partial
+  // products written (2 x 16 N bytes, or 2 x 8 N as FP32 products); each
+  // touched knot's matrices are read once (2 S2 values)
+  unsigned long long prof_seen[2] = {0, 0};
   auto timed_iters = [&](int sh, int n_it, double &gemm_ms) -> int {
     const int active0 = active;
+    unsigned long long cnts[2] = {0, 0};
     for (int it = 0; it < n_it; it++) {
       const int par = (int)((it_done + it) & 1);
       HIP_TRY(hipEventRecord(ds->prof_ev[3 * it], h->stream));
       gemm(sh, par);
       HIP_TRY(hipEventRecord(ds->prof_ev[3 * it + 1], h->stream));
-      step(par ^ 1);
+      step(par ^ 1, w32_of(sh));
       HIP_TRY(hipEventRecord(ds->prof_ev[3 * it + 2], h->stream));
     }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(&active, d.active, 4, hipMemcpyDeviceToHost, h->stream));
+    if (prof) HIP_TRY(hipMemcpyAsync(cnts, base + off_act + 8, 16, hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
     gemm_ms = 0.0;
     double step_ms = 0.0;
@@ -1623,6 +1654,12 @@ int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *t
     }
     it_done += n_it;
     if (prof) {
+      const double nb_rhs = (double)(cnts[0] - prof_seen[0]), n_knots = (double)(cnts[1] - prof_seen[1]);
+      prof_seen[0] = cnts[0];
+      prof_seen[1] = cnts[1];
+      ds->prof_flops += 16.0 * (double)S2 * nb_rhs;
+      ds->prof_bytes += (32.0 + (w32_of(sh) ? 16.0 : 32.0)) * N * nb_rhs +
+                        2.0 * (double)S2 * (knots_fp32 ? 4 : 8) * n_knots;
       ds->prof_ms += gemm_ms;
       ds->prof_launches += n_it;
       if (ds->prof_trace.size() < (1u << 22))
@@ -1728,7 +1765,7 @@ int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *t
       for (int it = 0; it < ITER; it++) {
         const int par = (it + p0) & 1;
         gemm(shape, par);
-        step(par ^ 1);
+        step(par ^ 1, w32_of(shape));
       }
       const hipError_t cap = hipStreamEndCapture(h->stream, &gg.g);
       HIP_TRY(cap);
@@ -1747,16 +1784,6 @@ int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *t
   HIP_TRY(hipEventElapsedTime(&ms, h->ev[5], h->ev[6]));
   ds->last_ms = ms;
   if (active > 0) return mqct_fail(MQCT_ERR_HIP, "batched: trajectories did not finish");
-  if (prof) {
-    unsigned long long cnts[2] = {0, 0};
-    HIP_TRY(hipMemcpy(cnts, base + off_act + 8, 16, hipMemcpyDeviceToHost));
-    // per bucketed right-hand side: 2 sides x 2 matrices x nb x nb x (Re, Im)
-    // multiply-adds summed over the blocks = 16 S2 flops; it reads its
-    // rotated amplitudes for both entries and writes both partial products
-    // (4 x 16 N bytes); each touched knot's matrices are read once
-    ds->prof_flops += 16.0 * (double)S2 * (double)cnts[0];
-    ds->prof_bytes += 64.0 * N * (double)cnts[0] + 2.0 * (double)S2 * (knots_fp32 ? 4 : 8) * (double)cnts[1];
-  }
   if (ds->d_q.ensure((size_t)n_traj * (n2 + 8) * 8) ||
       ds->d_stat.ensure((size_t)n_traj * sizeof(mqct_traj_stat_t)))
     return mqct_fail(MQCT_ERR_NOMEM, "batched: output");
