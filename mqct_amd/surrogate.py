@@ -3,7 +3,7 @@ an MLP 12 -> 256 x 4 -> 1 (SiLU, BF16 weights, FP32 accumulation) trained
 and evaluated by the library's HIP kernels.  It stands where the reference
 calls the user PES on the angular grid (pes_sys_type.f:88-97, through
 INI_POT_STORAGE); the plain potential stays the fallback below r_min and
-the judge of the network's error.
+the reference against which the network's error is measured.
 
     sur = Surrogate(dev, seed=1)
     geo = sample_geometries(4_000_000, 4.0, 25.0, seed=...)

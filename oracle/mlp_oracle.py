@@ -1,8 +1,8 @@
 """FP64 numpy restatement of the PES surrogate network (12 -> 256 x 4 -> 1,
 SiLU) and of its mean-squared-error gradient.
 
-TEST INFRASTRUCTURE ONLY: imported by tests/ as the checker of the HIP
-training and inference kernels; the product never imports it.
+TEST INFRASTRUCTURE ONLY: imported by tests/ to check the HIP training and
+inference kernels against; the product never imports it.
 
 The network has no counterpart in the reference (the reference evaluates
 its PES routines directly, pot_comp.f / pes_sys_type.f); what this file

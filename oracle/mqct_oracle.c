@@ -4,7 +4,7 @@
  * in mqct_oracle_dyn.c.
  *
  * TEST INFRASTRUCTURE ONLY: the parity tests, the smoke check and the CPU
- * baseline of the benchmark use this as the checker.  Nothing in the product
+ * baseline of the benchmark compare against it.  Nothing in the product
  * (mqct_amd/, libmqct_hip.so) links or calls it.
  *
  * Pins (what each function is checked against, besides the GPU library):

@@ -1,9 +1,9 @@
 /*
  * mqct_oracle.h -- CPU FP64 restatement of the MQCT hot paths.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker the parity tests,
- * the smoke check and the CPU-baseline leg of the benchmark compare the HIP
- * library against.  Product code never links, loads or calls it.
+ * TEST INFRASTRUCTURE ONLY.  The parity tests, the smoke check and the
+ * CPU-baseline leg of the benchmark compare the HIP library against this
+ * library.  Product code never links, loads or calls it.
  *
  * Every function names the reference lines it restates (paths relative to
  * the MQCT source tree, MQCT_SOURCE_CODES/).  Indices that the reference

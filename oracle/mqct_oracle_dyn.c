@@ -4,7 +4,8 @@
  * DERIVS_BK / DERIVS_BK_adia, the Numerical-Recipes integrators and the
  * stepping loop of DOTRAJECT.
  *
- * TEST INFRASTRUCTURE ONLY (the checker); the product never calls it.
+ * TEST INFRASTRUCTURE ONLY (the reference results of the tests); the product
+ * never calls it.
  *
  * Pins: or_derivs_bk on a two-state constant coupling reproduces the Rabi
  * formula; diagonal-only M gives a pure phase; the norm sum |a|^2 is

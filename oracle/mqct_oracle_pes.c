@@ -2,8 +2,8 @@
  * mqct_oracle_pes.c -- the three synthetic potentials of the workloads and
  * the V-grid fill of pot_comp.f INI_POT_STORAGE (pot_comp.f:333-1788).
  *
- * TEST INFRASTRUCTURE ONLY (the checker); the product evaluates the same
- * potentials in its own HIP kernels.
+ * TEST INFRASTRUCTURE ONLY (the reference results of the tests); the product
+ * evaluates the same potentials in its own HIP kernels.
  *
  * Potentials are in cm^-1, R in bohr, angles in radians.  They play the role
  * of USER_DEFINED_PES (EXAMPLES_IO/.../PES_H2O_He.f:1-10), and the grid fill

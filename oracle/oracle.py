@@ -1,8 +1,8 @@
 """ctypes front end of the CPU oracle (oracle/_build/libmqct_oracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-CPU-baseline leg of bench.py, always as the checker.  The product (mqct_amd)
-never imports it.
+CPU-baseline leg of bench.py, always to check against or as the timed CPU
+baseline.  The product (mqct_amd) never imports it.
 
 Each wrapper names the oracle C function it calls; those name the reference
 lines they restate.
