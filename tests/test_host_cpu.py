@@ -120,3 +120,15 @@ def test_mtrx_uf_layout_and_round_trip():
         assert np.array_equal(back["mat_el"], M) and np.array_equal(back["ind_mat"], b.ind_mat)
         assert np.array_equal(back["j12"], b.j12) and np.array_equal(back["R"], wl.R)
         assert np.array_equal(back["chan_qn"], b.chan_qn())
+
+
+def test_bench_refuses_several_gpus():
+    """bench.py measures one GPU: asked for more it exits with status 2 and
+    says why, before touching a device."""
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120, cwd=root)
+    assert r.returncode == 2
+    assert "one GPU" in (r.stdout + r.stderr)
