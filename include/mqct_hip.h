@@ -232,8 +232,8 @@ int mqct_hip_batched_tile(mqct_hip_t *h, int shape, int *form, int *entries,
  * the FP32 knot matrices with FP32 products (fp32=2). */
 int mqct_hip_batched_tile_levels(mqct_hip_t *h, int fp32, int n, int *shapes);
 /* Sets the shapes of levels 0..n-1 (n <= 24; -1: chosen by timing) for the
- * FP64 or FP32 knot matrices, e.g. to repeat a measured run's choices in a
- * profiled one.  A batched run refuses a set shape that does not fit its
+ * product form fp32 (0, 1, 2 as in batched_tile_levels), e.g. to repeat a
+ * measured run's choices in a profiled one.  A batched run refuses a set shape that does not fit its
  * matrix (pipelined forms: m-blocks of at most 128 states; work-queue forms:
  * at most 16383 knots). */
 int mqct_hip_batched_set_tile_levels(mqct_hip_t *h, int fp32, int n, const int *shapes);
