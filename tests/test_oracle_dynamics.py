@@ -62,3 +62,61 @@ def test_rk4_is_fourth_order():
         errs.append(abs(a1 - ref))
     r1, r2 = errs[0] / errs[1], errs[1] / errs[2]
     assert 12.0 < r1 < 20.0 and 12.0 < r2 < 20.0
+
+
+# -- the Cash-Karp step itself, on scalar problems with known solutions -----
+
+_DERIVS = None
+
+
+def _rkck(f, y0, x, h):
+    """One or_rkck step of y' = f(t, y) (scalar) from (x, y0): (y5, yerr)."""
+    import ctypes
+    global _DERIVS
+    lib = O.lib()
+    dp = ctypes.POINTER(ctypes.c_double)
+    if _DERIVS is None:
+        _DERIVS = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_double, dp, dp)
+    cb = _DERIVS(lambda ctx, t, q, dq: dq.__setitem__(0, f(t, q[0])))
+    lib.or_rkck.argtypes = [ctypes.c_void_p, dp, dp, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                            dp, dp, _DERIVS]
+    lib.or_rkck.restype = None
+    y = np.array([y0] + [0.0] * 8)
+    dydx = np.array([f(x, y0)] + [0.0] * 8)
+    yout = np.zeros(9)
+    yerr = np.zeros(9)
+    lib.or_rkck(None, y.ctypes.data_as(dp), dydx.ctypes.data_as(dp), 1, x, h,
+                yout.ctypes.data_as(dp), yerr.ctypes.data_as(dp), cb)
+    return yout[0], yerr[0]
+
+
+def test_rkck_fifth_order_and_embedded_fourth():
+    """y' = -2 t y^2, y(0) = 1 (y = 1 / (1 + t^2)): nonlinear and
+    non-autonomous, so a wrong Butcher coefficient or node lowers the
+    order.  Halving h divides the local error of the fifth-order solution
+    by about 2^6 and the embedded error estimate (fifth minus fourth order)
+    by about 2^5."""
+    f = lambda t, y: -2.0 * t * y * y
+    exact = lambda t: 1.0 / (1.0 + t * t)
+    x = 0.3
+    e5, ee = [], []
+    for h in (0.2, 0.1, 0.05):
+        y5, yerr = _rkck(f, exact(x), x, h)
+        e5.append(abs(y5 - exact(x + h)))
+        ee.append(abs(yerr))
+    for a, b in zip(e5, e5[1:]):
+        assert 40.0 < a / b < 100.0, e5
+    for a, b in zip(ee, ee[1:]):
+        assert 22.0 < a / b < 45.0, ee
+
+
+def test_rkck_keeps_the_single_precision_node():
+    """y' = t: every order condition of the fifth-order weights holds, so the
+    step is exact but for the node a4, which the reference writes as the
+    REAL*4 literal .6 (suppl_routines.f:1301): the increment exceeds h^2/2
+    by C4 h^2 (fl32(0.6) - 0.6), C4 = 125/594."""
+    x, h = 0.0, 1.0
+    y5, _ = _rkck(lambda t, y: t, 0.0, x, h)
+    shift = 125.0 / 594.0 * h * h * (float(np.float32(0.6)) - 0.6)
+    assert abs(shift) > 4e-9
+    assert abs(y5 - (h * h / 2.0 + shift)) < 1e-15
