@@ -132,3 +132,15 @@ def test_bench_refuses_several_gpus():
                        capture_output=True, text=True, timeout=120, cwd=root)
     assert r.returncode == 2
     assert "one GPU" in (r.stdout + r.stderr)
+
+
+def test_cpu_baseline_sample_is_nearest_to_the_mean():
+    """The CPU baseline times the trajectories whose right-hand-side counts
+    are closest to the batch mean."""
+    import importlib.util
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    spec = importlib.util.spec_from_file_location("bench_mod", os.path.join(root, "bench.py"))
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    assert bench.baseline_sample(np.array([10, 50, 30, 31, 29, 100]), 4) == [1, 3, 2, 4]
+    assert bench.baseline_sample(np.array([7]), 4) == [0]
