@@ -469,7 +469,9 @@ def cpu_baseline(dev, wl, paths, rows, st, n_sample=4):
     pick = baseline_sample(st["n_rhs"], n_sample)
     M, D, S = MatrixBuilder(dev, wl).build("fp64", copy_out=True)
     b = wl.basis
-    dyn = O.DynOracle(b.j12, b.m12, b.Ej * CM_TO_HARTREE, b.ind_mat, wl.R, M, D, wl.mass_amu * AMUTOAU)
+    # the GPU propagates M(R) - M(R_max) (MIJ_SHIFT=YES, matrix_ini.f:2226-2230)
+    dyn = O.DynOracle(b.j12, b.m12, b.Ej * CM_TO_HARTREE, b.ind_mat, wl.R, O.mij_shift(M), D,
+                      wl.mass_amu * AMUTOAU)
     el, nrhs = 0.0, 0
     for i in pick:
         dyn.set_path(paths[i])

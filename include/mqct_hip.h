@@ -128,6 +128,23 @@ int mqct_hip_matrix_get(mqct_hip_t *h, double *mat_el, double *mat_el_der);
  * (mqct_hip_contract_variant); that time is not in [2] or [4]. */
 int mqct_hip_build_timing(mqct_hip_t *h, double *ms5);
 
+/* MIJ_SHIFT= keyword (iotest.f:3754 default YES, parsed at iotest.f:4867).
+ * on=1 (the default of a new handle): the propagation sees M(R) - M(R_max).
+ * mqct_hip_use_built_matrix then applies the compute branch's shift
+ * (matrix_ini.f:2226-2230): the values are shifted after the spline, so
+ * Mat_el_der stays that of the unshifted values.  The built matrix itself
+ * (mqct_hip_matrix_get, MTRX_UF.DAT) is never shifted, as the reference
+ * writes it before the shift. */
+int mqct_hip_set_mij_shift(mqct_hip_t *h, int on);
+
+/* READ_MIJ_USER's work after each column is read (matrix_ini.f:6603-6621),
+ * on the GPU for the whole matrix: Mat_el(i,k) -= Mat_el(n_r,k) when the
+ * handle's MIJ_SHIFT is on, then the clamped spline of the (shifted)
+ * values into Mat_el_der with finite-difference end slopes.  mat_el is
+ * updated in place; the result is what mqct_hip_load_matrix takes. */
+int mqct_hip_mij_spline(mqct_hip_t *h, int n_r, const double *R_com,
+                        int64_t total_size, double *mat_el, double *mat_el_der);
+
 /* ---------------------------------------------------------------------- */
 /* Propagation: replaces the stepping of DOTRAJECT (propagator.f:3205-4137)
  * with DERIVS_BK / DERIVS_BK_adia (mqct_eqtns.f:112-499, 720-879), the

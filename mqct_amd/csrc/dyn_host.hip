@@ -8,15 +8,19 @@
 namespace {
 
 // Mt[r][k] = M[k][r]: the R-major copy the right-hand side reads
-// (bk_mat_splnt = transpose(Mat_el), mqct_eqtns.f:104-105).
-__global__ void transpose_kernel(long long T, int n_r,
+// (bk_mat_splnt = transpose(Mat_el), mqct_eqtns.f:104-105).  shift: the
+// compute branch's MIJ_SHIFT, Mat_el(i,k) - Mat_el(n_r,k) for every i
+// (matrix_ini.f:2226-2230; the last R point becomes exactly 0).
+__global__ void transpose_kernel(long long T, int n_r, int shift,
                                  const double *__restrict__ M,
                                  double *__restrict__ Mt) {
   long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= T * n_r) return;
   long long k = idx / n_r;
   int r = (int)(idx % n_r);
-  Mt[(long long)r * T + k] = M[idx];
+  double v = M[idx];
+  if (shift) v = v - M[k * n_r + n_r - 1];
+  Mt[(long long)r * T + k] = v;
 }
 
 template <class T>
@@ -66,11 +70,12 @@ int set_tables(mqct_hip *h, DynState *ds, const int *ind_mat) {
   return MQCT_OK;
 }
 
-int transpose_into(mqct_hip *h, DynState *ds, const double *d_src, DevBuf &dst) {
+int transpose_into(mqct_hip *h, DynState *ds, const double *d_src, DevBuf &dst,
+                   bool shift = false) {
   const long long n = ds->T * ds->n_r;
   if (dst.ensure((size_t)n * 8)) return mqct_fail(MQCT_ERR_NOMEM, "spline coefficients");
   transpose_kernel<<<(unsigned)((n + 255) / 256), 256, 0, h->stream>>>(
-      ds->T, ds->n_r, d_src, dst.as<double>());
+      ds->T, ds->n_r, shift ? 1 : 0, d_src, dst.as<double>());
   HIP_TRY(hipGetLastError());
   return MQCT_OK;
 }
@@ -145,7 +150,9 @@ extern "C" int mqct_hip_use_built_matrix(mqct_hip_t *h, const double *Ej, double
       }
   int rc = set_tables(h, ds, ind.data());
   if (rc) return rc;
-  if ((rc = transpose_into(h, ds, h->d_mat.as<double>(), ds->d_Mt))) return rc;
+  // MIJ_SHIFT acts after the spline in the compute branch: the values are
+  // shifted, their second derivatives are those of the unshifted values
+  if ((rc = transpose_into(h, ds, h->d_mat.as<double>(), ds->d_Mt, h->mij_shift))) return rc;
   if ((rc = transpose_into(h, ds, h->d_der.as<double>(), ds->d_Y2t))) return rc;
   HIP_TRY(hipStreamSynchronize(h->stream));
   return MQCT_OK;

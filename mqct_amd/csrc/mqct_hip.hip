@@ -368,32 +368,16 @@ extern "C" int mqct_hip_eval_pes(mqct_hip_t *h, int n_r, const double *R_com,
 }
 
 // ---------------------------------------------------------------------------
-// Cutoff (matrix_ini.f:1991-1997) and clamped spline (matrix_ini.f:2005-2016
-// with suppl_routines.f:914-965).  One thread per element k; the spline's
-// forward factors y2f depend only on the R grid and come precomputed, the
-// data-dependent u sweep is kept in the output array and back-substituted in
-// place.
-__global__ void post_kernel(int64_t T, int n, const double *__restrict__ R,
-                            const double *__restrict__ sig,
-                            const double *__restrict__ pp,
-                            const double *__restrict__ y2f, double cut,
-                            int ir1, int ir2,
-                            const double *__restrict__ cutv,
-                            double *__restrict__ M,
-                            double *__restrict__ D,
-                            unsigned char *__restrict__ skip) {
-  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= T) return;
-  double *y = M + k * n, *d = D + k * n;
-  // cutv (mixed precision): FP64 values at the two cutoff points
-  double a1 = cutv ? fabs(cutv[2 * k]) : fabs(y[ir1]);
-  double a2 = cutv ? fabs(cutv[2 * k + 1]) : fabs(y[ir2]);
-  if ((a1 > a2 ? a1 : a2) < cut) {
-    for (int i = 0; i < n; i++) { y[i] = 0.0; d[i] = 0.0; }
-    skip[k] = 1;
-    return;
-  }
-  skip[k] = 0;
+// Clamped cubic spline of one element's n values y (matrix_ini.f:2005-2016,
+// 6611-6621 with suppl_routines.f:914-965): end slopes from the first and
+// last finite differences.  The forward factors y2f depend only on the R
+// grid and come precomputed; the data-dependent u sweep is kept in d and
+// back-substituted in place.
+__device__ inline void spline_column(int n, const double *__restrict__ R,
+                                     const double *__restrict__ sig,
+                                     const double *__restrict__ pp,
+                                     const double *__restrict__ y2f,
+                                     const double *y, double *d) {
   double yp1 = (y[1] - y[0]) / (R[1] - R[0]);
   double ypn = (y[n - 1] - y[n - 2]) / (R[n - 1] - R[n - 2]);
   double u = (3.0 / (R[1] - R[0])) * ((y[1] - y[0]) / (R[1] - R[0]) - yp1);
@@ -420,20 +404,71 @@ __global__ void post_kernel(int64_t T, int n, const double *__restrict__ R,
   }
 }
 
-int mqct_run_post(mqct_hip *h, double cut, int ir1, int ir2,
-                  const double *cutv) {
-  int n = h->n_r;
-  std::vector<double> sig(n, 0.0), pp(n, 1.0), y2f(n, 0.0);
+// Cutoff (matrix_ini.f:1991-1997) and clamped spline of the compute branch.
+// One thread per element k.
+__global__ void post_kernel(int64_t T, int n, const double *__restrict__ R,
+                            const double *__restrict__ sig,
+                            const double *__restrict__ pp,
+                            const double *__restrict__ y2f, double cut,
+                            int ir1, int ir2,
+                            const double *__restrict__ cutv,
+                            double *__restrict__ M,
+                            double *__restrict__ D,
+                            unsigned char *__restrict__ skip) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= T) return;
+  double *y = M + k * n, *d = D + k * n;
+  // cutv (mixed precision): FP64 values at the two cutoff points
+  double a1 = cutv ? fabs(cutv[2 * k]) : fabs(y[ir1]);
+  double a2 = cutv ? fabs(cutv[2 * k + 1]) : fabs(y[ir2]);
+  if ((a1 > a2 ? a1 : a2) < cut) {
+    for (int i = 0; i < n; i++) { y[i] = 0.0; d[i] = 0.0; }
+    skip[k] = 1;
+    return;
+  }
+  skip[k] = 0;
+  spline_column(n, R, sig, pp, y2f, y, d);
+}
+
+// READ_MIJ_USER after each column is read (matrix_ini.f:6603-6621): the
+// MIJ_SHIFT subtraction Mat_el(i,k) - Mat_el(n_r,k), then the spline of the
+// shifted values.
+__global__ void read_post_kernel(int64_t T, int n, const double *__restrict__ R,
+                                 const double *__restrict__ sig,
+                                 const double *__restrict__ pp,
+                                 const double *__restrict__ y2f, int shift,
+                                 double *__restrict__ M,
+                                 double *__restrict__ D) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= T) return;
+  double *y = M + k * n, *d = D + k * n;
+  if (shift) {
+    double last = y[n - 1];
+    for (int i = 0; i < n; i++) y[i] = y[i] - last;
+  }
+  spline_column(n, R, sig, pp, y2f, y, d);
+}
+
+// Forward factors of the spline's tridiagonal sweep for the grid R:
+// tab = [sig | p | y2 forward] (suppl_routines.f:930-945).
+static std::vector<double> spline_tables(const std::vector<double> &R) {
+  const int n = (int)R.size();
+  std::vector<double> tab(3 * (size_t)n, 0.0);
+  double *sig = &tab[0], *pp = &tab[n], *y2f = &tab[2 * n];
+  for (int i = 0; i < n; i++) pp[i] = 1.0;
   y2f[0] = -0.5;
   for (int i = 1; i < n - 1; i++) {
-    sig[i] = (h->R[i] - h->R[i - 1]) / (h->R[i + 1] - h->R[i - 1]);
+    sig[i] = (R[i] - R[i - 1]) / (R[i + 1] - R[i - 1]);
     pp[i] = sig[i] * y2f[i - 1] + 2.0;
     y2f[i] = (sig[i] - 1.0) / pp[i];
   }
-  std::vector<double> tab(3 * (size_t)n);
-  std::memcpy(&tab[0], sig.data(), n * 8);
-  std::memcpy(&tab[n], pp.data(), n * 8);
-  std::memcpy(&tab[2 * n], y2f.data(), n * 8);
+  return tab;
+}
+
+int mqct_run_post(mqct_hip *h, double cut, int ir1, int ir2,
+                  const double *cutv) {
+  int n = h->n_r;
+  std::vector<double> tab = spline_tables(h->R);
   int rc = upload(h, h->d_spl, tab.data(), tab.size());
   if (rc) return rc;
   const double *t = h->d_spl.as<double>();
@@ -531,6 +566,43 @@ extern "C" int mqct_hip_matrix_get(mqct_hip_t *h, double *mat_el,
 extern "C" int mqct_hip_build_timing(mqct_hip_t *h, double *ms5) {
   if (!h || !ms5) return mqct_fail(MQCT_ERR_ARG, "build_timing: null argument");
   for (int i = 0; i < 5; i++) ms5[i] = h->timing[i];
+  return MQCT_OK;
+}
+
+extern "C" int mqct_hip_set_mij_shift(mqct_hip_t *h, int on) {
+  if (!h) return mqct_fail(MQCT_ERR_ARG, "set_mij_shift: null handle");
+  h->mij_shift = on != 0;
+  return MQCT_OK;
+}
+
+extern "C" int mqct_hip_mij_spline(mqct_hip_t *h, int n_r, const double *R_com,
+                                   int64_t total_size, double *mat_el,
+                                   double *mat_el_der) {
+  if (!h || !R_com || !mat_el || !mat_el_der)
+    return mqct_fail(MQCT_ERR_ARG, "mij_spline: null argument");
+  if (n_r < 3 || total_size <= 0) return mqct_fail(MQCT_ERR_ARG, "mij_spline: bad sizes");
+  for (int i = 1; i < n_r; i++)
+    if (!(R_com[i] > R_com[i - 1]))
+      return mqct_fail(MQCT_ERR_ARG, "mij_spline: R grid must increase");
+  HIP_TRY(hipSetDevice(h->device));
+  std::vector<double> R(R_com, R_com + n_r);
+  std::vector<double> tab = spline_tables(R);
+  const size_t bytes = (size_t)total_size * n_r * 8;
+  DevBuf d_M, d_D, d_tab, d_Rg;
+  if (d_M.ensure(bytes) || d_D.ensure(bytes) || d_tab.ensure(tab.size() * 8) ||
+      d_Rg.ensure((size_t)n_r * 8))
+    return mqct_fail(MQCT_ERR_NOMEM, "mij_spline: matrix does not fit in HBM");
+  HIP_TRY(hipMemcpyAsync(d_M.p, mat_el, bytes, hipMemcpyHostToDevice, h->stream));
+  HIP_TRY(hipMemcpyAsync(d_tab.p, tab.data(), tab.size() * 8, hipMemcpyHostToDevice, h->stream));
+  HIP_TRY(hipMemcpyAsync(d_Rg.p, R.data(), (size_t)n_r * 8, hipMemcpyHostToDevice, h->stream));
+  const double *t = d_tab.as<double>();
+  read_post_kernel<<<(unsigned)((total_size + 255) / 256), 256, 0, h->stream>>>(
+      total_size, n_r, d_Rg.as<double>(), t, t + n_r, t + 2 * n_r, h->mij_shift ? 1 : 0,
+      d_M.as<double>(), d_D.as<double>());
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(mat_el, d_M.p, bytes, hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(hipMemcpyAsync(mat_el_der, d_D.p, bytes, hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
   return MQCT_OK;
 }
 

@@ -96,6 +96,9 @@ struct mqct_hip {
   // re-timed when the basis or grid changes) or fixed by the caller
   int cvar[3] = {-1, -1, -1}, cvar_fixed[3] = {-1, -1, -1};
   double tune_ms = 0.0;  // time the last build spent choosing the variant
+  // MIJ_SHIFT= keyword (default YES, iotest.f:3754): the propagation sees
+  // M(R) - M(R_max) (matrix_ini.f:2226-2230, 6603-6608)
+  bool mij_shift = true;
 
   MlpState *mlp = nullptr;
   DynState *dyn = nullptr;

@@ -7,7 +7,8 @@
 !
 ! Input: namelist &demo on the file named by the first argument.
 ! Output: MTRX_UF.DAT (Fortran), MTRX_UF_C.DAT (library), demo_out.bin
-! (stream: Mat_el, Mat_el_der, then the final sys_var of each trajectory).
+! (stream: the MIJ_SHIFTed Mat_el the trajectories used, Mat_el_der, then
+! the final sys_var of each trajectory).
 program mqct_fortran_demo
   use iso_c_binding
   use mqct_hip_bridge
@@ -105,6 +106,12 @@ program mqct_fortran_demo
   call mqct_check(mqct_hip_write_mtrx_uf("MTRX_UF_C.DAT" // c_null_char, 1_c_int, int(number_of_channels, c_int), &
       int(states_size, c_int), total_size, int(n_r, c_int), indx_chann, j12, m12, chan_qn, &
       reshape(ind_mat, [2 * int(total_size)]), R_COM, Mat_el), "write_mtrx_uf")
+
+  ! MIJ_SHIFT (default YES): the driver's own compute-branch shift after the
+  ! spline, before the propagation (matrix_ini.f:2226-2230)
+  do i = 1, n_r
+    Mat_el(i, :) = Mat_el(i, :) - Mat_el(n_r, :)
+  end do
 
   ! CC-MQCT RK4 trajectories from j = 0 (state 1)
   allocate(Ej(states_size), traj(n_traj), stats(n_traj), sys_var(2 * states_size + 8, n_traj))

@@ -28,6 +28,10 @@ class Device:
         L.check(self.lib.mqct_hip_create(device, ctypes.byref(h)), "mqct_hip_create")
         self.h = h
 
+    def set_mij_shift(self, on=True):
+        """MIJ_SHIFT= (default YES): propagate M(R) - M(R_max)."""
+        L.check(self.lib.mqct_hip_set_mij_shift(self.h, int(bool(on))), "set_mij_shift")
+
     def close(self):
         if self.h:
             self.lib.mqct_hip_destroy(self.h)
@@ -151,6 +155,27 @@ def read_mtrx_uf(path):
             "read_mtrx_uf")
     return {"coll_type": coll, "n_channels": nch, "n_states": ns, "total_size": T, "n_r": nr,
             "indx_chann": ich, "j12": j12, "m12": m12, "chan_qn": qn, "ind_mat": ind, "R": R, "mat_el": M}
+
+
+def mij_spline(dev, R, mat_el):
+    """READ_MIJ_USER's post-read step (matrix_ini.f:6603-6621) on the GPU:
+    MIJ_SHIFT per the handle's setting, then the clamped spline.  Returns
+    (Mat_el, Mat_el_der) ready for Propagator(mat_el=..., mat_el_der=...)."""
+    M = np.array(mat_el, dtype=np.float64, order="C", copy=True)
+    D = np.zeros_like(M)
+    R = L.as_f64(R)
+    L.check(dev.lib.mqct_hip_mij_spline(dev.h, len(R), L.dptr(R), M.shape[0], L.dptr(M), L.dptr(D)),
+            "mij_spline")
+    return M, D
+
+
+def read_mij_user(dev, path):
+    """MTRX_UF.DAT -> the arrays READ_MIJ_USER leaves for the propagation
+    (matrix_ini.f:5794-6621): the file's tables plus the shifted and
+    re-splined Mat_el / Mat_el_der."""
+    f = read_mtrx_uf(path)
+    f["mat_el"], f["mat_el_der"] = mij_spline(dev, f["R"], f["mat_el"])
+    return f
 
 
 def _main(argv=None):

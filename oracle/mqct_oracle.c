@@ -19,6 +19,9 @@
  *   or_integrator_temp       : closed-form Gaunt (type 1) and 3j (type 4)
  *                              matrix elements, brute-force sums (type 0)
  *   or_spline / or_splint    : exactness on cubics
+ *   or_mij_shift / or_read_mij_post : last R point exactly 0, differences
+ *                              kept exactly, the read-path spline equals
+ *                              the unshifted spline minus the constant
  */
 #include "mqct_oracle.h"
 #include <math.h>
@@ -764,5 +767,32 @@ void or_matrix_make(const or_matrix_ctx *c, int64_t total_size,
       double d1 = (col[n - 1] - col[n - 2]) / (R[n - 1] - R[n - 2]);
       or_spline(R, col, n, d0, d1, der);
     }
+  }
+}
+
+/* MIJ_SHIFT in MATRIX_MAKE's compute branch (matrix_ini.f:2226-2230), run
+ * after the spline: Mat_el(i,:) = Mat_el(i,:) - Mat_el(n_r_coll,:) for
+ * i = 1..n_r_coll in that order, so rows i < n_r_coll subtract the
+ * unshifted last row and the last row becomes 0.  Mat_el_der is left as
+ * the unshifted values' second derivatives. */
+void or_mij_shift(int64_t total_size, int n, double *mat_el) {
+  for (int i = 0; i < n; i++)
+    for (int64_t k = 0; k < total_size; k++)
+      mat_el[(size_t)k * n + i] = mat_el[(size_t)k * n + i] - mat_el[(size_t)k * n + n - 1];
+}
+
+/* READ_MIJ_USER after each column k is read (matrix_ini.f:6603-6621):
+ * with MIJ_SHIFT, Mat_el(i,k) - Mat_el(n_r_coll,k) for i in order; then
+ * the clamped spline with finite-difference end slopes of the values as
+ * they now are. */
+void or_read_mij_post(const double *R, int n, int64_t total_size, int shift,
+                      double *mat_el, double *mat_el_der) {
+  for (int64_t k = 0; k < total_size; k++) {
+    double *col = mat_el + (size_t)k * n;
+    if (shift)
+      for (int i = 0; i < n; i++) col[i] = col[i] - col[n - 1];
+    double d0 = (col[1] - col[0]) / (R[1] - R[0]);
+    double d1 = (col[n - 1] - col[n - 2]) / (R[n - 1] - R[n - 2]);
+    or_spline(R, col, n, d0, d1, mat_el_der + (size_t)k * n);
   }
 }

@@ -86,6 +86,9 @@ void or_matrix_make(const or_matrix_ctx *c, int64_t total_size,
                     const int *ind_mat, const double *R, double conv_unit_e,
                     double mij_zero_cut_hartree, int ir_cut1, int ir_cut2,
                     double *mat_el, double *mat_el_der, uint8_t *skipped);
+void or_mij_shift(int64_t total_size, int n, double *mat_el);
+void or_read_mij_post(const double *R, int n, int64_t total_size, int shift,
+                      double *mat_el, double *mat_el_der);
 void or_spline(const double *x, const double *y, int n, double yp1,
                double ypn, double *y2);
 void or_splint(const double *xa, const double *ya, const double *y2a, int n,

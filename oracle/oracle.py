@@ -105,6 +105,8 @@ def lib():
         L.or_matrix_make.argtypes = [ctypes.POINTER(MatrixCtx), ctypes.c_int64, _ip, _dp, _d, _d, _i, _i,
                                      _dp, _dp, _u8p]
         L.or_spline.argtypes = [_dp, _dp, _i, _d, _d, _dp]
+        L.or_mij_shift.argtypes = [ctypes.c_int64, _i, _dp]
+        L.or_read_mij_post.argtypes = [_dp, _i, ctypes.c_int64, _i, _dp, _dp]
         L.or_splint.argtypes = [_dp, _dp, _dp, _i, _d, _dp, _dp]
         L.or_nearest_r.argtypes = [_dp, _i, _d]
         L.or_pes_legendre.argtypes = [_dp, _d, _d]
@@ -326,6 +328,24 @@ def spline(x, y, yp1, ypn):
     y2 = np.zeros(len(x))
     lib().or_spline(dp(x), dp(y), len(x), yp1, ypn, dp(y2))
     return y2
+
+
+def mij_shift(mat_el):
+    """or_mij_shift: the compute branch's MIJ_SHIFT (matrix_ini.f:2226-2230)
+    on a copy of mat_el[k, i]."""
+    M = np.array(mat_el, dtype=np.float64, order="C", copy=True)
+    lib().or_mij_shift(M.shape[0], M.shape[1], dp(M))
+    return M
+
+
+def read_mij_post(R, mat_el, shift=True):
+    """or_read_mij_post: READ_MIJ_USER's shift and re-spline
+    (matrix_ini.f:6603-6621) on a copy of mat_el[k, i]; returns (M, D)."""
+    M = np.array(mat_el, dtype=np.float64, order="C", copy=True)
+    D = np.zeros_like(M)
+    R = f64(R)
+    lib().or_read_mij_post(dp(R), len(R), M.shape[0], int(shift), dp(M), dp(D))
+    return M, D
 
 
 def splint(xa, ya, y2a, x):

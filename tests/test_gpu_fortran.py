@@ -2,8 +2,8 @@
 the iso_c_binding bridge (mqct_amd/fortran).  Checks: the MTRX_UF.DAT that
 Fortran's own unformatted WRITEs produce in the PRINT_MIJ_USER layout is
 byte-identical to the library's writer; the matrix the Fortran host got
-matches the oracle to 1e-10 relative; its CC-MQCT RK4 trajectories match
-the oracle's to 1e-9."""
+matches the oracle to 1e-10 relative; after the driver's MIJ_SHIFT its
+CC-MQCT RK4 trajectories match the oracle's to 1e-9."""
 import os
 import subprocess
 
@@ -47,7 +47,14 @@ def test_fortran_host_drives_the_library(tmp_path):
     q = raw[2 * T * n:].reshape(2, 2 * N + 8)
     orc = O.oracle_for_workload(wl)
     Mo, Do, So = orc.matrix_make(b.ind_mat, wl.conv_unit_e, wl.mij_zero_cut, *wl.ir_cut)
-    assert np.max(np.abs(M - Mo) / np.maximum(np.abs(Mo), 1e-300)) <= 1e-10
+    # the demo applies the driver's MIJ_SHIFT (matrix_ini.f:2226-2230) before
+    # propagating; the file it wrote before that holds the unshifted values
+    Ms = O.mij_shift(Mo)
+    assert np.all(M[:, -1] == 0.0)
+    assert np.max(np.abs(M - Ms)) <= 1e-10 * np.abs(Mo).max()
+    from mqct_amd.matrix import read_mtrx_uf
+    Mf = read_mtrx_uf(str(tmp_path / "MTRX_UF.DAT"))["mat_el"]
+    assert np.max(np.abs(Mf - Mo) / np.maximum(np.abs(Mo), 1e-300)) <= 1e-10
     assert np.max(np.abs(D - Do)) <= 1e-10 * np.abs(Do).max()
     dyn = O.DynOracle(b.j12, b.m12, b.Ej * CM_TO_HARTREE, b.ind_mat, wl.R, M, D, 3.6 * AMUTOAU)
     dt = 20.0 / np.sqrt(500.0 / 300.0)

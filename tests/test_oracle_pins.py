@@ -287,3 +287,40 @@ def test_cutoff_and_spline_of_matrix_make():
             y2 = O.spline(wl.R, raw, (raw[1] - raw[0]) / (wl.R[1] - wl.R[0]),
                           (raw[-1] - raw[-2]) / (wl.R[-1] - wl.R[-2]))
             assert np.array_equal(D[k], y2)
+
+
+def test_mij_shift_of_both_branches():
+    """MIJ_SHIFT (default YES, iotest.f:3754).  Compute branch
+    (matrix_ini.f:2226-2230): every row subtracts the UNSHIFTED last row,
+    so the last R point becomes exactly 0 and the others keep their
+    differences exactly.  Read branch (matrix_ini.f:6603-6621): the same
+    shift, then the spline of the shifted values; a cubic spline of y - c
+    has the second derivatives of y, so both branches carry the same y''
+    to rounding and the interpolant moves by exactly the constant.  A shift
+    by any other row (or a row shifted before the others read it) breaks
+    the zero at R_max or the differences."""
+    wl = W.diatom_atom(n_r=40)
+    b = wl.basis
+    orc = O.oracle_for_workload(wl)
+    M, D, S = orc.matrix_make(b.ind_mat, wl.conv_unit_e, wl.mij_zero_cut, *wl.ir_cut)
+    Ms = O.mij_shift(M)
+    assert np.all(Ms[:, -1] == 0.0)
+    assert np.array_equal(Ms[:, :-1], M[:, :-1] - M[:, -1:])
+    assert not np.array_equal(Ms, M)
+    Mr, Dr = O.read_mij_post(wl.R, M, shift=True)
+    assert np.array_equal(Mr, Ms)
+    M0, D0 = O.read_mij_post(wl.R, M, shift=False)
+    assert np.array_equal(M0, M)
+    # without the shift the read branch re-splines to the compute branch's y''
+    live = S == 0
+    assert np.array_equal(D0[live], D[live])
+    scale = np.abs(D0).max(axis=1, keepdims=True) + 1e-300
+    assert np.max(np.abs(Dr - D0) / scale) < 1e-8
+    for k in np.nonzero(live)[0][:12]:
+        for x in (3.3, 5.05, 9.9, 17.2, wl.R[-1]):
+            v0, d0 = O.splint(wl.R, M[k], D0[k], x)
+            v1, d1 = O.splint(wl.R, Mr[k], Dr[k], x)
+            big = max(abs(v0), abs(M[k, -1]), 1e-30)
+            assert abs((v1 + M[k, -1]) - v0) <= 1e-12 * big
+            assert abs(d1 - d0) <= 1e-8 * (abs(d0) + np.abs(M[k]).max())
+        assert O.splint(wl.R, Mr[k], Dr[k], wl.R[-1])[0] == 0.0
