@@ -106,7 +106,12 @@ def main():
     ap.add_argument("--sur-batch", type=int, default=8192)
     ap.add_argument("--sur-rmin", type=float, default=5.0,
                     help="the surrogate covers R >= this (bohr); the plain PES fills the grid below")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain device copy of known size (4 launches, each reading and writing "
+                         "256 MiB) for scaling rocprofv3's FETCH_SIZE / WRITE_SIZE counters, then exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        return copy_calibration()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(trajectories would shard across GPUs with replicated splines; not built).", flush=True)
@@ -345,6 +350,21 @@ def main():
     print(json.dumps(out), flush=True)
 
 
+def copy_calibration():
+    """A plain elementwise device copy (256 MiB read + 256 MiB written per
+    launch, 4 launches): the counter reading per launch divided into
+    262144 KiB is the calibration factor of FETCH_SIZE / WRITE_SIZE."""
+    import torch
+    n = (256 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    b = torch.empty_like(a)
+    for _ in range(4):
+        torch.mul(a, 1.0, out=b)
+    torch.cuda.synchronize()
+    print("copy calibration: 4 launches of %d bytes read + %d bytes written" % (4 * n, 4 * n), flush=True)
+    return 0
+
+
 def elem_err(M, M64):
     """Matrix-element error against the FP64 build, relative to the largest
     |M| at the same R (elements that cancel to almost nothing carry only
@@ -514,4 +534,4 @@ def baseline_sample(n_rhs, n_sample):
 
 
 if __name__ == "__main__":
-    main()
+    sys.exit(main())

@@ -126,6 +126,10 @@ typedef struct {
   const double *at_t;          /* [at_n] */
   const double *at_var;        /* [7][at_n]: R,P_R,Theta,P_Theta,Phi,P_Phi,V */
   double *at_var_der;          /* [7][at_n]  y'' of the path splines */
+  /* planted mistakes for the tests that show a check fails (0 = none):
+   * bit 0 flips the mean-field force on P_R, bit 1 the first half of the
+   * P_Phi Coriolis accumulator, bit 2 the quantum Coriolis coupling */
+  int plant;
 } or_dyn_ctx;
 
 void or_resize(or_dyn_ctx *d);

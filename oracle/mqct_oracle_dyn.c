@@ -135,6 +135,7 @@ static void pair_loop(const or_dyn_ctx *d, double t, const double *q,
 static void coriolis_diag(const or_dyn_ctx *d, const double *q, double *dqdt,
                           double dPhidT) {
   int N = d->n_states;
+  if (d->plant & 4) dPhidT = -dPhidT;
   for (int i = 0; i < N; i++) {
     double j_12 = d->j12[i], m_12 = d->m12[i];
     double m1 = m_12 + 1.0, m0 = m_12 - 1.0;
@@ -182,7 +183,7 @@ void or_derivs_bk(const or_dyn_ctx *d, double t, const double *q,
     /* mean-field force on P_R: mqct_eqtns.f:263-266 */
     dqdt[2 * N + 1] -= ((q[i] * q[j] + q[i + N] * q[j + N]) * rphase -
                         (q[i + N] * q[j] - q[i] * q[j + N]) * iphase) *
-                       d_f * bk_der[k];
+                       d_f * bk_der[k] * ((d->plant & 1) ? -1.0 : 1.0);
     /* Coriolis accumulators: mqct_eqtns.f:270-359 */
     double j_12 = d->j12[j], m_12 = d->m12[i];
     double m1 = m_12 + 1.0, m0 = m_12 - 1.0;
@@ -191,12 +192,13 @@ void or_derivs_bk(const or_dyn_ctx *d, double t, const double *q,
     else { s1 = PAD1; s11 = PAD2; }
     if ((int)fabs(m0) <= (int)j_12) { s0 = j - 1; s00 = s0 + N; }
     else { s0 = PAD1; s00 = PAD2; }
-    amp_phi[k] += -sin_tet * rf / 2.0 *
-                      (misc[0] * (q[i] * q[s1] + q[i + N] * q[s11]) +
-                       misc[1] * (q[i] * q[s0] + q[i + N] * q[s00])) -
-                  sin_tet * (-jf) / 2.0 *
-                      (misc[0] * (q[i + N] * q[s1] - q[i] * q[s11]) +
-                       misc[1] * (q[i + N] * q[s0] - q[i] * q[s00]));
+    amp_phi[k] += (-sin_tet * rf / 2.0 *
+                       (misc[0] * (q[i] * q[s1] + q[i + N] * q[s11]) +
+                        misc[1] * (q[i] * q[s0] + q[i + N] * q[s00])) -
+                   sin_tet * (-jf) / 2.0 *
+                       (misc[0] * (q[i + N] * q[s1] - q[i] * q[s11]) +
+                        misc[1] * (q[i + N] * q[s0] - q[i] * q[s00]))) *
+                  ((d->plant & 2) ? -1.0 : 1.0);
     if (d->theta_on)
       amp_theta[k] += d_f * rphase / 2.0 *
                           (misc[0] * (q[i] * q[s1] + q[i + N] * q[s11]) -

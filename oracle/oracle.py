@@ -55,6 +55,7 @@ class DynCtx(ctypes.Structure):
         ("mass", _d), ("theta_on", _i),
         ("misc", _dp), ("bk_indx", _ip), ("delta_E", _dp), ("n_dE", _i),
         ("at_n", _i), ("at_t", _dp), ("at_var", _dp), ("at_var_der", _dp),
+        ("plant", _i),
     ]
 
 
@@ -433,6 +434,12 @@ class DynOracle:
         d.at_n = len(rec)
         d.at_t, d.at_var, d.at_var_der = dp(self.at_t), dp(self.at_var), dp(self.at_der)
         lib().or_at_prepare(ctypes.byref(d))
+
+    def potential(self, R):
+        """or_potential_data_bk: M_k(R) and dM_k/dR of every pair."""
+        m, dm = np.zeros(self.d.total_size), np.zeros(self.d.total_size)
+        lib().or_potential_data_bk(ctypes.byref(self.d), R, dp(m), dp(dm))
+        return m, dm
 
     def derivs(self, t, q, adia=False):
         q = f64(q)

@@ -1,0 +1,196 @@
+"""Pins of the oracle's CC-MQCT right-hand side (or_derivs_bk, restating
+DERIVS_BK, mqct_eqtns.f:112-499) by the conservation laws of the mixed
+quantum/classical equations, on a system with P_Phi != 0 and an
+R-dependent coupling (diatom + atom, j <= 2: 9 states, the Legendre
+potential of the benchmark, MIJ_SHIFT applied as in the default run).
+
+The quantum part evolves under H_q = H_0 + V(R) + dPhi/dt J_x, where J_x
+couples m and m +- 1 of one channel with <j m+-1|J_x|j m> =
+sqrt(j(j+1) - m(m+-1))/2 (the term the RHS applies, mqct_eqtns.f:434-465);
+the classical part feels the mean-field force -d<V>/dR plus the centrifugal
+terms, and P_Phi the torque of the Coriolis accumulators
+(mqct_eqtns.f:263-370).  Then
+
+  E = P_R^2/2mu + P_Theta^2/(2 mu R^2) + P_Phi^2/(2 mu R^2 sin^2 Theta)
+      + sum |a_n|^2 E_n + <psi|V(R)|psi>
+
+-- the reference's own energy-conservation check "eror"
+(propagator.f:4342-4357, with pot as at propagator.f:3426-3448) -- is
+conserved exactly when dP_Phi/dt = d<J_x>/dt, i.e. when the total angular
+momentum P_Phi - <J_x> is conserved too.  Both are computed here from
+textbook expressions (psi_n = a_n e^{-i E_n t}, V from the pair list, J_x
+from j and m), not from the oracle's code.  The integration error of RK4
+shrinks 16x per halving of the step, so a drift that shrinks (at least 4x
+here, with rounding at the 1e-13 Hartree level) is the integrator's, and
+the quantity is conserved by the equations themselves.
+
+Each sign the laws fix is flipped in turn in the oracle (its test-only
+plant flags): the mean-field force (breaks E), one half of the P_Phi
+accumulator (breaks both), the quantum Coriolis coupling (breaks E).  Each
+flip moves the conserved quantity by orders of magnitude more than the
+bar.
+"""
+import numpy as np
+import pytest
+
+import oracle.oracle as O
+from mqct_amd import _lib as L
+from mqct_amd import workloads as W
+from mqct_amd.basis import AMUTOAU, CM_TO_HARTREE
+
+U_CM = 500.0
+
+
+@pytest.fixture(scope="module")
+def system():
+    wl = W.diatom_atom(jmax=2, n_beta=24, n_r=120)
+    b = wl.basis
+    orc = O.oracle_for_workload(wl)
+    M, D, S = orc.matrix_make(b.ind_mat, wl.conv_unit_e, 0.0, *wl.ir_cut)
+    # MIJ_SHIFT (the default run): M(R_max) = 0, so the coupling is
+    # continuous where splint_bk returns zero beyond the grid
+    # (mqct_eqtns.f:545-549); unshifted, the mean-field energy would jump by
+    # <psi|V(R_max)|psi> ~ C/R_max^6 ~ 1e-5 U on leaving the grid
+    M = O.mij_shift(M)
+    Ej = b.Ej * CM_TO_HARTREE
+    mu = wl.mass_amu * AMUTOAU
+    # J_x between states of one channel (type 1: channel = j), from j and m
+    N = b.n_states
+    Jx = np.zeros((N, N))
+    for p in range(N):
+        for q in range(N):
+            if b.indx_chann[p] == b.indx_chann[q] and abs(b.m12[p] - b.m12[q]) == 1:
+                j, m = b.j12[q], b.m12[q]
+                Jx[p, q] = np.sqrt(j * (j + 1.0) - m * b.m12[p]) / 2.0
+    assert np.allclose(Jx, Jx.T)
+    return wl, M, D, Ej, mu, Jx
+
+
+def invariants(dyn, b, Ej, mu, Jx, t, q):
+    N = b.n_states
+    a = q[:N] + 1j * q[N:2 * N]
+    R, PR, th, Pth, PPhi = q[2 * N], q[2 * N + 1], q[2 * N + 2], q[2 * N + 3], q[2 * N + 5]
+    m, _ = dyn.potential(R)
+    V = np.zeros((N, N))
+    i, j = b.ind_mat[:, 0] - 1, b.ind_mat[:, 1] - 1
+    V[i, j] = m
+    V[j, i] = m
+    psi = a * np.exp(-1j * Ej * t)
+    kin = PR ** 2 / 2 / mu + Pth ** 2 / 2 / mu / R ** 2 + PPhi ** 2 / 2 / mu / R ** 2 / np.sin(th) ** 2
+    E = kin + np.sum(np.abs(a) ** 2 * Ej) + np.real(np.conj(psi) @ V @ psi)
+    Jtot = PPhi - np.real(np.conj(a) @ Jx @ a)
+    return E, Jtot, R
+
+
+def run(system, s_st, l_real, dt, plant=0):
+    """RK4 from R = R_max inward and out again with the oracle's RHS;
+    returns the largest drift of E (Hartree) and of P_Phi - <J_x> along the
+    path inside the R grid, and the deepest R reached."""
+    wl, M, D, Ej, mu, Jx = system
+    b = wl.basis
+    dyn = O.DynOracle(b.j12, b.m12, Ej, b.ind_mat, wl.R, M, D, mu)
+    dyn.d.plant = plant
+    N = b.n_states
+    q = np.zeros(2 * N + 8)
+    q[s_st - 1] = 1.0
+    k = np.sqrt(2 * mu * U_CM * CM_TO_HARTREE)
+    lo = np.sqrt(l_real * (l_real + 1.0))
+    R0 = wl.R[-1]
+    q[2 * N], q[2 * N + 1], q[2 * N + 2], q[2 * N + 5] = R0, -np.sqrt(k * k - lo * lo / R0 ** 2), np.pi / 2, lo
+    E0, J0, _ = invariants(dyn, b, Ej, mu, Jx, 0.0, q)
+    dE = dJ = 0.0
+    Rmin = R0
+    t = 0.0
+    while True:
+        k1 = dyn.derivs(t, q)
+        k2 = dyn.derivs(t + dt / 2, q + dt / 2 * k1)
+        k3 = dyn.derivs(t + dt / 2, q + dt / 2 * k2)
+        k4 = dyn.derivs(t + dt, q + dt * k3)
+        q = q + dt / 6 * (k1 + 2 * k2 + 2 * k3 + k4)
+        t += dt
+        E, J, R = invariants(dyn, b, Ej, mu, Jx, t, q)
+        if R > R0 or t > 2e5:
+            # the step that leaves the grid straddles the kink of the
+            # force at R_max (zero beyond it): not a property of the RHS
+            break
+        dE, dJ, Rmin = max(dE, abs(E - E0)), max(dJ, abs(J - J0)), min(Rmin, R)
+    return dE, dJ, Rmin, E0, J0
+
+
+@pytest.mark.parametrize("s_st", [1, 3])
+def test_cc_rhs_conserves_energy_and_angular_momentum(system, s_st):
+    U = U_CM * CM_TO_HARTREE
+    dE, dJ, Rmin, E0, J0 = run(system, s_st, 20.0, 8.0)
+    dE2, dJ2, _, _, _ = run(system, s_st, 20.0, 4.0)
+    print(f"s_st {s_st}: R_min {Rmin:.2f}, dE/U {dE / U:.2e} -> {dE2 / U:.2e}, dJ {dJ:.2e} -> {dJ2:.2e}")
+    assert Rmin < 5.0  # the trajectory reaches the strongly coupled region
+    # conserved to RK4 accuracy, shrinking with the step as RK4's error does
+    assert dE2 / U < 1e-8 and dJ2 < 1e-7
+    # RK4's error drops 16x per halving; an error of the equations would
+    # not drop at all, a first-order one 2x
+    assert dE2 <= dE / 4 or dE / U < 1e-11
+    assert dJ2 <= dJ / 4 or dJ < 1e-10
+
+
+@pytest.mark.parametrize("plant,breaks_j", [(1, False), (2, True), (4, False)])
+def test_planted_sign_flips_break_conservation(system, plant, breaks_j):
+    U = U_CM * CM_TO_HARTREE
+    dE, dJ, _, _, _ = run(system, 3, 20.0, 8.0, plant=plant)
+    print(f"plant {plant}: dE/U {dE / U:.2e}, dJ {dJ:.2e}")
+    assert dE / U > 1e-4
+    if breaks_j:
+        assert dJ > 1e-3
+
+
+def test_oracle_dotraject_energy_error(system):
+    """The oracle's DOTRAJECT loop (CC RK4, TIME_STEP 2 a.u. scaled by
+    sqrt(U/300)) keeps the reference's eror, |E_final - E_initial|, within
+    1e-8 of the collision energy for a Coriolis-coupled trajectory."""
+    wl, M, D, Ej, mu, Jx = system
+    b = wl.basis
+    dyn = O.DynOracle(b.j12, b.m12, Ej, b.ind_mat, wl.R, M, D, mu)
+    U = U_CM * CM_TO_HARTREE
+    dt = 2.0 / np.sqrt(U_CM / 300.0)
+    qf, st, _ = dyn.dotraject(L.MQCT_PROP_CC_RK4, 3, 20.0, U, dt, R_start=wl.R[-1])
+    q0 = np.zeros_like(qf)
+    N = b.n_states
+    q0[2] = 1.0
+    lo = np.sqrt(20.0 * 21.0)
+    q0[2 * N], q0[2 * N + 2], q0[2 * N + 5] = wl.R[-1], np.pi / 2, lo
+    q0[2 * N + 1] = -np.sqrt(2 * mu * U - lo * lo / wl.R[-1] ** 2)
+    E0, J0, _ = invariants(dyn, b, Ej, mu, Jx, 0.0, q0)
+    E1, J1, R1 = invariants(dyn, b, Ej, mu, Jx, st["t_end"], qf)
+    assert R1 > wl.R[-1] - 1e-9
+    assert abs(E1 - E0) / U < 1e-8
+    assert abs(J1 - J0) < 1e-7
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", [L.MQCT_PROP_CC_RK4, L.MQCT_PROP_CC_ODEINT])
+def test_gpu_cc_trajectories_keep_energy_and_angular_momentum(dev, system, mode):
+    """The GPU's CC-MQCT kernel (DERIVS_BK with rk4 or odeint) on the same
+    system: the reference's eror and the total angular momentum between the
+    start and the end of Coriolis-coupled trajectories."""
+    from mqct_amd.propagate import Propagator
+    wl, M, D, Ej, mu, Jx = system
+    b = wl.basis
+    prop = Propagator(dev, b, wl.R, M, D, mass_amu=wl.mass_amu)
+    dyn = O.DynOracle(b.j12, b.m12, Ej, b.ind_mat, wl.R, M, D, mu)
+    U = U_CM * CM_TO_HARTREE
+    rows = []
+    for s_st in (1, 3, 6):
+        rows += prop.trajectories([20.0], U_CM, s_st=s_st, time_step=2.0, eps=1e-10, hmin=1e-3)
+    q, st = prop.run(mode, rows)
+    N = b.n_states
+    for i, r in enumerate(rows):
+        q0 = np.zeros(2 * N + 8)
+        q0[r["s_st"] - 1] = 1.0
+        lo = np.sqrt(20.0 * 21.0)
+        q0[2 * N], q0[2 * N + 2], q0[2 * N + 5] = wl.R[-1], np.pi / 2, lo
+        q0[2 * N + 1] = -np.sqrt(2 * mu * U - lo * lo / wl.R[-1] ** 2)
+        E0, J0, _ = invariants(dyn, b, Ej, mu, Jx, 0.0, q0)
+        E1, J1, R1 = invariants(dyn, b, Ej, mu, Jx, st["t_end"][i], q[i])
+        print(f"mode {mode} s_st {r['s_st']}: eror/U {abs(E1 - E0) / U:.2e}, dJ {abs(J1 - J0):.2e}")
+        assert R1 > wl.R[-1] - 1e-9
+        assert abs(E1 - E0) / U < 1e-8
+        assert abs(J1 - J0) < 1e-7
