@@ -1,0 +1,109 @@
+"""Closed forms the parity tests pin the matrix elements with (test helpers,
+not tests).
+
+Asymmetric top + atom (SYS_TYPE 4).  The reference's wavefunction is
+psi_{J M tau}(alpha, beta, gamma) = N_J sum_K c_K Djkm(J, K, M), where
+Djkm(j, k, m) = d^j_{m k}(beta) exp(i (k gamma + m alpha))
+(pot_comp.f:256-309, basis_wave_functions.f:83-116), i.e. the complex
+conjugate of the standard D^J_{M K}(alpha, beta, gamma) =
+exp(-i M alpha) d^J_{M K}(beta) exp(-i K gamma), with N_J =
+sqrt((2J+1)/8 pi^2).  For a potential that is a sum of
+D^l_{0 q}(alpha, beta, gamma) = d^l_{0 q}(beta) exp(-i q gamma) (a function
+of beta and gamma only), the integral over the three Euler angles is
+
+  int psi_1 psi_2^* D^l_{0q} dOmega
+     = sqrt((2J1+1)(2J2+1)) sum_{K1,K2} c1_K1 c2_K2 (-1)^(M-K1)
+       (J1 J2 l; -M M 0) (J1 J2 l; -K1 K2 q)
+
+(D^{J*}_{M K} = (-1)^(M-K) D^J_{-M -K}, and the integral of three D
+functions is 8 pi^2 times two 3j symbols).  The reference integrates beta
+and gamma on the grid and alpha analytically (2 pi), and takes the real
+part (REF_XZ=YES) or real + imaginary part (REF_XZ=NO, pot_comp.f:2544-2553)
+of that integral.  The integrand has trigonometric terms in gamma of order
+<= J1 + J2 + l, which an equidistant gamma grid of m points integrates
+exactly when J1 + J2 + l < m; in beta the reference's Gauss-Legendre nodes
+lie in beta itself (sin(beta) is a factor of the integrand), so that sum
+converges exponentially with the number of nodes rather than being exact.
+"""
+import functools
+import math
+from fractions import Fraction
+
+import numpy as np
+
+
+@functools.lru_cache(maxsize=None)
+def exact_3j(j1, j2, j3, m1, m2, m3):
+    """Wigner 3j symbol by the Racah sum in exact rationals (integer j),
+    rounded once at the end."""
+    if m1 + m2 + m3 != 0 or not (abs(j1 - j2) <= j3 <= j1 + j2):
+        return 0.0
+    if abs(m1) > j1 or abs(m2) > j2 or abs(m3) > j3:
+        return 0.0
+    f = math.factorial
+    tri = Fraction(f(j1 + j2 - j3) * f(j1 - j2 + j3) * f(-j1 + j2 + j3), f(j1 + j2 + j3 + 1))
+    pre = tri * f(j1 + m1) * f(j1 - m1) * f(j2 + m2) * f(j2 - m2) * f(j3 + m3) * f(j3 - m3)
+    s = Fraction(0)
+    for k in range(0, j1 + j2 + j3 + 1):
+        a = [k, j1 + j2 - j3 - k, j1 - m1 - k, j2 + m2 - k, j3 - j2 + m1 + k, j3 - j1 - m2 + k]
+        if min(a) < 0:
+            continue
+        den = 1
+        for x in a:
+            den *= f(x)
+        s += Fraction((-1) ** k, den)
+    if s == 0:
+        return 0.0
+    sign = (-1) ** (j1 - j2 - m3) * (1 if s > 0 else -1)
+    return sign * math.sqrt(s * s * pre)
+
+
+def wigner_d(j, mp, m, b):
+    """d^j_{m' m}(b) by the explicit Wigner sum (standard convention)."""
+    f = math.factorial
+    s = 0.0
+    for k in range(0, 2 * j + 1):
+        a = [j + m - k, k, mp - m + k, j - mp - k]
+        if min(a) < 0:
+            continue
+        s += ((-1) ** (mp - m + k) * math.sqrt(f(j + mp) * f(j - mp) * f(j + m) * f(j - m)) /
+              (f(a[0]) * f(a[1]) * f(a[2]) * f(a[3])) * math.cos(b / 2) ** (2 * j + m - mp - 2 * k) *
+              math.sin(b / 2) ** (mp - m + 2 * k))
+    return s
+
+
+def d_potential_coeffs(rng, lmax):
+    """Random complex weights w[l][q] (q = -l..l) of
+    V = Re sum_{l,q} w_lq D^l_{0q}(beta, gamma)."""
+    return [rng.standard_normal(2 * l + 1) + 1j * rng.standard_normal(2 * l + 1) for l in range(lmax + 1)]
+
+
+def d_potential_grid(w, beta, gamma):
+    """V[beta][gamma] of the potential with weights w."""
+    V = np.zeros((len(beta), len(gamma)), dtype=complex)
+    for l, wl in enumerate(w):
+        for iq, q in enumerate(range(-l, l + 1)):
+            db = np.array([wigner_d(l, 0, q, b) for b in beta])
+            V += wl[iq] * db[:, None] * np.exp(-1j * q * np.asarray(gamma))[None, :]
+    return V.real
+
+
+def type4_element(w, J1, c1, J2, c2, M, ref_xz=1):
+    """int psi_1 psi_2^* V over alpha, beta, gamma for V = Re sum w_lq
+    D^l_{0q}; c1, c2: coefficients c_K, K = -J..J.  Returns Re(I)
+    (REF_XZ=YES) or Re(I) + Im(I) (REF_XZ=NO)."""
+    def T(l, q):
+        s = 0.0
+        for i1, K1 in enumerate(range(-J1, J1 + 1)):
+            K2 = K1 - q
+            if abs(K2) > J2 or c1[i1] == 0.0:
+                continue
+            s += c1[i1] * c2[K2 + J2] * (-1) ** (M - K1) * exact_3j(J1, J2, l, -K1, K2, q)
+        return s * math.sqrt((2 * J1 + 1) * (2 * J2 + 1)) * exact_3j(J1, J2, l, -M, M, 0)
+
+    I = 0.0 + 0.0j
+    for l, wl in enumerate(w):
+        for iq, q in enumerate(range(-l, l + 1)):
+            # Re(w D) = (w D + w^* D^*)/2 and D^{l*}_{0q} = (-1)^q D^l_{0,-q}
+            I += 0.5 * (wl[iq] * T(l, q) + np.conj(wl[iq]) * (-1) ** q * T(l, -q))
+    return I.real if ref_xz else I.real + I.imag

@@ -6,9 +6,9 @@ cancels: an element whose terms sum in absolute value to A may differ by
 1e-13*A (a few hundred roundings of the largest term -- any summation order
 does that), so the test is |gpu - oracle| <= max(1e-10 |oracle|, 1e-13 A).
 State tables, pair list and the cutoff mask are bit-exact.  Mixed-precision
-forms are held to their worst-case rounding bound against A: FP32 4e-6 A,
-split-BF16x3 6e-5 A; per-element errors are also reported relative to the
-element and to its largest value over R.
+forms are held to their worst-case rounding bound against A: 4e-6 A for
+both FP32 and split-BF16x3; per-element errors are also reported relative
+to the element and to its largest value over R.
 """
 import numpy as np
 import pytest
@@ -92,6 +92,40 @@ def test_asym_top_atom_refxz_no(dev):
     (M, D, S), (Mo, Do, So), orc = build_both(dev, wl)
     assert np.array_equal(S, So)
     assert fp64_ok(M, Mo, abs_sums(orc, wl)) <= 1.0
+
+
+@pytest.mark.parametrize("ref_xz", [1, 0])
+def test_asym_top_atom_fp64_closed_form(dev, ref_xz):
+    """The GPU's FP64 build (host V grid through mqct_hip_set_vgrid) of
+    K-mixed asymmetric-top states (j <= 4) in a potential built from
+    D^l_{0q} (l <= 4) against the 3j closed form of tests/closed_forms.py,
+    in the XZ frame and with REF_XZ=NO (pot_comp.f:2544-2553): 1e-11 of the
+    largest element (the 28 beta points converge to ~1e-14)."""
+    import closed_forms as CF
+    from mqct_amd import basis as B
+    from mqct_amd.workloads import Workload
+    A_, B_, C_ = W.ROT_A, W.ROT_B, W.ROT_C
+    b = B.ini_arrays(B.channels_type4(0, 4, A_, B_, C_))
+    g = B.grid_type4(28, 24, ref_xz=ref_xz)
+    beta = g.xbm + g.xbl * np.asarray(g.xb)
+    gamma = np.asarray(g.xg)
+    rng = np.random.default_rng(43)
+    R = np.array([4.0, 5.0, 6.0, 8.0])
+    ws = [CF.d_potential_coeffs(rng, 4) for _ in R]
+    V = np.stack([CF.d_potential_grid(w, beta, gamma) for w in ws])
+    wl = Workload("d_potential", b, g, R, 1, np.zeros(1), 3.27, conv_unit_e=1.0, mij_cutoff_cm=0.0)
+    M, D, S = MatrixBuilder(dev, wl, vgrid=V.ravel()).build("fp64")
+    ref = np.zeros_like(M)
+    for k in range(b.total_size):
+        s1, s2 = b.ind_mat[k] - 1
+        c1, c2 = b.indx_chann[s1] - 1, b.indx_chann[s2] - 1
+        J1, J2, Mq = int(b.j1_ch[c1]), int(b.j1_ch[c2]), int(b.m12[s1])
+        for ir in range(len(R)):
+            ref[k, ir] = CF.type4_element(ws[ir], J1, b.eig1[c1, :2 * J1 + 1], J2, b.eig1[c2, :2 * J2 + 1], Mq,
+                                          ref_xz=ref_xz)
+    err = np.max(np.abs(M - ref)) / np.abs(ref).max()
+    print(f"ref_xz {ref_xz}: {b.total_size} pairs x {len(R)} R, max err / max |M| {err:.2e}")
+    assert err <= 1e-11
 
 
 def test_two_asym_tops_fp64_small(dev):

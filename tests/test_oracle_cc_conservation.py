@@ -179,8 +179,9 @@ def test_gpu_cc_trajectories_keep_energy_and_angular_momentum(dev, system, mode)
     U = U_CM * CM_TO_HARTREE
     rows = []
     for s_st in (1, 3, 6):
-        rows += prop.trajectories([20.0], U_CM, s_st=s_st, time_step=2.0, eps=1e-10, hmin=1e-3)
+        rows += prop.trajectories([20.0], U_CM, s_st=s_st, time_step=2.0, eps=1e-8, hmin=1e-3)
     q, st = prop.run(mode, rows)
+    assert not st["fail"].any()
     N = b.n_states
     for i, r in enumerate(rows):
         q0 = np.zeros(2 * N + 8)

@@ -324,3 +324,44 @@ def test_mij_shift_of_both_branches():
             assert abs((v1 + M[k, -1]) - v0) <= 1e-12 * big
             assert abs(d1 - d0) <= 1e-8 * (abs(d0) + np.abs(M[k]).max())
         assert O.splint(wl.R, Mr[k], Dr[k], wl.R[-1])[0] == 0.0
+
+
+@pytest.mark.parametrize("ref_xz", [1, 0])
+def test_type4_elements_closed_form(ref_xz):
+    """Type-4 elements of K-mixed asymmetric-top states (j <= 3, the
+    benchmark's rotor, eigenvectors with K != 0 components) for a potential
+    built from D^l_{0q} (l <= 4, random complex weights, two R points):
+    the oracle's INTEGRATOR_TEMP (pot_comp.f:2503-2557) against the 3j
+    closed form of tests/closed_forms.py, in the XZ frame (real part) and
+    with REF_XZ=NO (real + imaginary part, pot_comp.f:2544-2553).  The
+    12 x 24 grid integrates these degrees exactly.  A sign slip in the
+    REF_XZ=NO integrand, a transposed pair (psi_1^* psi_2) or a wrong K
+    phase fails at the first element with Im != 0."""
+    import closed_forms as CF
+    A, Bc, Cc = 27.9, 14.5, 9.3
+    b = B.ini_arrays(B.channels_type4(0, 3, A, Bc, Cc))
+    nb, ng = 28, 24
+    g = B.grid_type4(nb, ng, ref_xz=ref_xz)
+    beta = g.xbm + g.xbl * np.asarray(g.xb)
+    gamma = np.asarray(g.xg)
+    rng = np.random.default_rng(41)
+    ws = [CF.d_potential_coeffs(rng, 4) for _ in range(2)]
+    V = np.stack([CF.d_potential_grid(w, beta, gamma) for w in ws])  # [R][beta][gamma]
+    R = np.array([5.0, 7.0])
+    o = O.MatrixOracle(4, b.j12, b.m12, b.indx_chann, b.j1_ch, O.grid_dict(g), R, V.ravel(), eig1=b.eig1)
+    n_mixed = n_imag = 0
+    for k in range(b.total_size):
+        s1, s2 = b.ind_mat[k] - 1
+        c1, c2 = b.indx_chann[s1] - 1, b.indx_chann[s2] - 1
+        J1, J2, M = int(b.j1_ch[c1]), int(b.j1_ch[c2]), int(b.m12[s1])
+        e1, e2 = b.eig1[c1, :2 * J1 + 1], b.eig1[c2, :2 * J2 + 1]
+        n_mixed += int(np.count_nonzero(np.abs(e1) > 1e-12) > 1)
+        for ir in range(2):
+            ref = CF.type4_element(ws[ir], J1, e1, J2, e2, M, ref_xz=ref_xz)
+            if ref_xz == 0:
+                n_imag += int(abs(ref - CF.type4_element(ws[ir], J1, e1, J2, e2, M, ref_xz=1)) > 1e-6)
+            got = o.element(b.ind_mat, k, ir)
+            assert abs(got - ref) <= 1e-12 * max(1.0, abs(ref)), (k, ir, got, ref)
+    assert n_mixed > 100
+    if ref_xz == 0:
+        assert n_imag > 100  # the imaginary part is present and pinned
