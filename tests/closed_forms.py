@@ -107,3 +107,30 @@ def type4_element(w, J1, c1, J2, c2, M, ref_xz=1):
             # Re(w D) = (w D + w^* D^*)/2 and D^{l*}_{0q} = (-1)^q D^l_{0,-q}
             I += 0.5 * (wl[iq] * T(l, q) + np.conj(wl[iq]) * (-1) ** q * T(l, -q))
     return I.real if ref_xz else I.real + I.imag
+
+
+def h2ohe_reference_system():
+    """The reference's SYS_TYPE 4 example EXAMPLES_IO/SYS_TYPE_4/
+    H2O_He_external_rot_wfn as data: its 9 channels with the explicit
+    eigenvectors of USER_DEFINED_BASIS.DAT (LEVELS_FILE=YES), the 10 x 20
+    beta x gamma grid and R = 6, 8 bohr of its deck, the V grid of its own
+    PES (compiled from source; tests/golden/make_h2ohe_vgrid.py) and the
+    Mij of its MTRX.DAT (Hartree, 12 significant digits).  Returns
+    (basis, grid, R, V[R][beta][gamma] in cm^-1, mij[k][iR])."""
+    import json
+    import os
+    from mqct_amd import basis as B
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    fx = json.load(open(os.path.join(here, "fixtures.json")))["h2o_he"]
+    vg = json.load(open(os.path.join(here, "h2ohe_vgrid.json")))
+    ch = np.array(fx["channels"])
+    j1 = ch[:, 0].astype(np.int32)
+    eig = np.zeros((len(ch), 2 * int(j1.max()) + 1))
+    for c, v in enumerate(fx["eigvecs"]):
+        eig[c, :len(v)] = v
+    b = B.ini_arrays(B.Basis(4, j1, ch[:, 3], ch[:, 1].astype(np.int32), ch[:, 2].astype(np.int32), eig1=eig))
+    st = np.array(fx["states"])
+    assert np.array_equal(b.j12, st[:, 2]) and np.array_equal(b.m12, st[:, 3])
+    assert np.array_equal(b.ind_mat, np.array(fx["pairs"])[:, 1:3])
+    g = B.grid_type4(vg["grid"]["n_beta"], vg["grid"]["n_gamma"])
+    return b, g, np.array(vg["R"]), np.array(vg["V"]), np.array(vg["mij"])

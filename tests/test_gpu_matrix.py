@@ -128,6 +128,27 @@ def test_asym_top_atom_fp64_closed_form(dev, ref_xz):
     assert err <= 1e-11
 
 
+def test_asym_top_atom_matches_reference_mtrx_dat(dev):
+    """The GPU's FP64 build of the reference's own SYS_TYPE 4 example
+    (H2O + He, explicit K-mixed eigenvectors, the V grid of the example's
+    own PES) against the example's MTRX.DAT: within 1e-12 of the largest
+    element (12 printed digits), 1e-11 relative where significant, and the
+    cutoff mask equal to the file's zero elements."""
+    import closed_forms as CF
+    from mqct_amd import basis as B
+    from mqct_amd.workloads import Workload
+    b, g, R, V, ref = CF.h2ohe_reference_system()
+    wl = Workload("h2o_he", b, g, R, 1, np.zeros(1), 3.2748)
+    M, D, S = MatrixBuilder(dev, wl, vgrid=V.ravel()).build("fp64")
+    scale = np.abs(ref).max()
+    err = np.max(np.abs(M - ref)) / scale
+    print(f"H2O-He MTRX.DAT: max |gpu - file| / max |M| {err:.2e}")
+    assert np.array_equal(S.astype(bool), np.all(ref == 0.0, axis=1))
+    assert err <= 1e-12
+    big = np.abs(ref) > 1e-3 * scale
+    assert np.max(np.abs(M[big] - ref[big]) / np.abs(ref[big])) <= 1e-11
+
+
 def test_two_asym_tops_fp64_small(dev):
     wl = W.two_asym_tops(jmax1=1, jmax2=1, n_r=8, n_ang_each=(2, 3, 3, 2, 3))
     (M, D, S), (Mo, Do, So), orc = build_both(dev, wl)

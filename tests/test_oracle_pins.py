@@ -365,3 +365,23 @@ def test_type4_elements_closed_form(ref_xz):
     assert n_mixed > 100
     if ref_xz == 0:
         assert n_imag > 100  # the imaginary part is present and pinned
+
+
+def test_type4_matches_reference_mtrx_dat():
+    """The reference's own SYS_TYPE 4 run (EXAMPLES_IO/SYS_TYPE_4/
+    H2O_He_external_rot_wfn/MTRX.DAT: 147 elements x 2 R, direct quadrature
+    with K-mixed eigenvectors given in USER_DEFINED_BASIS.DAT) reproduced by
+    the oracle from the example's own PES on the example's grid: every
+    element within 1e-12 of the largest (the file prints 12 digits), the
+    significant ones within 1e-11 relative, and the MIJ_CUTOFF mask equal to
+    the file's zero elements."""
+    import closed_forms as CF
+    b, g, R, V, ref = CF.h2ohe_reference_system()
+    o = O.MatrixOracle(4, b.j12, b.m12, b.indx_chann, b.j1_ch, O.grid_dict(g), R, V.ravel(), eig1=b.eig1)
+    M, D, S = o.matrix_make(b.ind_mat, B.CM_TO_HARTREE, 1e-12 * B.CM_TO_HARTREE, 1, 2)
+    scale = np.abs(ref).max()
+    assert np.array_equal(S.astype(bool), np.all(ref == 0.0, axis=1))
+    assert np.max(np.abs(M - ref)) <= 1e-12 * scale
+    big = np.abs(ref) > 1e-3 * scale
+    assert big.sum() > 100
+    assert np.max(np.abs(M[big] - ref[big]) / np.abs(ref[big])) <= 1e-11
