@@ -285,6 +285,26 @@ int mqct_hip_propagate_record(mqct_hip_t *h, int n_traj,
                               double *records, int *rec_n, double *q_out,
                               mqct_traj_stat_t *stat_out);
 
+/* Cross sections from final amplitudes: replaces the per-channel
+ * probabilities of DOTRAJECT (propagator.f:4364-4383) and the sums of
+ * INELAST_CALC (propagator.f:4752-4983) for a batch, on the GPU.
+ *   P_n(t)      = sum over the states of channel n of |a_{t,s}|^2
+ *   sigma[g][n] = sum_t weight[g][t] P_n(t)
+ * q: n_traj rows of ldq doubles [Re a (n_states) | Im a (n_states) | ...]
+ * (what mqct_hip_propagate returns, ldq = 2*n_states+8); indx_chann: the
+ * channel of every state (1-based, INI_ARRAYS order); weight[g][t]: the
+ * INELAST_CALC factors (2J+1) pi/(k^2 (2j+1)) of trajectory t summed over
+ * its (J, l) pairs, one row per group (a collision energy for sigma(U), or
+ * a J for the J-resolved table).  sigma: [n_groups][n_channels] in bohr^2
+ * (the caller applies a0^2 and the billing factor, propagator.f:1198-1276);
+ * opacity (optional): P[n_traj][n_channels].  Sums run in a fixed order,
+ * so results repeat bit for bit. */
+int mqct_hip_cross_sections(mqct_hip_t *h, int n_traj, int n_states, int ldq,
+                            const double *q, int n_channels,
+                            const int *indx_chann, int n_groups,
+                            const double *weight, double *sigma,
+                            double *opacity);
+
 /* ---------------------------------------------------------------------- */
 /* MTRX_UF.DAT (Fortran sequential unformatted, 4-byte record markers):
  * PRINT_MIJ_USER (matrix_ini.f:6640-6780) writer and READ_MIJ_USER

@@ -148,6 +148,39 @@ def inelastic_sigma(P, l_values, J_range, j_ini, k):
     return sig
 
 
+def inelastic_weights(l_values, J_range, j_ini, k, per_J=False):
+    """The INELAST_CALC factors (propagator.f:4778-4833) as weights per
+    trajectory: sigma_n = sum_t w[t] P_n(t) with w[t] = sum over the J whose
+    l range |J - j| .. J + j holds l_t of (2J+1) pi / (k^2 (2j+1)).
+    per_J: one row per J of J_range (the J-resolved table) instead.
+    Returns [groups][trajectories]."""
+    lmap = {int(l): i for i, l in enumerate(l_values)}
+    Js = list(range(J_range[0], J_range[1] + 1))
+    W = np.zeros((len(Js) if per_J else 1, len(l_values)))
+    for gi, J in enumerate(Js):
+        for l in range(abs(J - j_ini), J + j_ini + 1):
+            if l in lmap:
+                W[gi if per_J else 0, lmap[l]] += (2.0 * J + 1.0) / k ** 2 / (2 * j_ini + 1) * np.pi
+    return W
+
+
+def gpu_cross_sections(dev, q, basis, weights, opacity=False):
+    """sigma[g][n] = sum_t weights[g][t] P_n(t) from the final amplitudes q
+    (rows as Propagator.run returns them) on the GPU
+    (mqct_hip_cross_sections); bohr^2.  With opacity=True also returns
+    P[t][n]."""
+    q = L.as_f64(q)
+    W = L.as_f64(np.atleast_2d(weights))
+    T, ldq = q.shape
+    C = basis.n_channels
+    sig = np.zeros((W.shape[0], C))
+    P = np.zeros((T, C)) if opacity else None
+    L.check(dev.lib.mqct_hip_cross_sections(dev.h, T, basis.n_states, ldq, L.dptr(q), C,
+                                            L.iptr(L.as_i32(basis.indx_chann)), W.shape[0], L.dptr(W),
+                                            L.dptr(sig), L.dptr(P)), "cross_sections")
+    return (sig, P) if opacity else sig
+
+
 def billing_scale(E_ch_cm, chann_ini, U_cm):
     """a0^2 U / E_bill with E_bill = U + dE/2 + dE^2/(16U) (propagator.f:276-287, 1208)."""
     dE = np.asarray(E_ch_cm) - E_ch_cm[chann_ini - 1]

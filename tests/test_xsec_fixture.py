@@ -58,3 +58,18 @@ def test_partial_cross_sections_per_J():
         got = (2 * J + 1) * np.pi / k ** 2 * P[J] * A_BOHR ** 2  # j = 0: l = J only
         m = ref > 0
         assert np.max(np.abs(got[m] - ref[m]) / ref[m]) <= 1e-4
+
+
+def test_inelastic_weights_reproduce_the_sums():
+    """The per-trajectory weights the GPU kernel takes give the same sums
+    as INELAST_CALC's double loop over J and l (propagator.f:4778-4833),
+    for j_ini = 0 and 2 and the J-resolved rows."""
+    f, mass, b, L, P = _setup()
+    from mqct_amd.propagate import inelastic_weights
+    for j_ini in (0, 2):
+        ls, Jr, k = l_grid(mass, f["U"], f["B_IMPCT"], j_ini, j_ini)
+        Pj = P[:len(ls)] if len(ls) <= len(P) else np.resize(P, (len(ls), P.shape[1]))
+        Wt = inelastic_weights(ls, Jr, j_ini, k)
+        assert np.allclose(Wt[0] @ Pj, inelastic_sigma(Pj, ls, Jr, j_ini, k), rtol=1e-14, atol=0)
+        WJ = inelastic_weights(ls, Jr, j_ini, k, per_J=True)
+        assert np.allclose(WJ.sum(0), Wt[0], rtol=1e-14, atol=0)
