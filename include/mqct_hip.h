@@ -305,6 +305,25 @@ int mqct_hip_cross_sections(mqct_hip_t *h, int n_traj, int n_states, int ldq,
                             const double *weight, double *sigma,
                             double *opacity);
 
+/* The same cross sections without any allocation: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * All array arguments are device pointers the caller owns (q_dev,
+ * indx_chann_dev, weight_dev in, sigma_dev and the optional opacity_dev
+ * out); the working memory is a fixed 32 MiB device array inside the
+ * library, one per GPU, held until the process ends.  A call needs
+ * 8 * (min(64, n_traj) * n_groups * n_channels + n_traj * n_channels
+ * [omitted when opacity_dev is given] + n_channels/2 + 3) bytes of it and is
+ * refused with MQCT_ERR_ARG beyond 32 MiB.  Runs on the handle's stream and
+ * returns when the results are in place.  Every call on one GPU uses the
+ * same working memory: calls that overlap on one GPU (from several threads
+ * or handles) race. */
+int mqct_hip_cross_sections_noalloc(mqct_hip_t *h, int n_traj, int n_states,
+                                    int ldq, const double *q_dev,
+                                    int n_channels, const int *indx_chann_dev,
+                                    int n_groups, const double *weight_dev,
+                                    double *sigma_dev, double *opacity_dev);
+
 /* ---------------------------------------------------------------------- */
 /* MTRX_UF.DAT (Fortran sequential unformatted, 4-byte record markers):
  * PRINT_MIJ_USER (matrix_ini.f:6640-6780) writer and READ_MIJ_USER

@@ -106,6 +106,9 @@ SIGNATURES = {
                                                  ctypes.POINTER(TrajStatT)]),
     "mqct_hip_cross_sections": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, c_dbl_p,
                                                ctypes.c_int, c_int_p, ctypes.c_int, c_dbl_p, c_dbl_p, c_dbl_p]),
+    "mqct_hip_cross_sections_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                                       ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                                                       ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "mqct_hip_write_mtrx_uf": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                               ctypes.c_int64, ctypes.c_int, c_int_p, c_int_p, c_int_p, c_int_p,
                                               c_int_p, c_dbl_p, c_dbl_p]),

@@ -95,6 +95,7 @@ extern "C" int mqct_hip_version(void) { return 10000; }
 
 void mqct_dyn_free(mqct_hip *h);
 void mqct_mlp_free(mqct_hip *h);
+int mqct_xsec_scratch_init(int device);
 
 extern "C" int mqct_hip_create(int device, mqct_hip_t **out) {
   if (!out) return mqct_fail(MQCT_ERR_ARG, "create: null handle pointer");
@@ -109,6 +110,8 @@ extern "C" int mqct_hip_create(int device, mqct_hip_t **out) {
     return mqct_fail(MQCT_ERR_ARG, std::string("create: library is built for "
                                                "gfx950, device is ") +
                                        prop.gcnArchName);
+  int src = mqct_xsec_scratch_init(device);
+  if (src) return src;
   mqct_hip *h = new mqct_hip();
   h->device = device;
   HIP_TRY(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));

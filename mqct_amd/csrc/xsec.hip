@@ -17,6 +17,7 @@
 #include "mqct_internal.h"
 
 #include <algorithm>
+#include <mutex>
 
 namespace {
 
@@ -159,6 +160,63 @@ extern "C" int mqct_hip_cross_sections(mqct_hip_t *h, int n_traj, int n_states, 
   HIP_TRY(hipMemcpyAsync(sigma, d_sig.p, (size_t)G * C * 8, hipMemcpyDeviceToHost, h->stream));
   if (opacity)
     HIP_TRY(hipMemcpyAsync(opacity, d_op.p, (size_t)T * C * 8, hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  return MQCT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// The variant that allocates nothing (mqct_hip_cross_sections_noalloc):
+// the working memory is this file-scope device array, one copy per GPU,
+// held until the process ends.  Host code never passes its name to a
+// kernel or copy: the per-device address comes from hipGetSymbolAddress,
+// taken once per device at handle creation with that device current.
+constexpr long long kScratchDoubles = 4LL << 20;  // 32 MiB
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+constexpr int kMaxDevices = 64;
+static double *g_scratch_ptr[kMaxDevices] = {};
+static std::mutex g_scratch_mu;
+
+int mqct_xsec_scratch_init(int device) {
+  if (device < 0 || device >= kMaxDevices)
+    return mqct_fail(MQCT_ERR_ARG, "create: device index beyond the library's table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device]) return MQCT_OK;
+  void *ptr = nullptr;
+  hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr)
+    return mqct_fail(MQCT_ERR_HIP, std::string("create: hipGetSymbolAddress(scratch): ") +
+                                       hipGetErrorString(e));
+  g_scratch_ptr[device] = static_cast<double *>(ptr);
+  return MQCT_OK;
+}
+
+extern "C" int mqct_hip_cross_sections_noalloc(mqct_hip_t *h, int n_traj, int n_states, int ldq,
+                                               const double *q_dev, int n_channels,
+                                               const int *indx_chann_dev, int n_groups,
+                                               const double *weight_dev, double *sigma_dev,
+                                               double *opacity_dev) {
+  if (!h || !q_dev || !indx_chann_dev || !weight_dev || !sigma_dev)
+    return mqct_fail(MQCT_ERR_ARG, "cross_sections_noalloc: null argument");
+  int rc = xsec_args(n_traj, n_states, ldq, n_channels, n_groups);
+  if (rc) return rc;
+  long long need = mqct_xsec_work_doubles(n_traj, n_channels, n_groups, opacity_dev != nullptr);
+  if (need > kScratchDoubles)
+    return mqct_fail(MQCT_ERR_ARG, "cross_sections_noalloc: needs " + std::to_string(need * 8) +
+                                       " bytes of working memory, the fixed scratch holds " +
+                                       std::to_string(kScratchDoubles * 8) +
+                                       " (fewer trajectories, channels or groups per call, or "
+                                       "mqct_hip_cross_sections)");
+  double *work = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    work = (h->device >= 0 && h->device < kMaxDevices) ? g_scratch_ptr[h->device] : nullptr;
+  }
+  if (!work) return mqct_fail(MQCT_ERR_STATE, "cross_sections_noalloc: no scratch address for this device");
+  HIP_TRY(hipSetDevice(h->device));
+  rc = mqct_xsec_run(h, n_traj, n_states, ldq, q_dev, n_channels, indx_chann_dev, n_groups, weight_dev,
+                     sigma_dev, opacity_dev, work);
+  if (rc) return rc;
   HIP_TRY(hipStreamSynchronize(h->stream));
   return MQCT_OK;
 }
