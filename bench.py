@@ -4,24 +4,29 @@ Headline (`value`): AT-MQCT trajectories per second for the batched
 propagation of the asymmetric top + atom system (1771 states, 86 779
 coupling pairs, 300 spline knots; 4096 trajectories with b uniform in
 [0, 18] bohr and E uniform in [200, 2000] cm^-1; amplitudes complex FP64,
-Cash-Karp RK45 with tolerance 1e-8 and MIN_TMSTP 20 a.u.; by default the
-spline coefficients are stored in FP32 and the pipelined interval-GEMM forms
-form their products in FP32 on f32 MFMA).  One step = one propagation of the
+Cash-Karp RK45 (odeint/rkqs) with EPS_ODEINT 1e-8 and MIN_TMSTP 20 a.u.
+at TIME_STEP 10 a.u.: the first trial of every step lies at or below
+MIN_TMSTP, so rkqs accepts steps of 20 a.u. without its error control
+(suppl_routines.f:1245-1253); MIJ_SHIFT on; by default the spline
+coefficients are stored in FP32 and the pipelined interval-GEMM forms form
+their products in FP32 on f32 MFMA).  One step = one propagation of the
 whole batch, inputs (matrix splines and recorded classical paths) resident
 in HBM.
 
 Reported beside it:
   * the same batch with FP64 coefficients, and the cross-section error of
-    the timed form against it (FP32 is the storage the workload names for
-    the spline coefficients: "complex FP32 1771x1771"; the metric admits
-    1e-5 relative cross-section error);
+    the timed form against it (the metric admits 1e-5 relative);
+  * the error-controlled form on a sample of the batch: TIME_STEP 60 a.u.
+    puts every first trial above MIN_TMSTP, so rkqs's error control decides
+    each step (one rejected trial per step at EPS_ODEINT 1e-8); its rate and
+    the cross-section difference of the timed form from it;
   * the plain FP64 per-trajectory kernel on a sample of the batch (the
     comparison the metric names);
   * the roofline of the dominant kernel (the interval GEMM) from HIP events
     recorded around each of its launches on its stream;
   * matrix-build times of the asymmetric top + atom system (24 x 48 grid,
-    300 R) and of the two-asymmetric-tops system (j1, j2 <= 2, 16^5 grid,
-    128 R) in FP64, FP32 and split-BF16, and the cross-section error of the
+    300 R) and of the two-asymmetric-tops system (16^5 grid, 128 R) in
+    FP64, FP32 and split-BF16, and the cross-section error of the
     mixed-precision builds through the AT propagation on a sample;
   * the CPU baseline: the C oracle's AT-RK45 trajectory on one host core.
 
@@ -75,6 +80,11 @@ def main():
     ap.add_argument("--n-traj", type=int, default=4096)
     ap.add_argument("--hmin", type=float, default=20.0,
                     help="MIN_TMSTP (a.u.): the step floor of odeint's error control")
+    ap.add_argument("--time-step", type=float, default=10.0,
+                    help="TIME_STEP (a.u.) of the timed form: odeint's first trial step is TIME_STEP/sqrt(U/300)")
+    ap.add_argument("--ctrl-sample", type=int, default=1024,
+                    help="trajectories of the error-controlled run (TIME_STEP 60: every first trial above "
+                         "MIN_TMSTP); 0 skips it")
     ap.add_argument("--knots", choices=("fp32", "fp64", "fp32p"), default="fp32p",
                     help="storage precision of the spline-coefficient matrices of the timed form "
                          "(fp32p: FP32 storage and FP32 MFMA products in the pipelined GEMM forms)")
@@ -84,7 +94,7 @@ def main():
                     help="comma-separated interval-GEMM shape per level floor(log2(running trajectories)), "
                          "from level 0 up (-1: chosen by timing), for the timed form's knot precision: "
                          "repeats a measured run's choices in a profiled one")
-    ap.add_argument("--plain-sample", type=int, default=512)
+    ap.add_argument("--plain-sample", type=int, default=256)
     ap.add_argument("--xsec-sample", type=int, default=512)
     ap.add_argument("--no-graph", action="store_true",
                     help="time the steps with per-launch HIP events instead of replaying the captured "
@@ -148,10 +158,13 @@ def main():
         arr = (ctypes.c_int * len(lv_set))(*lv_set)
         L.check(dev.lib.mqct_hip_batched_set_tile_levels(dev.h, form - 1, len(lv_set), arr),
                 "set_tile_levels")
-    rows = []
-    for i in range(n):
-        rows += prop.trajectories([l[i]], float(E[i]), 1, time_step=10.0, eps=1e-8, hmin=args.hmin,
-                                  at_paths=[i])
+    def batch_rows(time_step, count=n):
+        out = []
+        for i in range(count):
+            out += prop.trajectories([l[i]], float(E[i]), 1, time_step=time_step, eps=1e-8, hmin=args.hmin,
+                                     at_paths=[i])
+        return out
+    rows = batch_rows(args.time_step)
 
     # prime: knot matrices, kernels, graph (not a step); without graphs when
     # the steps are event-timed (kernel tracers cannot follow graph launches)
@@ -169,9 +182,16 @@ def main():
         q, st = prop.run(L.MQCT_PROP_AT_ODEINT, rows, batched=form)
     el = time.perf_counter() - t0
     value = n * args.steps / el
+    # each rkqs trial after the first of a step costs 5 more right-hand sides
+    rejected = (st["n_rhs"] - 6 * st["n_steps"]) / 5.0
+    integ = {"rhs_mean": float(st["n_rhs"].mean()), "rhs_max": int(st["n_rhs"].max()),
+             "steps_mean": float(st["n_steps"].mean()), "rejected_trials_mean": float(rejected.mean()),
+             "trajectories_with_rejections": int((rejected > 0).sum()), "norm_err_max": float(st["norm_err"].max()),
+             "maxstp_reached": int((st["n_steps"] >= 10000).sum())}
     log("batched AT (%s coefficients): %d traj x %d steps in %.2f s -> %.1f traj/s; rhs mean %.0f max %d; "
-        "steps max %d; norm err max %.2e" % (args.knots, n, args.steps, el, value, st["n_rhs"].mean(),
-                                             st["n_rhs"].max(), st["n_steps"].max(), st["norm_err"].max()))
+        "steps mean %.0f, rejected trials mean %.0f; norm err max %.2e" % (
+            args.knots, n, args.steps, el, value, st["n_rhs"].mean(), st["n_rhs"].max(), st["n_steps"].mean(),
+            rejected.mean(), st["norm_err"].max()))
 
     # -- one step with HIP events around every GEMM launch --------------------
     if args.no_graph:
@@ -268,6 +288,24 @@ def main():
     else:
         q64 = q
 
+    # -- the error-controlled form on a sample (TIME_STEP 60) -------------------
+    ctrl = None
+    if args.ctrl_sample > 0:
+        nc = min(args.ctrl_sample, n)
+        rows_c = batch_rows(60.0, nc)
+        t0 = time.perf_counter()
+        qc, sc = prop.run(L.MQCT_PROP_AT_ODEINT, rows_c, batched=form)
+        cel = time.perf_counter() - t0
+        ec, nchc = rel_err(xsec_functional(prop, q[:nc], l[:nc], E[:nc]), xsec_functional(prop, qc, l[:nc], E[:nc]))
+        ctrl = {"traj_per_s": round(nc / cel, 3), "n_traj": nc, "time_step": 60.0,
+                "rhs_mean": float(sc["n_rhs"].mean()), "steps_mean": float(sc["n_steps"].mean()),
+                "rejected_trials_mean": float(((sc["n_rhs"] - 6 * sc["n_steps"]) / 5.0).mean()),
+                "maxstp_reached": int((sc["n_steps"] >= 10000).sum()),
+                "xsec_rel_err_of_timed_form": ec, "channels": nchc}
+        log("error-controlled (TIME_STEP 60), %d traj: %.2f s -> %.1f traj/s; rhs mean %.0f, rejected trials mean "
+            "%.0f; timed form's xsec vs it %.2e" % (nc, cel, nc / cel, sc["n_rhs"].mean(),
+                                                    ctrl["rejected_trials_mean"], ec))
+
     # -- plain FP64 per-trajectory kernel on a sample ------------------------
     plain = None
     if not args.no_plain and args.plain_sample > 0:
@@ -330,8 +368,10 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "f64/f32" if form == 3 else "f64", "data": "synthetic",
         "config": {"workload": "AT-MQCT batched propagation: asymmetric top + atom, 1771 states, "
                                "86779 pairs, 300 knots, 4096 trajectories (b 0-18 bohr, E 200-2000 cm^-1), "
-                               "RK45 tol 1e-8, MIN_TMSTP %g a.u., FP64 amplitudes, %s"
-                               % (args.hmin, {"fp64": "FP64 spline coefficients",
+                               "Cash-Karp RK45 (odeint/rkqs) EPS_ODEINT 1e-8, TIME_STEP %g a.u., MIN_TMSTP "
+                               "%g a.u. (first trials at or below MIN_TMSTP: steps accepted at the floor "
+                               "without error control), MIJ_SHIFT, FP64 amplitudes, %s"
+                               % (args.time_step, args.hmin, {"fp64": "FP64 spline coefficients",
                                               "fp32": "FP32 spline coefficients, FP64 products",
                                               "fp32p": "FP32 spline coefficients, FP32 products in the "
                                                        "pipelined GEMM forms"}[args.knots]),
@@ -339,6 +379,8 @@ def main():
                    "knots": int(len(wl.R)), "form": "batched", "graph": not args.no_graph},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "integrator": integ,
+        "error_controlled_sample": ctrl,
         "batched_other_coefficients": other,
         "plain_fp64": plain,
         "speedup_vs_plain": round(value / plain["traj_per_s"], 2) if plain else None,

@@ -202,10 +202,12 @@ def _levels(jmin, jmax, A, B, C, ld):
     return out
 
 
-def channels_type0(jmin1, jmax1, jmin2, jmax2, A1, B1, C1, A2, B2, C2, keep=None):
+def channels_type0(jmin1, jmax1, jmin2, jmax2, A1, B1, C1, A2, B2, C2, keep=None, emax=None):
     """Distinguishable asym. top + asym. top; channel = (level1, level2),
     molecule 1 outermost (iotest.f:10383-10411).  keep(j1,ka1,kc1,j2,ka2,kc2)
-    optionally selects a CHNLS_LIST-like subset (order preserved)."""
+    optionally selects a CHNLS_LIST-like subset (order preserved); emax
+    keeps the pairs with E1 + E2 <= emax (cm^-1), also in that order -- the
+    CHNLS_LIST of those pairs, not the energy-sorted EMAX= selection."""
     ld = 2 * max(jmax1, jmax2) + 1
     L1 = _levels(jmin1, jmax1, A1, B1, C1, ld)
     L2 = _levels(jmin2, jmax2, A2, B2, C2, ld)
@@ -213,6 +215,8 @@ def channels_type0(jmin1, jmax1, jmin2, jmax2, A1, B1, C1, A2, B2, C2, keep=None
     for a in L1:
         for b in L2:
             if keep is not None and not keep(a[0], a[1], a[2], b[0], b[1], b[2]):
+                continue
+            if emax is not None and a[3] + b[3] > emax:
                 continue
             rows.append((a, b))
     return Basis(

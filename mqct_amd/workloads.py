@@ -135,15 +135,30 @@ def asym_top_atom(jmax=10, n_beta=24, n_gamma=48, n_r=300, seed=None, ref_xz=1):
     return Workload("asym_top_atom", b, g, R, 1, par, 3.27)
 
 
+# The two-tops basis of the benchmark: the CHNLS_LIST of every pair level
+# with j1, j2 <= 4 and E1 + E2 <= 236.5 cm^-1, molecule 1 outermost (123
+# channels, 2043 states, 282 179 pairs; m-blocks of up to 369 states).  The
+# literal "j1, j2 <= 4 with all m and tau" is 27 225 states and 32.8 M
+# pairs; an energy cut over whole pair levels reaches 2001 or 2043 states
+# around the stated 2025, and the cut above the 236.41 cm^-1 levels is taken.
+C3_JMAX, C3_EMAX = 4, 236.5
+
+
 def two_asym_tops(jmax1=2, jmax2=2, n_ang=16, n_r=128, seed=None, keep=None,
-                  n_ang_each=None):
+                  n_ang_each=None, emax=None):
     b = B.ini_arrays(B.channels_type0(0, jmax1, 0, jmax2, ROT_A, ROT_B, ROT_C,
-                                      ROT_A, ROT_B, ROT_C, keep=keep))
+                                      ROT_A, ROT_B, ROT_C, keep=keep, emax=emax))
     na = n_ang_each or (n_ang,) * 5
     g = B.grid_type0(*na)
     R = B.r_grid(n_r, 4.0, 25.0)
     par = pes_sites_params(SEEDS["two_asym_tops"] if seed is None else seed)
     return Workload("two_asym_tops", b, g, R, 2, par, 9.0)
+
+
+def two_asym_tops_c3(n_ang=16, n_r=128, seed=None):
+    """The benchmark's two-tops system: C3_JMAX, C3_EMAX pair levels on the
+    16^5 grid with 128 R from 4 to 25 bohr."""
+    return two_asym_tops(C3_JMAX, C3_JMAX, n_ang=n_ang, n_r=n_r, seed=seed, emax=C3_EMAX)
 
 
 def at_batch(n_traj=4096, b_max=18.0, e_min=200.0, e_max=2000.0, seed=20240504):

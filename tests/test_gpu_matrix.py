@@ -156,25 +156,54 @@ def test_two_asym_tops_fp64_small(dev):
     assert fp64_ok(M, Mo, abs_sums(orc, wl)) <= 1.0
 
 
-def test_asym_top_atom_full_size_sampled(dev):
-    """Full size (1771 states, 86 779 pairs, 24x48 grid, 300 R) on the GPU;
-    the oracle checks a seeded sample of 150 elements at 5 R points each."""
+@pytest.fixture(scope="module")
+def c1_sample():
+    """Config 1 at full size (1771 states, 86 779 pairs, 24x48 grid, 300 R):
+    the oracle's values of a seeded sample of 150 elements at 5 R points,
+    with their absolute-term sums A."""
     wl = W.asym_top_atom()
-    mb = MatrixBuilder(dev, wl)
-    M, D, S = mb.build("fp64")
     orc = O.oracle_for_workload(wl)
     rng = np.random.default_rng(7)
     ks = rng.choice(wl.basis.total_size, 150, replace=False)
     irs = [0, 57, 149, 233, 299]
     A = abs_sums(orc, wl, ks, irs)
-    worst = 0.0
-    for a, k in enumerate(ks):
-        if S[k]:
-            continue
-        for b, ir in enumerate(irs):
-            o = orc.element(wl.basis.ind_mat, int(k), ir) * wl.conv_unit_e
-            worst = max(worst, abs(M[k, ir] - o) / max(1e-10 * abs(o), 1e-13 * A[a, b]))
+    Mo = np.array([[orc.element(wl.basis.ind_mat, int(k), ir) * wl.conv_unit_e for ir in irs] for k in ks])
+    return wl, ks, irs, A, Mo
+
+
+def test_asym_top_atom_full_size_sampled(dev, c1_sample):
+    """FP64 at full size against the oracle on the sample; reports how many
+    sampled values miss a pure 1e-10 relative bar (strong cancellation:
+    |value| far below A) besides the combined bar."""
+    wl, ks, irs, A, Mo = c1_sample
+    M, D, S = MatrixBuilder(dev, wl).build("fp64")
+    G = M[ks][:, irs]
+    live = ~S[ks].astype(bool)
+    worst = np.max((np.abs(G - Mo) / np.maximum(np.maximum(1e-10 * np.abs(Mo), 1e-13 * A), 1e-300))[live])
+    rel = np.abs(G - Mo) / np.maximum(np.abs(Mo), 1e-300)
+    miss = (rel > 1e-10) & live[:, None]
+    print(f"FP64 full size: {int(miss.sum())} of {int(live.sum()) * len(irs)} sampled values miss 1e-10 relative; "
+          f"their |value|/A is at most {np.max((np.abs(Mo) / A)[miss]) if miss.any() else 0:.1e}")
     assert worst <= 1.0
+    if miss.any():
+        assert np.max((np.abs(Mo) / A)[miss]) < 1e-3   # only strongly cancelling values
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+def test_asym_top_atom_full_size_mixed_sampled(dev, c1_sample, prec):
+    """Config 1's precision forms at full size against the oracle on the
+    sample: the per-element maximum relative error (over the sampled R
+    points, relative to the element's largest |value| there) is reported,
+    and every value is held to 4e-6 A."""
+    wl, ks, irs, A, Mo = c1_sample
+    M, D, S = MatrixBuilder(dev, wl).build(prec)
+    G = M[ks][:, irs]
+    live = ~S[ks].astype(bool)
+    per_el = (np.abs(G - Mo).max(axis=1) / np.maximum(np.abs(Mo).max(axis=1), 1e-300))[live]
+    back = np.max((np.abs(G - Mo) / A)[live])
+    print(f"{prec} full size: per-element max relative error {per_el.max():.2e} (median {np.median(per_el):.2e}), "
+          f"|err|/A max {back:.2e}")
+    assert back <= MIXED_TOL[prec]
 
 
 # FP32: operands and chunk sums in FP32; split-BF16: three BF16 terms per
