@@ -235,9 +235,10 @@ int mqct_hip_batched_prof_trace(mqct_hip_t *h, int cap, double *rows, int *n);
  * over the fragment-ordered knot tiles, 3 fragment-ordered work queue of
  * (knot, tile group, entry chunk) items over persistent workgroups, 4
  * LDS-staged work queue of (knot, 64-row group, entry chunk) items, 5
- * software-pipelined work queue of (knot, m-block, entry chunk) items
- * (m-blocks of at most 128 states), 6 the same with the items handed out
- * one at a time from a device counter, largest m-blocks first;
+ * software-pipelined work queue of (knot, row group, entry chunk) items
+ * (a row group: up to 128 rows of one m-block, contracted over the whole
+ * block), 6 the same with the items handed out one at a time from a device
+ * counter, largest row groups first;
  * entries: bucket entries per super-tile (per wave for forms 1-3); k_rows:
  * K rows per LDS slice (0 without LDS); row_tiles: 16-row tiles per wave.
  * The runner chooses a shape per level floor(log2(running trajectories)) by
@@ -250,9 +251,9 @@ int mqct_hip_batched_tile(mqct_hip_t *h, int shape, int *form, int *entries,
 int mqct_hip_batched_tile_levels(mqct_hip_t *h, int fp32, int n, int *shapes);
 /* Sets the shapes of levels 0..n-1 (n <= 24; -1: chosen by timing) for the
  * product form fp32 (0, 1, 2 as in batched_tile_levels), e.g. to repeat a
- * measured run's choices in a profiled one.  A batched run refuses a set shape that does not fit its
- * matrix (pipelined forms: m-blocks of at most 128 states; work-queue forms:
- * at most 16383 knots). */
+ * measured run's choices in a profiled one.  A batched run refuses a set
+ * shape that does not fit its matrix (work-queue forms: at most 16383
+ * knots). */
 int mqct_hip_batched_set_tile_levels(mqct_hip_t *h, int fp32, int n, const int *shapes);
 /* Fixes the interval-GEMM tile shape (index into the candidate list,
  * 0..21) for the following batched runs, or -1 to choose by timing again. */
@@ -341,6 +342,19 @@ int mqct_hip_write_mtrx_uf(const char *path, int coll_type, int n_channels,
 int mqct_hip_read_mtrx_uf(const char *path, int64_t *hdr, int *indx_chann,
                           int *j12, int *m12, int *chan_qn, int *ind_mat,
                           double *R_com, double *mat_el);
+
+/* The classical paths of the first AT run, in the reference's files:
+ * dir/AT_<j12>_<m12>_<l>_<parity> (one Fortran unformatted record of 8
+ * REAL*8 -- t, R, P_R, Theta, P_Theta, Phi, P_Phi, V -- per step) and
+ * dir/n_AT_<...> (one INTEGER*4 record: the number of records).  Replaces
+ * SAVE_TRAJECT's writes (propagator.f:2838-2852 open, 3457-3462 and
+ * 4330-4333 write) and the AT run's read-back (propagator.f:2994-3005).
+ * records: [n_rec][8] as mqct_hip_propagate_record returns them.
+ * read_at_path with records == NULL returns the count in n_rec only. */
+int mqct_hip_write_at_path(const char *dir, int j12, int m12, int l,
+                           int parity, int n_rec, const double *records);
+int mqct_hip_read_at_path(const char *dir, int j12, int m12, int l,
+                          int parity, int cap, double *records, int *n_rec);
 
 /* ---------------------------------------------------------------------- */
 /* Learned potential: an MLP (12 features -> 4 x 256 SiLU -> 1) trained and

@@ -114,6 +114,10 @@ SIGNATURES = {
                                               c_int_p, c_dbl_p, c_dbl_p]),
     "mqct_hip_read_mtrx_uf": (ctypes.c_int, [ctypes.c_char_p, c_i64_p, c_int_p, c_int_p, c_int_p, c_int_p,
                                              c_int_p, c_dbl_p, c_dbl_p]),
+    "mqct_hip_write_at_path": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                              ctypes.c_int, c_dbl_p]),
+    "mqct_hip_read_at_path": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                             ctypes.c_int, c_dbl_p, c_int_p]),
     "mqct_hip_mlp_init": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64]),
     "mqct_hip_mlp_n_params": (ctypes.c_int, []),
     "mqct_hip_mlp_get_params": (ctypes.c_int, [ctypes.c_void_p, c_flt_p]),

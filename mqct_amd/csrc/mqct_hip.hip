@@ -768,3 +768,67 @@ extern "C" int mqct_hip_read_mtrx_uf(const char *path, int64_t *hdr,
   if (!ok) return mqct_fail(MQCT_ERR_ARG, "read_mtrx_uf: truncated or malformed file");
   return MQCT_OK;
 }
+
+// ---------------------------------------------------------------------------
+// The classical path of the first AT run (SAVE_TRAJECT): in directory dir,
+// AT_<j12>_<m12>_<L>_<parity> holds one Fortran unformatted record of 8
+// REAL*8 (t, R, P_R, Theta, P_Theta, Phi, P_Phi, V) per step and
+// n_AT_<...> one record with the number of records as INTEGER*4
+// (propagator.f:2838-2852 names and opens them, 3457-3462 and 4330-4333
+// write, 2994-3005 read back).
+static std::string at_name(const char *dir, const char *pre, int j12, int m12, int l, int parity) {
+  char nm[128];
+  std::snprintf(nm, sizeof nm, "%sAT_%d_%d_%d_%d", pre, j12, m12, l, parity);
+  std::string d(dir);
+  if (!d.empty() && d.back() != '/') d += '/';
+  return d + nm;
+}
+
+extern "C" int mqct_hip_write_at_path(const char *dir, int j12, int m12, int l, int parity,
+                                      int n_rec, const double *records) {
+  if (!dir || !records || n_rec < 0) return mqct_fail(MQCT_ERR_ARG, "write_at_path: bad argument");
+  const std::string p = at_name(dir, "", j12, m12, l, parity), pn = at_name(dir, "n_", j12, m12, l, parity);
+  FILE *f = std::fopen(p.c_str(), "wb");
+  if (!f) return mqct_fail(MQCT_ERR_ARG, "write_at_path: cannot open " + p);
+  RecWriter w{f, {}};
+  bool ok = true;
+  for (int i = 0; i < n_rec && ok; i++) {
+    for (int c = 0; c < 8; c++) w.f64(records[8 * (size_t)i + c]);
+    ok &= w.flush();
+  }
+  ok &= std::fclose(f) == 0;
+  FILE *g = std::fopen(pn.c_str(), "wb");
+  if (!g) return mqct_fail(MQCT_ERR_ARG, "write_at_path: cannot open " + pn);
+  RecWriter wn{g, {}};
+  wn.i32(n_rec);
+  ok &= wn.flush();
+  ok &= std::fclose(g) == 0;
+  if (!ok) return mqct_fail(MQCT_ERR_ARG, "write_at_path: write failed for " + p);
+  return MQCT_OK;
+}
+
+extern "C" int mqct_hip_read_at_path(const char *dir, int j12, int m12, int l, int parity, int cap,
+                                     double *records, int *n_rec) {
+  if (!dir || !n_rec) return mqct_fail(MQCT_ERR_ARG, "read_at_path: bad argument");
+  const std::string p = at_name(dir, "", j12, m12, l, parity), pn = at_name(dir, "n_", j12, m12, l, parity);
+  FILE *g = std::fopen(pn.c_str(), "rb");
+  if (!g) return mqct_fail(MQCT_ERR_ARG, "read_at_path: cannot open " + pn);
+  RecReader rn{g, {}};
+  int n = -1;
+  bool ok = rn.next() && rn.buf.size() == 4 && rn.i32(n);
+  std::fclose(g);
+  if (!ok || n < 0) return mqct_fail(MQCT_ERR_ARG, "read_at_path: bad record count in " + pn);
+  *n_rec = n;
+  if (!records) return MQCT_OK;  // size query
+  if (n > cap) return mqct_fail(MQCT_ERR_ARG, "read_at_path: more records than cap in " + p);
+  FILE *f = std::fopen(p.c_str(), "rb");
+  if (!f) return mqct_fail(MQCT_ERR_ARG, "read_at_path: cannot open " + p);
+  RecReader r{f, {}};
+  for (int i = 0; i < n && ok; i++) {
+    ok = r.next() && r.buf.size() == 64;
+    for (int c = 0; c < 8 && ok; c++) ok = r.f64(records[8 * (size_t)i + c]);
+  }
+  std::fclose(f);
+  if (!ok) return mqct_fail(MQCT_ERR_ARG, "read_at_path: " + p + " is not 8 REAL*8 per record for the count");
+  return MQCT_OK;
+}

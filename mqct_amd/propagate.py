@@ -193,6 +193,29 @@ def m_average_weight(m, jmin_ind, jmax_ind):
     return 2.0 / (1.0 + (1.0 if m == 0 else 0.0)) / ((jmax_ind + jmin_ind + 1.0) * (jmax_ind - jmin_ind + 1.0))
 
 
+def write_at_paths(directory, paths, j12, m12, l_values, parity=0):
+    """Writes every path to directory/AT_<j12>_<m12>_<L>_<parity> and
+    n_AT_<...> in the reference's record layout (mqct_hip_write_at_path)."""
+    lib = L.load()
+    for p, l in zip(paths, l_values):
+        rec = L.as_f64(p)
+        L.check(lib.mqct_hip_write_at_path(str(directory).encode(), int(j12), int(m12), int(l), int(parity),
+                                           len(rec), L.dptr(rec)), "write_at_path")
+
+
+def read_at_path(directory, j12, m12, l, parity=0):
+    """The records [n][8] of directory/AT_<j12>_<m12>_<l>_<parity>."""
+    lib = L.load()
+    n = ctypes.c_int()
+    d = str(directory).encode()
+    L.check(lib.mqct_hip_read_at_path(d, int(j12), int(m12), int(l), int(parity), 0, None, ctypes.byref(n)),
+            "read_at_path")
+    rec = np.zeros((n.value, 8))
+    L.check(lib.mqct_hip_read_at_path(d, int(j12), int(m12), int(l), int(parity), n.value, L.dptr(rec),
+                                      ctypes.byref(n)), "read_at_path")
+    return rec
+
+
 def at_first_step(dev, wl_small, b, E_cm, s_st=1, time_step=40.0, rec_cap=8000):
     """The first AT run: CC-MQCT trajectories on a small basis that record
     the classical path (SAVE_TRAJECT; propagator.f:3457-3462).  Returns the

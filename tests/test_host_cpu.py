@@ -144,3 +144,22 @@ def test_cpu_baseline_sample_is_nearest_to_the_mean():
     spec.loader.exec_module(bench)
     assert bench.baseline_sample(np.array([10, 50, 30, 31, 29, 100]), 4) == [1, 3, 2, 4]
     assert bench.baseline_sample(np.array([7]), 4) == [0]
+
+
+def test_at_path_files_round_trip_reference_bytes(tmp_path):
+    """The reference's own AT path files (EXAMPLES_IO/SYS_TYPE_0/
+    H2O_H2O_AT_Approximation/AT_MQCT_2nd_Step/AT_APPROX_TRAJS/500.00000/0/0/
+    AT_0_0_256_0 and n_AT_0_0_256_0, copied as data to tests/golden/at_files):
+    the reader gets the count of n_AT and 8 REAL*8 per record (a path from
+    R = 30 bohr inward and out, t increasing), and the writer reproduces
+    both files byte for byte (propagator.f:2838-2852, 3457-3462, 4330-4333)."""
+    from mqct_amd.propagate import read_at_path, write_at_paths
+    src = os.path.join(os.path.dirname(__file__), "golden", "at_files")
+    rec = read_at_path(src, 0, 0, 256, 0)
+    assert rec.shape == (2227, 8)
+    assert np.all(np.diff(rec[:, 0]) > 0)
+    assert rec[0, 0] == 0.0 and rec[0, 1] == 30.0
+    assert rec[:, 1].min() < 30.0
+    write_at_paths(tmp_path, [rec], 0, 0, [256], 0)
+    for nm in ("AT_0_0_256_0", "n_AT_0_0_256_0"):
+        assert (tmp_path / nm).read_bytes() == open(os.path.join(src, nm), "rb").read(), nm
