@@ -106,15 +106,17 @@ def main():
     ap.add_argument("--no-xsec", action="store_true")
     ap.add_argument("--no-build", action="store_true")
     ap.add_argument("--no-c3", action="store_true",
-                    help="skip the two-asymmetric-tops forms (builds, surrogate, 256 trajectories)")
-    ap.add_argument("--c3-traj", type=int, default=256)
-    ap.add_argument("--c3-dt", type=float, default=4.0,
+                    help="skip the two-asymmetric-tops forms (builds, surrogate, trajectory sample)")
+    ap.add_argument("--c3-traj", type=int, default=16,
+                    help="two-tops AT trajectories (a sample of the 1024-trajectory batch)")
+    ap.add_argument("--c3-plain-steps", type=int, default=40,
+                    help="steps of the bounded plain-kernel run that gives its right-hand-side rate")
+    ap.add_argument("--c3-dt", type=float, default=0.5,
                     help="TIME_STEP (a.u.) of the two-tops rk4 propagation")
-    ap.add_argument("--c3-plain", type=int, default=32, help="two-tops trajectories timed in the plain form")
     ap.add_argument("--sur-samples", type=int, default=4_000_000)
-    ap.add_argument("--sur-steps", type=int, default=50000)
+    ap.add_argument("--sur-steps", type=int, default=30000)
     ap.add_argument("--sur-batch", type=int, default=8192)
-    ap.add_argument("--sur-rmin", type=float, default=5.0,
+    ap.add_argument("--sur-rmin", type=float, default=4.0,
                     help="the surrogate covers R >= this (bohr); the plain PES fills the grid below")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of known size (4 launches, each reading and writing "
@@ -417,44 +419,51 @@ def elem_err(M, M64):
 
 
 def two_tops_section(dev, args):
-    """The two-asymmetric-tops system (j1, j2 <= 2: 1225 states, 115 070
-    pairs, 16^5 grid, 128 R): FP64, FP32 and split-BF16 builds with the
-    plain PES, and a split-BF16 build on the surrogate PES grid; each matrix
-    drives the same batch of AT trajectories (paths from a first step on
-    j1, j2 <= 1), and the cross sections are compared with the FP64
-    build's.  The propagator is rk4 with TIME_STEP = 4 a.u.: the site-site
-    potential couples the states strongly near the inner turning points, so
-    rk4 at 10 a.u. and odeint (whose error scale |y| + |h dydx| lets long
-    steps from the uncoupled region run into the coupled one) lose the norm
-    by orders of magnitude on some trajectories, while rk4 at 4 a.u. keeps
-    it within 4e-4 and its cross sections within 7e-3 of rk4 at 2 a.u."""
+    """The two-asymmetric-tops system of the metric (the CHNLS_LIST of the
+    pair levels with j1, j2 <= 4 and E1 + E2 <= 236.5 cm^-1: 2043 states,
+    282 179 pairs; 16^5 grid, 128 R): FP64, FP32 and split-BF16 builds with
+    the plain PES, and a split-BF16 build on the surrogate PES grid; the
+    FP64, split-BF16 and surrogate matrices drive the same sample of AT
+    trajectories (paths from a first
+    step on j1, j2 <= 1), batched with FP64 coefficients, rk4 at dt = c3_dt
+    a.u. (0.5: norm drift below 1e-6), and the cross sections are compared
+    with the FP64 build's.  The plain FP64 kernel's rate is measured on a
+    bounded run (its first c3_plain_steps steps of up to 64 trajectories, one
+    workgroup each) and
+    scaled by the sample's mean number of right-hand sides per trajectory."""
     from mqct_amd import _lib as L
     from mqct_amd import workloads as W
     from mqct_amd.matrix import MatrixBuilder
     from mqct_amd.propagate import Propagator, at_first_step
     from mqct_amd.surrogate import train_surrogate
     n = args.c3_traj
-    wl = W.two_asym_tops()
-    bb, E = W.two_tops_at_batch(n)
+    wl = W.two_asym_tops_c3()
+    bb, E = W.two_tops_at_batch(1024)
+    sel = np.linspace(0, 1023, n).astype(int) if n < 1024 else np.arange(1024)
+    bb, E = bb[sel], E[sel]
     t0 = time.time()
     paths, l, st0 = at_first_step(dev, W.two_asym_tops(jmax1=1, jmax2=1), bb, E, time_step=40.0, rec_cap=8000)
     log("two tops: first AT step %d paths in %.1f s, records max %d, failed %d" %
         (n, time.time() - t0, max(len(p) for p in paths), int(st0["fail"].sum())))
-    out = {"states": int(wl.basis.n_states), "pairs": int(wl.basis.total_size), "grid_points": int(wl.grid.G),
-           "r_points": int(len(wl.R)), "n_traj": n}
+    out = {"states": int(wl.basis.n_states), "pairs": int(wl.basis.total_size), "channels": int(wl.basis.n_channels),
+           "grid_points": int(wl.grid.G), "r_points": int(len(wl.R)), "n_traj": n,
+           "sample_of": 1024, "integrator": "AT rk4, dt %g a.u." % args.c3_dt}
 
-    def propagate(batched=1, sample=None):
+    def rows_for(prop, count=n, max_steps=3000000):
+        rows = []
+        for i in range(count):
+            rows += prop.trajectories([l[i]], float(E[i]), 1, time_step=10.0, at_paths=[i], max_steps=max_steps)
+            rows[-1]["dt"] = args.c3_dt
+        return rows
+
+    def propagate(batched=1):
         prop = Propagator(dev, wl.basis, wl.R, mass_amu=wl.mass_amu, from_build=True)
         prop.load_paths(paths)
-        m = n if sample is None else sample
-        rows = []
-        for i in range(m):
-            rows += prop.trajectories([l[i]], float(E[i]), 1, time_step=args.c3_dt, eps=1e-8, hmin=args.hmin,
-                                      at_paths=[i])
+        rows = rows_for(prop)
         t0 = time.perf_counter()
         q, st = prop.run(L.MQCT_PROP_AT_RK4, rows, batched=batched)
         el = time.perf_counter() - t0
-        return prop, q, st, m / el
+        return prop, q, st, n / el
 
     mb = MatrixBuilder(dev, wl)
     M64 = None
@@ -462,22 +471,47 @@ def two_tops_section(dev, args):
     for prec in ("fp64", "fp32", "bf16x3"):
         M, D, S = mb.build(prec)
         tm = {k: round(float(v), 3) for k, v in mb.timing().items()}
+        rec = {"build_ms": tm}
+        if prec == "fp32":
+            # build time and element error only (the sample propagates the
+            # FP64 and split-BF16 builds)
+            rec["elem_err"] = elem_err(M, M64)
+            rec["build_speedup_vs_fp64"] = round(out["fp64"]["build_ms"]["total_ms"] / tm["total_ms"], 3)
+            out[prec] = rec
+            log("two tops fp32: build %.1f ms (contract %.1f), elem err max %.2e" % (
+                tm["total_ms"], tm["contract_ms"], rec["elem_err"]["max"]))
+            continue
         prop, q, st, tps = propagate(1)
         sig = xsec_functional(prop, q, l, E)
-        rec = {"build_ms": tm, "batched_traj_per_s": round(tps, 2), "norm_err_max": float(st["norm_err"].max())}
+        rec.update({"batched_traj_per_s": round(tps, 3), "norm_err_max": float(st["norm_err"].max()),
+                    "rhs_mean": float(st["n_rhs"].mean())})
         if prec == "fp64":
             M64, sig64 = M, sig
-            _, qp, _, tpp = propagate(0, sample=min(args.c3_plain, n))
-            rec["plain_traj_per_s"] = round(tpp, 2)
-            rec["plain_max_amp_diff"] = float(np.max(np.abs(qp[:, :2 * prop.N] - q[:len(qp), :2 * prop.N])))
+            # the plain kernel on a bounded run: right-hand sides per second
+            pr = Propagator(dev, wl.basis, wl.R, mass_amu=wl.mass_amu, from_build=True)
+            pr.load_paths(paths)
+            rows_b = rows_for(pr, min(64, n), max_steps=args.c3_plain_steps)
+            t0 = time.perf_counter()
+            _, sp = pr.run(L.MQCT_PROP_AT_RK4, rows_b, batched=0)
+            pel = time.perf_counter() - t0
+            rhs_rate = float(sp["n_rhs"].sum()) / pel
+            rec["plain_rhs_per_s"] = round(rhs_rate, 1)
+            rec["plain_traj_per_s_est"] = round(rhs_rate / st["n_rhs"].mean(), 5)
+            rec["plain_note"] = ("plain FP64 kernel: %d trajectories x first %d steps in %.1f s, scaled by the "
+                                 "sample's mean right-hand sides per trajectory" % (len(rows_b),
+                                                                                    args.c3_plain_steps, pel))
+            rec["speedup_batched_vs_plain"] = round(tps / rec["plain_traj_per_s_est"], 2)
         else:
             rec["elem_err"] = elem_err(M, M64)
             rec["xsec_rel_err"], rec["channels"] = rel_err(sig, sig64)
+            rec["build_speedup_vs_fp64"] = round(out["fp64"]["build_ms"]["total_ms"] / tm["total_ms"], 3)
         out[prec] = rec
-        log("two tops %s: build %.1f ms (contract %.1f), batched %.1f traj/s%s" % (
+        log("two tops %s: build %.1f ms (contract %.1f), batched %.3f traj/s%s" % (
             prec, tm["total_ms"], tm["contract_ms"], tps,
-            "" if prec == "fp64" else ", xsec err %.2e, elem err max %.2e" % (rec["xsec_rel_err"],
-                                                                             rec["elem_err"]["max"])))
+            (", plain est %.4f traj/s" % rec["plain_traj_per_s_est"]) if prec == "fp64" else
+            ", xsec err %.2e, elem err max %.2e" % (rec["xsec_rel_err"], rec["elem_err"]["max"])))
+    if args.sur_steps <= 0:
+        return out
     # the learned potential: trained on the GPU, fills the V grid, split-BF16 build
     sur, stats = train_surrogate(dev, wl, n_samples=args.sur_samples, steps=args.sur_steps, batch=args.sur_batch,
                                  r_lo=args.sur_rmin, log=log)
@@ -497,7 +531,7 @@ def two_tops_section(dev, args):
     e, nch = rel_err(sig, sig64)
     stats.update({"grid_fill_s": round(fill_s, 4), "plain_pes_grid_fill_gpu_s": round(plain_fill_s, 4),
                   "grid_points": int(wl.grid.G) * len(wl.R), "build_ms": tm, "elem_err": elem_err(M, M64),
-                  "xsec_rel_err": e, "channels": nch, "batched_traj_per_s": round(tps, 2)})
+                  "xsec_rel_err": e, "channels": nch, "batched_traj_per_s": round(tps, 3)})
     out["surrogate_bf16x3"] = stats
     log("two tops surrogate: fit %.1f s, grid fill %.3f s, xsec err %.2e, elem err max %.2e" % (
         stats["fit_s"], fill_s, e, stats["elem_err"]["max"]))

@@ -178,7 +178,9 @@ typedef struct {
   double time_lim;     /* TIME_LIM (<=0: none) */
   int numb_oscl_prds;
   int numb_orb_prds;
-  int max_steps;       /* safety cap on outer steps (<=0: none) */
+  int max_steps;       /* safety cap on outer steps (<=0: none); the plain
+                        * form ends the trajectory there with fail = 2 (the
+                        * batched AT form runs every path to its end) */
 } mqct_traj_t;
 
 typedef struct {

@@ -76,7 +76,10 @@ struct BDev {
   const int *tile_blk;        // [ntile] block of the tile
   const int *tile_r0;         // [ntile] first row of the tile in its block
   const int *blk_tile0;       // [nblk] first tile of the block
-  const int *blk_order;       // [nblk] blocks by decreasing size
+  const int *blk_order;       // [npg] row groups of the pipelined forms by decreasing cost
+  const int *pg_blk;          // [npg] block of the row group (at most 128 rows of one block)
+  const int *pg_t0;           // [npg] its first 16-row tile inside the block
+  int npg;
   int *qctr;                  // item counter of the dynamically scheduled GEMM
   const long long *tile_off;  // [ntile] offset of the tile in a knot's KT
   int nrg;                    // 64-row groups over all blocks
@@ -717,8 +720,13 @@ struct PipeShared {
   double wy[GE], wy2[GE];
 };
 
+// Blocks of more than 128 states are cut into row groups of up to 128
+// rows (tiles tg .. tg + 7 of the block); every group of a block contracts
+// over the whole block (K = 2 nb), so the group's items read the chunk's
+// amplitudes once each.
 template <typename KT, int GE, int GKS = 16, bool M32 = false>
-__device__ void pipe_item(const BDev &d, int par, int r, int b, int c, PipeShared<KT, GE, GKS, M32> &sh) {
+__device__ void pipe_item(const BDev &d, int par, int r, int b, int tg, int c,
+                          PipeShared<KT, GE, GKS, M32> &sh) {
   using F = Frag<KT>;
   using FV = typename F::V;
   using P = PipeShape<KT, GE, GKS>;
@@ -756,13 +764,13 @@ __device__ void pipe_item(const BDev &d, int par, int r, int b, int c, PipeShare
     sh.wy2[tid] = wy2;
   }
   __syncthreads();
-  // the wave's tiles and their fragment streams
+  // the wave's tiles (of the row group) and their fragment streams
   const int ntl = (nb + 15) / 16;
   bool on[2];
   const FV *A[2];
 #pragma unroll
   for (int h = 0; h < 2; h++) {
-    const int tl = wave + 4 * h;
+    const int tl = tg + wave + 4 * h;
     on[h] = tl < ntl;
     const int g = d.blk_tile0[b] + (on[h] ? tl : 0);
     A[h] = reinterpret_cast<const FV *>(static_cast<const KT *>(d.KT) + (long long)r * d.KS + d.tile_off[g]) +
@@ -846,7 +854,7 @@ __device__ void pipe_item(const BDev &d, int par, int r, int b, int c, PipeShare
 #pragma unroll
   for (int h = 0; h < 2; h++) {
     if (!on[h]) continue;
-    const int rt0 = (wave + 4 * h) * 16;
+    const int rt0 = (tg + wave + 4 * h) * 16;
 #pragma unroll
     for (int ct = 0; ct < GC; ct++) {
       if (ct >= nct) break;
@@ -881,15 +889,15 @@ __global__ __launch_bounds__(BS, GE >= 64 ? 1 : 2) void gemm_pipe_kernel(BDev d,
   extern __shared__ int pre[];  // [n_r + 1] chunk prefix
   __shared__ int wtot[BS / 64];
   __shared__ PipeShared<KT, GE, 16, M32> sh;
-  const int nr = d.n_r, nblk = d.nblk;
-  const long long M = (long long)chunk_prefix(d, par, GE, pre, wtot) * nblk;
+  const int nr = d.n_r, npg = d.npg;
+  const long long M = (long long)chunk_prefix(d, par, GE, pre, wtot) * npg;
   const int X = 8, xcd = blockIdx.x % X, slot = blockIdx.x / X, Gx = gridDim.x / X;
   const long long lo = M * xcd / X, hi = M * (xcd + 1) / X;
   for (long long it = lo + slot; it < hi; it += Gx) {
-    // item -> (knot r, block b, chunk c)
-    const int r = chunk_knot(pre, nr, (int)(it / nblk)), cr = pre[r + 1] - pre[r];
-    const int loc = (int)(it - (long long)pre[r] * nblk), b = loc / cr, c = loc % cr;
-    pipe_item<KT, GE, 16, M32>(d, par, r, b, c, sh);
+    // item -> (knot r, row group g, chunk c)
+    const int r = chunk_knot(pre, nr, (int)(it / npg)), cr = pre[r + 1] - pre[r];
+    const int loc = (int)(it - (long long)pre[r] * npg), g = loc / cr, c = loc % cr;
+    pipe_item<KT, GE, 16, M32>(d, par, r, d.pg_blk[g], d.pg_t0[g], c, sh);
   }
 }
 
@@ -908,16 +916,16 @@ __global__ __launch_bounds__(BS, GE >= 64 ? 1 : (GE >= 32 ? 2 : 3)) void gemm_pi
   __shared__ int s_next;
   const int tid = threadIdx.x, nr = d.n_r;
   const int C = chunk_prefix(d, par, GE, pre, wtot);
-  const int M = C * d.nblk;
+  const int M = C * d.npg;
   if (tid == 0) s_next = atomicAdd(d.qctr, 1);
   __syncthreads();
   int it = s_next;
   while (it < M) {
     int nxt = 0;
     if (tid == 0) nxt = atomicAdd(d.qctr, 1);
-    const int b = d.blk_order[it / C], qc = it % C;
+    const int g = d.blk_order[it / C], qc = it % C;
     const int r = chunk_knot(pre, nr, qc), c = qc - pre[r];
-    pipe_item<KT, GE, GKS, M32>(d, par, r, b, c, sh);
+    pipe_item<KT, GE, GKS, M32>(d, par, r, d.pg_blk[g], d.pg_t0[g], c, sh);
     __syncthreads();  // everyone has read s_next
     if (tid == 0) s_next = nxt;
     __syncthreads();
@@ -1484,11 +1492,22 @@ int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *t
     }
   }
   const int ntile = (int)tile_blk.size();
-  std::vector<int> blk_order(nblk);
-  for (int b = 0; b < nblk; b++) blk_order[b] = b;
-  std::stable_sort(blk_order.begin(), blk_order.end(), [&](int x, int y) {
-    return blk_start[x + 1] - blk_start[x] > blk_start[y + 1] - blk_start[y];
-  });
+  // row groups of the pipelined forms: up to 128 rows (8 tiles) of a block;
+  // handed out by decreasing cost rows x nb
+  std::vector<int> pg_blk, pg_t0;
+  std::vector<long long> pg_cost;
+  for (int b = 0; b < nblk; b++) {
+    const int nb = blk_start[b + 1] - blk_start[b];
+    for (int t0 = 0; 16 * t0 < nb; t0 += 8) {
+      pg_blk.push_back(b);
+      pg_t0.push_back(t0);
+      pg_cost.push_back((long long)std::min(128, nb - 16 * t0) * nb);
+    }
+  }
+  const int npg = (int)pg_blk.size();
+  std::vector<int> blk_order(npg);
+  for (int g = 0; g < npg; g++) blk_order[g] = g;
+  std::stable_sort(blk_order.begin(), blk_order.end(), [&](int x, int y) { return pg_cost[x] > pg_cost[y]; });
   // 64-row groups of the blocks (the LDS-staged work queue)
   std::vector<int> rg_blk, rg_row0;
   for (int b = 0; b < nblk; b++)
@@ -1497,16 +1516,12 @@ int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *t
       rg_row0.push_back(r0);
     }
   const int nrg = (int)rg_blk.size();
-  if (maxnb > 128) {
-    // the pipelined form holds at most two 16-row tiles per wave (128 rows)
-    if (ds->gemm_fixed >= 0 && kShapes[ds->gemm_fixed].form >= kPipe)
-      return mqct_fail(MQCT_ERR_ARG, "batched: the pipelined GEMM form takes m-blocks of at most 128 states");
-  }
-  DevBuf d_tb, d_tr0, d_to, d_bt0, d_rgb, d_rgr, d_bord;
+  DevBuf d_tb, d_tr0, d_to, d_bt0, d_rgb, d_rgr, d_bord, d_pgb, d_pgt;
   if (up(d_tb, tile_blk.data(), ntile * 4) || up(d_tr0, tile_r0.data(), ntile * 4) ||
       up(d_to, tile_off.data(), ntile * 8) || up(d_bt0, blk_tile0.data(), nblk * 4) ||
       up(d_rgb, rg_blk.data(), nrg * 4) || up(d_rgr, rg_row0.data(), nrg * 4) ||
-      up(d_bord, blk_order.data(), nblk * 4))
+      up(d_bord, blk_order.data(), npg * 4) || up(d_pgb, pg_blk.data(), npg * 4) ||
+      up(d_pgt, pg_t0.data(), npg * 4))
     return mqct_fail(MQCT_ERR_NOMEM, "batched: tile tables");
   if (!ds->kt_valid || ds->kt_fp32 != knots_fp32) {
     const size_t need_kt = (size_t)n_r * KS * (knots_fp32 ? 4 : 8);
@@ -1557,6 +1572,9 @@ int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *t
   d.tile_r0 = d_tr0.as<int>();
   d.blk_tile0 = d_bt0.as<int>();
   d.blk_order = d_bord.as<int>();
+  d.pg_blk = d_pgb.as<int>();
+  d.pg_t0 = d_pgt.as<int>();
+  d.npg = npg;
   d.tile_off = d_to.as<long long>();
   d.nrg = nrg;
   d.rg_blk = d_rgb.as<int>();
@@ -1686,7 +1704,7 @@ int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *t
   // shapes set per level (the tuner's own choices always fit)
   for (int l = 0; l < DynState::kLevels; l++) {
     const int sh = ds->gemm_shape[pi_][l];
-    if (sh >= 0 && ((uses_queue(sh) && !queue_ok) || (kShapes[sh].form >= kPipe && maxnb > 128)))
+    if (sh >= 0 && uses_queue(sh) && !queue_ok)
       return mqct_fail(MQCT_ERR_ARG, "batched: a GEMM shape set for a level does not fit this matrix");
   }
   auto shape_for = [&](int &sh) -> int {
@@ -1706,7 +1724,7 @@ int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *t
       double tot[kNShapes] = {};
       bool keep[kNShapes];
       for (int s = 0; s < kNShapes; s++)
-        keep[s] = (queue_ok || !uses_queue(s)) && (kShapes[s].form < kPipe || maxnb <= 128);
+        keep[s] = queue_ok || !uses_queue(s);
       for (int round = 0; round < 3 && active > 0; round++) {
         for (int s = 0; s < ncand && active > 0; s++) {
           if (!keep[s]) continue;
@@ -1729,7 +1747,7 @@ int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *t
   };
   // every trajectory ends after at most MAXSTP accepted steps and a bounded
   // number of step-size reductions; the cap only guards the host loop
-  const int max_rounds = 1 << 17;
+  const int max_rounds = 1 << 20;  // a safety bound: 16.7 M iterations
   HIP_TRY(hipEventRecord(h->ev[5], h->stream));
   int shape = -1;
   if (int rc = shape_for(shape)) return rc;

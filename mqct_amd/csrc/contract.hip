@@ -556,28 +556,33 @@ __global__ __launch_bounds__(64 * NW) void contract_bf16x3_v_kernel(
         br[j] = sB[buf][0][row][k0 + j];
         bi[j] = has_im ? sB[buf][1][row][k0 + j] : 0.0f;
       }
+      // the split pair products of all the wave's a-states first, then each
+      // R tile's three weight fragments read from LDS once for all of them
+      bf16x8 p0[AWn], p1[AWn], p2[AWn];
 #pragma unroll
       for (int a = 0; a < AWn; a++) {
         const int al = wave * AWn + a;
-        bf16x8 p0, p1, p2;
         float pv[8];
 #pragma unroll
         for (int j = 0; j < 8; j++) {
           pv[j] = sA[buf][0][al][k0 + j] * br[j];
           if (has_im) pv[j] += sA[buf][1][al][k0 + j] * bi[j];
         }
-        split3_pack8(pv, p0, p1, p2);
+        split3_pack8(pv, p0[a], p1[a], p2[a]);
+      }
 #pragma unroll
-        for (int n = 0; n < NTB; n++) {
-          const bf16x8 w0 = *reinterpret_cast<const bf16x8 *>(&sW[buf][0][n * 16 + row][k0]);
-          const bf16x8 w1 = *reinterpret_cast<const bf16x8 *>(&sW[buf][1][n * 16 + row][k0]);
-          const bf16x8 w2 = *reinterpret_cast<const bf16x8 *>(&sW[buf][2][n * 16 + row][k0]);
-          acc[a][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p2, w0, acc[a][n], 0, 0, 0);
-          acc[a][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p1, w1, acc[a][n], 0, 0, 0);
-          acc[a][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p0, w2, acc[a][n], 0, 0, 0);
-          acc[a][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p1, w0, acc[a][n], 0, 0, 0);
-          acc[a][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p0, w1, acc[a][n], 0, 0, 0);
-          acc[a][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p0, w0, acc[a][n], 0, 0, 0);
+      for (int n = 0; n < NTB; n++) {
+        const bf16x8 w0 = *reinterpret_cast<const bf16x8 *>(&sW[buf][0][n * 16 + row][k0]);
+        const bf16x8 w1 = *reinterpret_cast<const bf16x8 *>(&sW[buf][1][n * 16 + row][k0]);
+        const bf16x8 w2 = *reinterpret_cast<const bf16x8 *>(&sW[buf][2][n * 16 + row][k0]);
+#pragma unroll
+        for (int a = 0; a < AWn; a++) {
+          acc[a][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p2[a], w0, acc[a][n], 0, 0, 0);
+          acc[a][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p1[a], w1, acc[a][n], 0, 0, 0);
+          acc[a][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p0[a], w2, acc[a][n], 0, 0, 0);
+          acc[a][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p1[a], w0, acc[a][n], 0, 0, 0);
+          acc[a][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p0[a], w1, acc[a][n], 0, 0, 0);
+          acc[a][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p0[a], w0, acc[a][n], 0, 0, 0);
         }
       }
     }

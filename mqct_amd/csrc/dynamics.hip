@@ -513,6 +513,8 @@ __global__ __launch_bounds__(BS) void traj_kernel(Dev d) {
       if (adia) {
         if (d.mode == MQCT_PROP_AT_RK4) tcur += dt;
         if (fail) break;
+        if (p.max_steps > 0 && nsteps >= p.max_steps && run) { run = false; fail = 2; }
+        vsync();
         continue;
       }
       double uR = y[2 * N], uphi = y[2 * N + 4];
