@@ -566,8 +566,15 @@ int mqct_dyn_run_plain(mqct_hip *h, int mode, int n_traj,
   const int N = ds->N;
   const int nv = 2 * N + 8;
   size_t lds = (size_t)N * 2 * sizeof(double2) + 32 * sizeof(double);
-  if (lds > 64 * 1024)
+  // gfx950 gives a workgroup up to 160 KiB of LDS; above 64 KiB the kernel
+  // has to be told so before the launch
+  int lds_max = 0;
+  HIP_TRY(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, h->device));
+  if (lds + 64 > (size_t)lds_max)
     return mqct_fail(MQCT_ERR_ARG, "propagate: the plain per-trajectory form holds 2N complex amplitudes in LDS; N too large (use batched=1)");
+  if (lds > 64 * 1024)
+    HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(&traj_kernel),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   int cus = 0;
   HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device));
   int grid = n_traj < 2 * cus ? n_traj : 2 * cus;

@@ -427,38 +427,31 @@ def test_at_batched_fp32_products(dev, shape):
     assert np.array_equal(s0["n_steps"], s1["n_steps"])
 
 
-@pytest.mark.parametrize("shape,knots", [(15, 1), (17, 1), (18, 1), (20, 1), (18, 2), (18, 3), (21, 3)])
-def test_at_batched_pipelined_blocks_over_128(dev, shape, knots):
-    """m-blocks above 128 states (the two-tops basis of the benchmark, 2043
-    states, m-blocks of up to 369, on a coarse 192-point grid with a smooth
-    potential of a few hundred cm^-1): the
-    pipelined forms cut such blocks into row groups of 128 rows that
-    contract over the whole block.  Against the LDS-staged batched form
-    (shape 0, checked against the plain kernel on every basis the plain
-    kernel holds): 1e-9 with FP64 coefficients, 1e-6 with FP32
-    coefficients or products, same step counts."""
+def _smooth_v(w):
+    """A smooth anisotropic potential of a few hundred cm^-1 on the grid of
+    w (the benchmark's site-site potential needs steps below 1 a.u.)."""
+    g = w.grid
+    b1 = g.xbm + g.xbl * np.asarray(g.xb)
+    b2 = g.xbbm + g.xbbl * np.asarray(g.xbb)
+    a2, g1, g2 = np.asarray(g.xaa), np.asarray(g.xg), np.asarray(g.xgg)
+    A2, B1, B2, G1, G2 = np.meshgrid(a2, b1, b2, g1, g2, indexing="ij")
+    ang = (1 + 0.5 * np.cos(B1) + 0.4 * np.cos(B2) + 0.3 * np.cos(A2) * np.sin(B1) * np.sin(B2)
+           + 0.2 * np.cos(2 * G1) + 0.2 * np.cos(G2))
+    rad = 300.0 * np.exp(-0.8 * (np.asarray(w.R) - 4.0))
+    return (rad[:, None, None, None, None, None] * ang[None]).ravel()
+
+
+@pytest.fixture(scope="module")
+def big_blocks(dev):
+    """The two-tops basis of the benchmark (2043 states, m-blocks of up to
+    369) on a coarse 192-point grid with the smooth potential, built in FP64,
+    with AT paths from a first step on j1, j2 <= 1 for 12 trajectories."""
     from mqct_amd.matrix import MatrixBuilder
-    from mqct_amd.propagate import at_first_step
-
-    def smooth_v(w):
-        # a smooth anisotropic potential of a few hundred cm^-1 (the
-        # benchmark's site-site potential needs steps below 1 a.u.)
-        g = w.grid
-        b1 = g.xbm + g.xbl * np.asarray(g.xb)
-        b2 = g.xbbm + g.xbbl * np.asarray(g.xbb)
-        a2, g1, g2 = np.asarray(g.xaa), np.asarray(g.xg), np.asarray(g.xgg)
-        A2, B1, B2, G1, G2 = np.meshgrid(a2, b1, b2, g1, g2, indexing="ij")
-        ang = (1 + 0.5 * np.cos(B1) + 0.4 * np.cos(B2) + 0.3 * np.cos(A2) * np.sin(B1) * np.sin(B2)
-               + 0.2 * np.cos(2 * G1) + 0.2 * np.cos(G2))
-        rad = 300.0 * np.exp(-0.8 * (np.asarray(w.R) - 4.0))
-        return (rad[:, None, None, None, None, None] * ang[None]).ravel()
-
     wl = W.two_asym_tops(W.C3_JMAX, W.C3_JMAX, emax=W.C3_EMAX, n_ang_each=(2, 4, 3, 2, 4), n_r=40)
     assert max(np.unique(wl.basis.m12, return_counts=True)[1]) > 256
     small = W.two_asym_tops(1, 1, n_ang_each=(2, 4, 3, 2, 4), n_r=40)
     bb, E = W.two_tops_at_batch(12, seed=41)
-    mb_small = MatrixBuilder(dev, small, vgrid=smooth_v(small))
-    mb_small.build("fp64", copy_out=False)
+    MatrixBuilder(dev, small, vgrid=_smooth_v(small)).build("fp64", copy_out=False)
     ps = Propagator(dev, small.basis, small.R, mass_amu=small.mass_amu, from_build=True)
     k = np.sqrt(ps.mass * 2.0 * E * CM_TO_HARTREE)
     l = np.floor(bb * k)
@@ -466,12 +459,32 @@ def test_at_batched_pipelined_blocks_over_128(dev, shape, knots):
     for i in range(len(bb)):
         rws += ps.trajectories([l[i]], float(E[i]), 1, time_step=20.0)
     _, _, paths = ps.run_record(rws, 20000)
-    MatrixBuilder(dev, wl, vgrid=smooth_v(wl)).build("fp64", copy_out=False)
+    return wl, paths, l, E
+
+
+def _big_propagator(dev, big_blocks):
+    from mqct_amd.matrix import MatrixBuilder
+    wl, paths, l, E = big_blocks
+    MatrixBuilder(dev, wl, vgrid=_smooth_v(wl)).build("fp64", copy_out=False)
     prop = Propagator(dev, wl.basis, wl.R, mass_amu=wl.mass_amu, from_build=True)
     prop.load_paths(paths)
     rows = []
-    for i in range(len(bb)):
+    for i in range(len(l)):
         rows += prop.trajectories([l[i]], float(E[i]), 1, time_step=20.0, at_paths=[i])
+    return wl, prop, rows
+
+
+@pytest.mark.parametrize("shape,knots", [(15, 1), (17, 1), (18, 1), (20, 1), (18, 2), (18, 3), (21, 3)])
+def test_at_batched_pipelined_blocks_over_128(dev, big_blocks, shape, knots):
+    """m-blocks above 128 states (the two-tops basis of the benchmark, 2043
+    states, m-blocks of up to 369, on a coarse 192-point grid with a smooth
+    potential of a few hundred cm^-1): the
+    pipelined forms cut such blocks into row groups of 128 rows that
+    contract over the whole block.  Against the LDS-staged batched form
+    (shape 0, checked against the plain kernel on the same basis below):
+    1e-9 with FP64 coefficients, 1e-6 with FP32 coefficients or products,
+    same step counts."""
+    wl, prop, rows = _big_propagator(dev, big_blocks)
     L.check(dev.lib.mqct_hip_batched_set_tile(dev.h, 0), "set_tile")
     try:
         q0, s0 = prop.run(L.MQCT_PROP_AT_RK4, rows, batched=1)
@@ -484,4 +497,24 @@ def test_at_batched_pipelined_blocks_over_128(dev, shape, knots):
     tol = 1e-9 if knots == 1 else 1e-6
     print(f"shape {shape} knots {knots}: amplitude diff {d:.2e}")
     assert d <= tol, f"shape {shape} knots {knots}: pipelined vs LDS-staged amplitude diff {d}"
+    assert np.array_equal(s0["n_steps"], s1["n_steps"])
+
+
+def test_plain_kernel_lds_above_64k(dev, big_blocks):
+    """The plain per-trajectory kernel on 2043 states keeps 2N complex
+    amplitudes in 65 632 bytes of LDS (above the 64 KiB a launch gets
+    without asking): against the LDS-staged batched form, 1e-9, same
+    step counts."""
+    wl, prop, rows = _big_propagator(dev, big_blocks)
+    N = wl.basis.n_states
+    assert 2 * N * 16 + 256 > 64 * 1024
+    L.check(dev.lib.mqct_hip_batched_set_tile(dev.h, 0), "set_tile")
+    try:
+        q0, s0 = prop.run(L.MQCT_PROP_AT_RK4, rows, batched=1)
+    finally:
+        L.check(dev.lib.mqct_hip_batched_set_tile(dev.h, -1), "set_tile")
+    q1, s1 = prop.run(L.MQCT_PROP_AT_RK4, rows, batched=0)
+    d = np.max(np.abs(q1[:, :2 * N] - q0[:, :2 * N]))
+    print(f"plain vs batched on {N} states: amplitude diff {d:.2e}")
+    assert d <= 1e-9, f"plain vs LDS-staged batched amplitude diff {d}"
     assert np.array_equal(s0["n_steps"], s1["n_steps"])
