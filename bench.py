@@ -109,6 +109,8 @@ def main():
                     help="skip the two-asymmetric-tops forms (builds, surrogate, trajectory sample)")
     ap.add_argument("--c3-traj", type=int, default=16,
                     help="two-tops AT trajectories (a sample of the 1024-trajectory batch)")
+    ap.add_argument("--exp-terms", type=int, default=1,
+                    help="also build the two-tops matrix through the PES expansion (0: skip)")
     ap.add_argument("--c3-plain-steps", type=int, default=40,
                     help="steps of the bounded plain-kernel run that gives its right-hand-side rate")
     ap.add_argument("--c3-dt", type=float, default=0.5,
@@ -510,6 +512,8 @@ def two_tops_section(dev, args):
             prec, tm["total_ms"], tm["contract_ms"], tps,
             (", plain est %.4f traj/s" % rec["plain_traj_per_s_est"]) if prec == "fp64" else
             ", xsec err %.2e, elem err max %.2e" % (rec["xsec_rel_err"], rec["elem_err"]["max"])))
+    if args.exp_terms:
+        out["expansion"] = expansion_section(dev, mb, wl, M64, out["fp64"]["build_ms"]["total_ms"])
     if args.sur_steps <= 0:
         return out
     # the learned potential: trained on the GPU, fills the V grid, split-BF16 build
@@ -536,6 +540,34 @@ def two_tops_section(dev, args):
     log("two tops surrogate: fit %.1f s, grid fill %.3f s, xsec err %.2e, elem err max %.2e" % (
         stats["fit_s"], fill_s, e, stats["elem_err"]["max"]))
     return out
+
+
+def expansion_section(dev, mb, wl, M64, fp64_build_ms):
+    """The PES-expansion route to the same two-tops matrix: the 312 terms of
+    L1, L2 <= 4 with even NJU1, NJU2 (both partners have a C2 axis) are
+    projected from the FP64 V grid on the GPU (CALC_EXPANSION_TERMS), then
+    the matrix is built from them (EXPANSION_MATRIX_ELEMENT: angular
+    coefficients, one GEMM over the terms, cutoff and spline).  elem_err is
+    the expansion's truncation against the quadrature build, not a
+    rounding error."""
+    from mqct_amd import expansion as X
+    terms = X.terms_type0(4, 4, du1=2, du2=2)
+    ex = X.Expansion(dev, wl, terms)
+    t0 = time.perf_counter()
+    ex.project()
+    proj_s = time.perf_counter() - t0
+    ex.build(copy_out=False)          # first build: CG table and buffers
+    M, D, S = ex.build()
+    t = {k: round(float(v), 3) for k, v in ex.timing().items()}
+    bt = {k: round(float(v), 3) for k, v in mb.timing().items()}
+    rec = {"terms": len(terms), "project_wall_s": round(proj_s, 3), "device_ms": t,
+           "build_ms": bt, "build_speedup_vs_fp64_quadrature": round(fp64_build_ms / bt["total_ms"], 2),
+           "elem_err_vs_quadrature": elem_err(M, M64)}
+    log("two tops expansion: %d terms, projection %.1f + %.1f ms, build %.1f ms (angular %.1f, GEMM %.1f) = "
+        "%.1fx the FP64 quadrature build; truncation vs quadrature max %.2e" % (
+            len(terms), t["project_terms_ms"], t["project_gemm_ms"], bt["total_ms"], t["angular_ms"],
+            t["gemm_ms"], rec["build_speedup_vs_fp64_quadrature"], rec["elem_err_vs_quadrature"]["max"]))
+    return rec
 
 
 def traffic_from_profiles(knots):

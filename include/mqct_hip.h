@@ -146,6 +146,49 @@ int mqct_hip_mij_spline(mqct_hip_t *h, int n_r, const double *R_com,
                         int64_t total_size, double *mat_el, double *mat_el_der);
 
 /* ---------------------------------------------------------------------- */
+/* PES expansion.  The potential as a sum of analytic angular terms with
+ * R-dependent coefficients: the term list A_TOP (CALC_EXPANSION,
+ * pot_comp.f:3898-4693, or READ_EXPANSION_TERMS' TERMS= list,
+ * pot_comp.f:6276-6341), the projection of the V grid on the terms
+ * (CALC_EXPANSION_TERMS, pot_comp.f:4695-5920), and the matrix elements
+ * from the coefficients (EXPANSION_MATRIX_ELEMENT, pot_comp.f:6465-8751, as
+ * MATRIX_MAKE calls it with expansion_defined, matrix_ini.f:1950-1956).
+ * Distinguishable partners (types 1, 4 and 0 without IDENTICAL).          */
+
+/* a_top: n_terms rows of five ints, A_TOP(1:5, t): type 1 (L, 0, 0, 0, 0),
+ * type 4 (L, NJU, 0, 0, 0), type 0 (L1, NJU1, L2, NJU2, L).  Rejected as
+ * the reference rejects them: negative NJU (type 4) or NJU1, NJU1 = 0 with
+ * NJU2 < 0, a broken triangle (L1, L2, L), a term given twice (also as its
+ * (-NJU1, -NJU2) mirror).  Needs the basis; clears earlier coefficients. */
+int mqct_hip_set_expansion_terms(mqct_hip_t *h, int n_terms, const int *a_top);
+
+/* CALC_EXPANSION_TERMS on the V grid set by mqct_hip_set_vgrid or
+ * mqct_hip_eval_pes: expansion_terms(n_r, n_terms), column-major as the
+ * reference keeps it (terms[t * n_r + ir]); the coefficients stay on the
+ * device for mqct_hip_expansion_build.  terms may be NULL. */
+int mqct_hip_expansion_project(mqct_hip_t *h, double *terms);
+
+/* The matrix from expansion coefficients: Mat_el = conv_unit_e *
+ * EXPANSION_MATRIX_ELEMENT, then the cutoff and clamped spline of
+ * mqct_hip_matrix_build (same arguments and outputs; the result also stays
+ * on the device for mqct_hip_use_built_matrix).  terms != NULL: the caller's
+ * coefficients on R_com (n_r points), as READ_EXPANSION_TERMS reads them
+ * from PES_EXPAN_TERMS.DAT; terms == NULL: those of the last projection
+ * (R_com may then be NULL).  mqct_hip_build_timing reports [0] angular
+ * coefficients, [1] 0, [2] the coefficient GEMM, [3] cutoff + spline. */
+int mqct_hip_expansion_build(mqct_hip_t *h, int n_r, const double *R_com,
+                             const double *terms, int64_t total_size,
+                             const int *ind_mat, double conv_unit_e,
+                             double mij_zero_cut, int ir_cut1, int ir_cut2,
+                             double *mat_el, double *mat_el_der,
+                             uint8_t *skipped);
+
+/* Device milliseconds of the expansion path: [0] term functions and
+ * weights of the last projection, [1] its GEMM and reduction, [2] angular
+ * coefficients of the last expansion build, [3] its coefficient GEMM. */
+int mqct_hip_expansion_timing(mqct_hip_t *h, double *ms4);
+
+/* ---------------------------------------------------------------------- */
 /* Propagation: replaces the stepping of DOTRAJECT (propagator.f:3205-4137)
  * with DERIVS_BK / DERIVS_BK_adia (mqct_eqtns.f:112-499, 720-879), the
  * spline interpolation potential_data_bk / splint_bk (mqct_eqtns.f:501-611)

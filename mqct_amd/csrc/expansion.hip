@@ -1,0 +1,648 @@
+// expansion.hip -- the PES-expansion path.
+//
+// The reference can represent the potential by its projections on analytic
+// angular functions (CALC_EXPANSION_TERMS, pot_comp.f:4695-5920; written to
+// and read back from PES_EXPAN_TERMS.DAT by READ_EXPANSION_TERMS,
+// pot_comp.f:6276-6395) and then builds every matrix element as
+//   M_k(R) = conv * f_k * sum_t A_kt V_t(R)
+// (EXPANSION_MATRIX_ELEMENT, pot_comp.f:6465-8751, called by MATRIX_MAKE at
+// matrix_ini.f:1950-1956; the cutoff and spline that follow are those of the
+// quadrature branch).  Here:
+//   projection  V_t(R) = s * sum_g F_t(g) w_g V(R, g): the term functions on
+//               the grid (F, [n_terms][G]; type 0 from per-molecule Wigner
+//               tables), then a split-K f64 MFMA GEMM against W = w V
+//               ([n_r][G], the quadrature branch's weights) and a reduction
+//               over the K chunks in fixed order;
+//   angular     A_kt, R-independent, one thread per (pair, term), with the
+//               Clebsch-Gordan coefficients read from a table built once on
+//               the host;
+//   matrix      M = conv * A V_t as an f64 MFMA GEMM (16 pairs x 128 R
+//               points per wave), then the shared cutoff and clamped spline.
+// Term rows are A_TOP(1:5, t): type 1 (L), type 4 (L, NJU), type 0
+// (L1, NJU1, L2, NJU2, L).  Distinguishable partners only.
+#include "mqct_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace {
+
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+inline unsigned nblk(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+__device__ inline double sgnp(int e) { return (e & 1) ? -1.0 : 1.0; }
+
+// Wigner D as the reference's Djkm (pot_comp.f:256-309):
+// d^j_{k m}(beta) exp(i (k gamma + m alpha)).
+__device__ inline double2 djkm(int j, int k, int m, double alpha, double beta,
+                               double gamma) {
+  double d = mq_wigner_d(j, k, m, cos(beta), sin(beta / 2.0), cos(beta / 2.0));
+  double ph = k * gamma + m * alpha;
+  return make_double2(d * cos(ph), d * sin(ph));
+}
+
+// ---- term functions on the grid (projection operand) ----------------------
+
+// Types 1 and 4: F[t][g], g = ib * ng + ig.  EXPANSION_TERM cases 1 and 4
+// (pot_comp.f:3756-3766): WF_DIAT_TOP(l, nju) (basis_wave_functions.f:
+// 153-175), times (-1)^nju for type 4.
+__global__ void fgrid14_kernel(int coll_type, int n_terms, const int *__restrict__ atop,
+                               const double *__restrict__ beta, int nb,
+                               const double *__restrict__ gamma, int ng,
+                               double *__restrict__ F) {
+  int64_t G = (int64_t)nb * ng;
+  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)n_terms * G) return;
+  int t = (int)(idx / G);
+  int64_t g = idx % G;
+  int ib = (int)(g / ng), ig = (int)(g % ng);
+  int l = atop[5 * t];
+  int nju = coll_type == 4 ? atop[5 * t + 1] : 0;
+  int an = nju < 0 ? -nju : nju;
+  double p = mq_plgndr(l, an, cos(beta[ib]));
+  double nrm = sqrt((2.0 * l + 1.0) / 4.0 / MQCT_PI *
+                    exp(mq_logfact(l - an) - mq_logfact(l + an)));
+  double v = p * nrm * (coll_type == 4 ? cos(nju * gamma[ig]) : 1.0);
+  if (nju < 0) v *= sgnp(nju);
+  if (coll_type == 4) v *= sgnp(nju);
+  F[idx] = v;
+}
+
+// Type 0 one-molecule tables, rows (t, m) with m = -lm..lm:
+// T1 = sqrt((2 l1+1)/8pi^2) Djkm(l1, nju1, m; 0, beta1, gamma1) on (b1, g1);
+// T2 = CG(l1 l2 l; m -m 0) sqrt((2 l2+1)/8pi^2) Djkm(l2, nju2, -m; alpha2,
+// beta2, gamma2) on (a2, b2, g2).  Rows with |m| > min(l1, l2) are zero.
+__global__ void tab0_kernel(int which, int n_terms, int lm,
+                            const int *__restrict__ atop,
+                            const double *__restrict__ cgl,  // [t][2lm+1]
+                            const double *__restrict__ a_ang, int na,
+                            const double *__restrict__ b_ang, int nb,
+                            const double *__restrict__ g_ang, int ngm,
+                            double2 *__restrict__ tab) {
+  const int nm = 2 * lm + 1;
+  int64_t Gm = (int64_t)na * nb * ngm;
+  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)n_terms * nm * Gm) return;
+  int64_t g = idx % Gm;
+  int m = (int)((idx / Gm) % nm) - lm;
+  int t = (int)(idx / (Gm * nm));
+  const int *a = atop + 5 * t;
+  int l1 = a[0], n1 = a[1], l2 = a[2], n2 = a[3];
+  int mm = l1 < l2 ? l1 : l2;
+  double2 out = make_double2(0.0, 0.0);
+  if (m >= -mm && m <= mm) {
+    int ia = (int)(g / ((int64_t)nb * ngm));
+    int ib = (int)((g / ngm) % nb);
+    int ig = (int)(g % ngm);
+    if (which == 1) {
+      double s = sqrt((2.0 * l1 + 1.0) / 8.0 / (MQCT_PI * MQCT_PI));
+      double2 d = djkm(l1, n1, m, 0.0, b_ang[ib], g_ang[ig]);
+      out = make_double2(d.x * s, d.y * s);
+    } else {
+      double s = sqrt((2.0 * l2 + 1.0) / 8.0 / (MQCT_PI * MQCT_PI)) *
+                 cgl[(size_t)t * nm + (m + lm)];
+      double2 d = djkm(l2, n2, -m, a_ang[ia], b_ang[ib], g_ang[ig]);
+      out = make_double2(d.x * s, d.y * s);
+    }
+  }
+  tab[idx] = out;
+}
+
+// Type 0 term functions, EXPANSION_TERM case 0 (pot_comp.f:3877-3895), real
+// part: F[t][g] = sum_m Re(T1 T2), g in the quadrature order (a2, b1, b2,
+// g1, g2).
+__global__ void fgrid0_kernel(int n_terms, int lm, const double2 *__restrict__ T1,
+                              const double2 *__restrict__ T2, int na2, int nb1,
+                              int nb2, int ng1, int ng2, double *__restrict__ F) {
+  const int nm = 2 * lm + 1;
+  int64_t G = (int64_t)na2 * nb1 * nb2 * ng1 * ng2;
+  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)n_terms * G) return;
+  int t = (int)(idx / G);
+  int64_t g = idx % G;
+  int ig2 = (int)(g % ng2);
+  int ig1 = (int)((g / ng2) % ng1);
+  int ib2 = (int)((g / ((int64_t)ng2 * ng1)) % nb2);
+  int ib1 = (int)((g / ((int64_t)ng2 * ng1 * nb2)) % nb1);
+  int ia2 = (int)(g / ((int64_t)ng2 * ng1 * nb2 * nb1));
+  int64_t G1 = (int64_t)nb1 * ng1, G2 = (int64_t)na2 * nb2 * ng2;
+  int64_t g1 = (int64_t)ib1 * ng1 + ig1;
+  int64_t g2 = ((int64_t)ia2 * nb2 + ib2) * ng2 + ig2;
+  double wr = 0.0;
+  for (int m = 0; m < nm; m++) {
+    double2 a = T1[((int64_t)t * nm + m) * G1 + g1];
+    double2 b = T2[((int64_t)t * nm + m) * G2 + g2];
+    wr += a.x * b.x - a.y * b.y;
+  }
+  F[idx] = wr;
+}
+
+// ---- projection GEMM --------------------------------------------------------
+// part[c][t][r] = sum_{g in chunk c} F[t][g] W[r][g].  A block of four waves
+// takes four 16-term tiles of one chunk (they share the chunk's W lines in
+// the CU cache); each wave runs the R points in passes of 128 (8 f64
+// 16x16x4 MFMA tiles).  Operand layout of f64 16x16x4: A[row = lane%16][k =
+// lane/16], B[k = lane/16][col = lane%16]; accumulator element i holds row
+// lane/16 + 4i, column lane%16.
+constexpr int PROJ_NT = 8;
+
+__global__ __launch_bounds__(256) void proj_gemm_kernel(
+    int n_terms, int ntp16, int64_t G, int n_r, int64_t chunk,
+    const double *__restrict__ F, const double *__restrict__ W,
+    double *__restrict__ part) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int tt = blockIdx.x * 4 + wave;
+  if (tt * 16 >= ntp16) return;
+  const int c = blockIdx.y;
+  const int64_t g0 = (int64_t)c * chunk;
+  const int64_t g1 = g0 + chunk < G ? g0 + chunk : G;
+  const int row = lane & 15, kq = lane >> 4;
+  const int t = tt * 16 + row;
+  const double *Frow = F + (int64_t)(t < n_terms ? t : 0) * G;
+  for (int r0 = 0; r0 < n_r; r0 += 16 * PROJ_NT) {
+    dbl4 acc[PROJ_NT];
+#pragma unroll
+    for (int n = 0; n < PROJ_NT; n++) acc[n] = dbl4{0, 0, 0, 0};
+    for (int64_t g = g0; g < g1; g += 4) {
+      const int64_t gk = g + kq;
+      const bool ok = gk < g1;
+      double a = (ok && t < n_terms) ? Frow[gk] : 0.0;
+#pragma unroll
+      for (int n = 0; n < PROJ_NT; n++) {
+        int r = r0 + n * 16 + row;
+        double b = (ok && r < n_r) ? W[(int64_t)r * G + gk] : 0.0;
+        acc[n] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[n], 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int n = 0; n < PROJ_NT; n++) {
+      int r = r0 + n * 16 + (lane & 15);
+      if (r >= n_r) continue;
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        int tr = tt * 16 + (lane >> 4) + 4 * i;
+        part[((int64_t)c * ntp16 + tr) * n_r + r] = acc[n][i];
+      }
+    }
+  }
+}
+
+// V_t(R) = s * sum_c part[c][t][r], chunks in order.
+__global__ void proj_reduce_kernel(int n_terms, int ntp16, int n_r, int nchunk,
+                                   double s, const double *__restrict__ part,
+                                   double *__restrict__ P) {
+  int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n_terms * n_r) return;
+  int t = idx / n_r, r = idx % n_r;
+  double acc = 0.0;
+  for (int c = 0; c < nchunk; c++) acc += part[((int64_t)c * ntp16 + t) * n_r + r];
+  P[idx] = s * acc;
+}
+
+// ---- angular coefficients ---------------------------------------------------
+struct AngCtx {
+  int coll_type, n_terms, ntp;
+  int64_t T;
+  const int2 *pairs;    // (stp, stpp) 0-based: ind_mat(1,k), ind_mat(2,k)
+  const int *j12, *m12, *chan;  // per state; chan 0-based
+  const int *chj;       // [2][n_ch]: j1, j2 per channel
+  int n_ch, ldvec;
+  const double *eig1, *eig2;
+  const int *atop;
+  const double *cgt;    // CG table
+  int ca, cc;           // first/second argument max, third argument max
+};
+
+__device__ inline double cgtab(const AngCtx &x, int j1, int j2, int J, int m1, int m2) {
+  if (j1 > x.ca || j2 > x.ca || J > x.cc) return 0.0;
+  int m3 = m1 + m2;
+  if (m1 < -j1 || m1 > j1 || m2 < -j2 || m2 > j2 || m3 < -J || m3 > J) return 0.0;
+  const int w = 2 * x.ca + 1;
+  int64_t i = (((int64_t)j1 * (x.ca + 1) + j2) * (x.cc + 1) + J) * w * w +
+              (int64_t)(m1 + x.ca) * w + (m2 + x.ca);
+  return x.cgt[i];
+}
+
+__device__ inline bool triang(int a, int b, int c) {
+  int m = a > b ? a : b;
+  if (c > m) m = c;
+  return 2 * m <= a + b + c;
+}
+
+// A[k][t] including the per-pair factor: EXPANSION_MATRIX_ELEMENT case 1
+// (pot_comp.f:6551-6600), case 4 (6955-7048), case 0 for distinguishable
+// partners (8257-8412, 8541-8716).
+__global__ void angular_kernel(AngCtx x, double *__restrict__ A) {
+  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= x.T * x.ntp) return;
+  const int64_t k = idx / x.ntp;
+  const int t = (int)(idx % x.ntp);
+  if (t >= x.n_terms) { A[idx] = 0.0; return; }
+  const int2 pr = x.pairs[k];
+  const int stp = pr.x, stpp = pr.y;
+  const int chp = x.chan[stp], chpp = x.chan[stpp];
+  const int m = x.m12[stp];
+  const int *a = x.atop + 5 * t;
+  double out = 0.0;
+  if (x.coll_type == 1 || x.coll_type == 4) {
+    const int jp = x.chj[chp], jpp = x.chj[chpp];
+    const double coef1 = sqrt((2.0 * jp + 1.0) / (2.0 * jpp + 1.0));
+    const int l = a[0];
+    if (triang(jpp, l, jp)) {
+      const double coef2 = sqrt((2.0 * l + 1.0) / (4.0 * MQCT_PI));
+      const double cgm = cgtab(x, jp, l, jpp, m, 0);
+      if (x.coll_type == 1) {
+        out = coef1 * coef2 * cgm * cgtab(x, jp, l, jpp, 0, 0);
+      } else {
+        const double *cp = x.eig1 + (size_t)chp * x.ldvec + jp;
+        const double *cpp = x.eig1 + (size_t)chpp * x.ldvec + jpp;
+        int nju = a[1];
+        double s = 0.0;
+        const int npl = 2 - (nju == 0);
+        for (int pl = 1; pl <= npl; pl++) {
+          double phase = 1.0;
+          if (pl == 2) { phase = sgnp(nju); nju = -nju; }
+          for (int kp = -jp; kp <= jp; kp++) {
+            int kpp = kp + nju;
+            if (kpp < -jpp || kpp > jpp) continue;
+            s += phase * sgnp(nju) * cgtab(x, jp, l, jpp, kp, nju) * cp[kp] * cpp[kpp];
+          }
+        }
+        out = coef1 * coef2 * cgm * s;
+      }
+    }
+  } else {
+    const int j1p = x.chj[chp], j2p = x.chj[x.n_ch + chp];
+    const int j1pp = x.chj[chpp], j2pp = x.chj[x.n_ch + chpp];
+    const int j12p = x.j12[stp], j12pp = x.j12[stpp];
+    const int l1 = a[0], l2 = a[2], l = a[4];
+    int n1 = a[1], n2 = a[3];
+    if (triang(j1pp, l1, j1p) && triang(j2pp, l2, j2p)) {
+      const double *c1p = x.eig1 + (size_t)chp * x.ldvec + j1p;
+      const double *c1pp = x.eig1 + (size_t)chpp * x.ldvec + j1pp;
+      const double *c2p = x.eig2 + (size_t)chp * x.ldvec + j2p;
+      const double *c2pp = x.eig2 + (size_t)chpp * x.ldvec + j2pp;
+      double cgjk = 0.0;
+      const int npl = 2 - (n1 == 0 && n2 == 0);
+      for (int pl = 1; pl <= npl; pl++) {
+        double sc = 1.0;
+        if (pl == 2) { sc = sgnp(l1 + l2 + l + n1 + n2); n1 = -n1; n2 = -n2; }
+        double db1 = 0.0, db2 = 0.0;
+        for (int k1p = -j1p; k1p <= j1p; k1p++) {
+          int k1pp = k1p - n1;
+          if (k1pp < -j1pp || k1pp > j1pp) continue;
+          db1 += cgtab(x, j1pp, l1, j1p, k1pp, n1) * c1p[k1p] * c1pp[k1pp];
+        }
+        for (int k2p = -j2p; k2p <= j2p; k2p++) {
+          int k2pp = k2p - n2;
+          if (k2pp < -j2pp || k2pp > j2pp) continue;
+          db2 += cgtab(x, j2pp, l2, j2p, k2pp, n2) * c2p[k2p] * c2pp[k2pp];
+        }
+        cgjk += db1 * db2 * sc;
+      }
+      double m5 = 0.0;
+      if (cgjk != 0.0) {
+        const int mx = l1 < l2 ? l1 : l2;
+        for (int mp = -j1p; mp <= j1p; mp++) {
+          if (m - mp < -j2p || m - mp > j2p) continue;
+          const double cg_p = cgtab(x, j1p, j2p, j12p, mp, m - mp);
+          double m1 = 0.0;
+          for (int me = -mx; me <= mx; me++) {
+            int mpp = mp - me;
+            if (mpp < -j1pp || mpp > j1pp) continue;
+            if (m - mpp < -j2pp || m - mpp > j2pp) continue;
+            double cg_pp = cgtab(x, j1pp, j2pp, j12pp, mpp, m - mpp);
+            double ca = cgtab(x, j1pp, l1, j1p, mpp, me);
+            double cb = cgtab(x, j2pp, l2, j2p, m - mpp, -me);
+            m1 += cgtab(x, l1, l2, l, me, -me) * cg_pp * ca * cb;
+          }
+          m5 += m1 * cg_p * cgjk;
+        }
+      }
+      const double coef1 = sqrt((2.0 * j1pp + 1.0) / (2.0 * j1p + 1.0));
+      const double coef2 = sqrt((2.0 * j2pp + 1.0) / (2.0 * j2p + 1.0));
+      const double coef3 = sqrt((2.0 * l1 + 1.0) * (2.0 * l2 + 1.0)) / (8.0 * MQCT_PI * MQCT_PI);
+      out = coef1 * coef2 * coef3 * m5;
+    }
+  }
+  A[idx] = out;
+}
+
+// ---- matrix GEMM ------------------------------------------------------------
+// M[k][r] = conv * sum_t A[k][t] P[t][r]: one wave per 16 pairs, R points in
+// passes of 128.
+constexpr int MAT_NT = 8;
+
+__global__ __launch_bounds__(256) void exp_matrix_kernel(
+    int64_t T, int ntp, int n_terms, int n_r, const double *__restrict__ A,
+    const double *__restrict__ P, double conv, double *__restrict__ M) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int64_t k0 = ((int64_t)blockIdx.x * 4 + wave) * 16;
+  if (k0 >= T) return;
+  const int row = lane & 15, kq = lane >> 4;
+  const int64_t ka = k0 + row;
+  const double *Arow = A + (ka < T ? ka : 0) * ntp;
+  for (int r0 = 0; r0 < n_r; r0 += 16 * MAT_NT) {
+    dbl4 acc[MAT_NT];
+#pragma unroll
+    for (int n = 0; n < MAT_NT; n++) acc[n] = dbl4{0, 0, 0, 0};
+    for (int kk = 0; kk < ntp; kk += 4) {
+      const int t = kk + kq;
+      double a = ka < T ? Arow[t] : 0.0;
+#pragma unroll
+      for (int n = 0; n < MAT_NT; n++) {
+        int r = r0 + n * 16 + row;
+        double b = (t < n_terms && r < n_r) ? P[(int64_t)t * n_r + r] : 0.0;
+        acc[n] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[n], 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int n = 0; n < MAT_NT; n++) {
+      int r = r0 + n * 16 + (lane & 15);
+      if (r >= n_r) continue;
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        int64_t k = k0 + (lane >> 4) + 4 * i;
+        if (k < T) M[k * n_r + r] = conv * acc[n][i];
+      }
+    }
+  }
+}
+
+int check_terms(int coll_type, int n, const int *a) {
+  // the checks CALC_EXPANSION_TERMS / EXPANSION_MATRIX_ELEMENT make before
+  // using a term list (pot_comp.f:5060-5072, 5563-5612, 6978-6995)
+  auto row = [&](int i) {
+    std::string s;
+    int w = coll_type == 1 ? 1 : coll_type == 4 ? 2 : 5;
+    for (int j = 0; j < w; j++) s += (j ? "," : "") + std::to_string(a[5 * i + j]);
+    return s;
+  };
+  for (int i = 0; i < n; i++) {
+    const int *t = a + 5 * i;
+    if (t[0] < 0 || (coll_type == 0 && (t[2] < 0 || t[4] < 0)))
+      return mqct_fail(MQCT_ERR_ARG, "set_expansion_terms: negative L in term " + row(i));
+    if (coll_type == 4 && t[1] < 0)
+      return mqct_fail(MQCT_ERR_ARG, "The values of M should be always positive. Please check the expansion term: " + row(i));
+    if (coll_type == 0) {
+      if (2 * std::max(std::max(t[0], t[2]), t[4]) > t[0] + t[2] + t[4])
+        return mqct_fail(MQCT_ERR_ARG, "TRINAGULAR RULE FAILED for the expansion term: " + row(i));
+      if (t[1] < 0)
+        return mqct_fail(MQCT_ERR_ARG, "The values of M1 should be always positive. Please check the expansion term: " + row(i));
+      if (t[1] == 0 && t[3] < 0)
+        return mqct_fail(MQCT_ERR_ARG, "The values of M2 should be positive only for M1 = 0. Please check the expansion term: " + row(i));
+      if (t[1] > t[0] || std::abs(t[3]) > t[2])
+        return mqct_fail(MQCT_ERR_ARG, "set_expansion_terms: |NJU| above L in term " + row(i));
+      for (int j = 0; j < n; j++) {
+        if (j == i) continue;
+        const int *u = a + 5 * j;
+        for (int pl = 0; pl < (t[1] == 0 && t[3] == 0 ? 1 : 2); pl++) {
+          int s = pl ? -1 : 1;
+          if (u[0] == t[0] && u[1] == s * t[1] && u[2] == t[2] && u[3] == s * t[3] && u[4] == t[4])
+            return mqct_fail(MQCT_ERR_ARG, "Found two same terms. Please check the expansion terms: " + row(i) + " " + row(j));
+        }
+      }
+    }
+  }
+  return MQCT_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+extern "C" int mqct_hip_set_expansion_terms(mqct_hip_t *h, int n_terms, const int *a_top) {
+  if (!h || n_terms <= 0 || !a_top) return mqct_fail(MQCT_ERR_ARG, "set_expansion_terms: empty term list");
+  if (!h->basis_ok) return mqct_fail(MQCT_ERR_STATE, "set_expansion_terms: call set_basis first");
+  int rc = check_terms(h->coll_type, n_terms, a_top);
+  if (rc) return rc;
+  HIP_TRY(hipSetDevice(h->device));
+  h->n_terms = n_terms;
+  h->atop.assign(a_top, a_top + 5 * (size_t)n_terms);
+  if (h->coll_type != 0)
+    for (int i = 0; i < n_terms; i++)
+      for (int j = (h->coll_type == 4 ? 2 : 1); j < 5; j++) h->atop[5 * i + j] = 0;
+  if ((rc = upload(h, h->d_atop, h->atop.data(), h->atop.size()))) return rc;
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  h->terms_nr = 0;
+  return MQCT_OK;
+}
+
+extern "C" int mqct_hip_expansion_project(mqct_hip_t *h, double *terms) {
+  if (!h) return mqct_fail(MQCT_ERR_ARG, "expansion_project: null handle");
+  if (h->n_terms <= 0) return mqct_fail(MQCT_ERR_STATE, "expansion_project: call set_expansion_terms first");
+  if (!h->grid_ok || !h->v_ok) return mqct_fail(MQCT_ERR_STATE, "expansion_project: grid and potential must be set");
+  HIP_TRY(hipSetDevice(h->device));
+  const int nt = h->n_terms, n_r = h->n_r;
+  const int64_t G = h->G;
+  const int ntp16 = (nt + 15) / 16 * 16;
+  if (h->d_expF.ensure((size_t)nt * G * 8)) return mqct_fail(MQCT_ERR_NOMEM, "expansion_project: term functions do not fit in HBM");
+  HIP_TRY(hipEventRecord(h->ev[0], h->stream));
+  if (h->coll_type == 0) {
+    int lm = 0;
+    for (int i = 0; i < nt; i++) lm = std::max(lm, std::min(h->atop[5 * i], h->atop[5 * i + 2]));
+    const int nm = 2 * lm + 1;
+    std::vector<double> cgl((size_t)nt * nm, 0.0);
+    for (int i = 0; i < nt; i++) {
+      const int *a = &h->atop[5 * i];
+      int mm = std::min(a[0], a[2]);
+      for (int m = -mm; m <= mm; m++) cgl[(size_t)i * nm + m + lm] = mq_cg(a[0], a[2], a[4], m, -m, 0);
+    }
+    int rc;
+    if ((rc = upload(h, h->d_expcgl, cgl.data(), cgl.size()))) return rc;
+    int64_t G1 = (int64_t)h->n_beta * h->n_gamma;
+    int64_t G2 = (int64_t)h->n_alpha2 * h->n_beta2 * h->n_gamma2;
+    if (h->d_expT1.ensure((size_t)nt * nm * G1 * 16) || h->d_expT2.ensure((size_t)nt * nm * G2 * 16))
+      return mqct_fail(MQCT_ERR_NOMEM, "expansion_project: term tables");
+    tab0_kernel<<<nblk((int64_t)nt * nm * G1, 256), 256, 0, h->stream>>>(
+        1, nt, lm, h->d_atop.as<int>(), h->d_expcgl.as<double>(), h->d_alpha2.as<double>(), 1,
+        h->d_beta.as<double>(), h->n_beta, h->d_gamma.as<double>(), h->n_gamma,
+        h->d_expT1.as<double2>());
+    tab0_kernel<<<nblk((int64_t)nt * nm * G2, 256), 256, 0, h->stream>>>(
+        2, nt, lm, h->d_atop.as<int>(), h->d_expcgl.as<double>(), h->d_alpha2.as<double>(), h->n_alpha2,
+        h->d_beta2.as<double>(), h->n_beta2, h->d_gamma2.as<double>(), h->n_gamma2,
+        h->d_expT2.as<double2>());
+    fgrid0_kernel<<<nblk((int64_t)nt * G, 256), 256, 0, h->stream>>>(
+        nt, lm, h->d_expT1.as<double2>(), h->d_expT2.as<double2>(), h->n_alpha2, h->n_beta,
+        h->n_beta2, h->n_gamma, h->n_gamma2, h->d_expF.as<double>());
+  } else {
+    fgrid14_kernel<<<nblk((int64_t)nt * G, 256), 256, 0, h->stream>>>(
+        h->coll_type, nt, h->d_atop.as<int>(), h->d_beta.as<double>(), h->n_beta,
+        h->d_gamma.as<double>(), h->coll_type == 4 ? h->n_gamma : 1, h->d_expF.as<double>());
+  }
+  HIP_TRY(hipGetLastError());
+  int rc = mqct_build_weights(h, MQCT_PREC_FP64);
+  if (rc) return rc;
+  HIP_TRY(hipEventRecord(h->ev[1], h->stream));
+  // K chunks: enough blocks to fill the chip, at least 1024 points each
+  const int ttiles = ntp16 / 16;
+  int64_t want = std::max<int64_t>(1, 2048 / ((ttiles + 3) / 4));
+  int64_t chunk = std::max<int64_t>(1024, (G + want - 1) / want);
+  chunk = (chunk + 3) / 4 * 4;
+  const int nchunk = (int)((G + chunk - 1) / chunk);
+  if (h->d_exppart.ensure((size_t)nchunk * ntp16 * n_r * 8) || h->d_terms.ensure((size_t)nt * n_r * 8))
+    return mqct_fail(MQCT_ERR_NOMEM, "expansion_project: partial sums");
+  dim3 grid((ttiles + 3) / 4, nchunk);
+  proj_gemm_kernel<<<grid, 256, 0, h->stream>>>(nt, ntp16, G, n_r, chunk, h->d_expF.as<double>(),
+                                                h->d_W64.as<double>(), h->d_exppart.as<double>());
+  // the type-4 projection has no 2 pi factor (pot_comp.f:5099); the
+  // quadrature weights w_g carry it (pot_comp.f:2302)
+  const double s = h->coll_type == 4 ? 1.0 / (2.0 * MQCT_PI) : 1.0;
+  proj_reduce_kernel<<<nblk((int64_t)nt * n_r, 256), 256, 0, h->stream>>>(
+      nt, ntp16, n_r, nchunk, s, h->d_exppart.as<double>(), h->d_terms.as<double>());
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(h->ev[2], h->stream));
+  HIP_TRY(hipEventSynchronize(h->ev[2]));
+  float ms;
+  HIP_TRY(hipEventElapsedTime(&ms, h->ev[0], h->ev[1]));
+  h->exp_timing[0] = ms;
+  HIP_TRY(hipEventElapsedTime(&ms, h->ev[1], h->ev[2]));
+  h->exp_timing[1] = ms;
+  h->terms_nr = n_r;
+  if (terms) HIP_TRY(hipMemcpy(terms, h->d_terms.p, (size_t)nt * n_r * 8, hipMemcpyDeviceToHost));
+  return MQCT_OK;
+}
+
+extern "C" int mqct_hip_expansion_build(mqct_hip_t *h, int n_r, const double *R_com,
+                                        const double *terms, int64_t total_size,
+                                        const int *ind_mat, double conv_unit_e,
+                                        double mij_zero_cut, int ir_cut1, int ir_cut2,
+                                        double *mat_el, double *mat_el_der, uint8_t *skipped) {
+  if (!h) return mqct_fail(MQCT_ERR_ARG, "expansion_build: null handle");
+  if (!h->basis_ok || h->n_terms <= 0)
+    return mqct_fail(MQCT_ERR_STATE, "expansion_build: basis and expansion terms must be set");
+  if (total_size != h->total_pairs)
+    return mqct_fail(MQCT_ERR_ARG, "ERROR: total_size does not match the basis (INI_ARRAYS pair count)");
+  if (!ind_mat) return mqct_fail(MQCT_ERR_ARG, "expansion_build: ind_mat required");
+  HIP_TRY(hipSetDevice(h->device));
+  int rc;
+  if (terms) {
+    if (!R_com || n_r < 2) return mqct_fail(MQCT_ERR_ARG, "expansion_build: R grid required with the coefficients");
+    bool same = (n_r == h->n_r);
+    for (int i = 0; same && i < n_r; i++) same = (R_com[i] == h->R[i]);
+    if (!same) {
+      if ((rc = mqct_set_r(h, n_r, R_com))) return rc;
+      h->v_ok = false;  // a V grid on another R grid no longer matches
+    }
+    for (size_t i = 0; i < (size_t)h->n_terms * n_r; i++)
+      if (terms[i] != terms[i]) return mqct_fail(MQCT_ERR_ARG, "ERROR IN READING TERMS.DAT (NaN coefficient)");
+    if ((rc = upload(h, h->d_terms, terms, (size_t)h->n_terms * n_r))) return rc;
+    h->terms_nr = n_r;
+  } else {
+    if (h->terms_nr <= 0) return mqct_fail(MQCT_ERR_STATE, "expansion_build: no coefficients (project first or pass them)");
+    if (n_r != h->terms_nr) return mqct_fail(MQCT_ERR_ARG, "expansion_build: n_r differs from the projected coefficients");
+    if (R_com)
+      for (int i = 0; i < n_r; i++)
+        if (R_com[i] != h->R[i]) return mqct_fail(MQCT_ERR_ARG, "expansion_build: R grid differs from the projection's");
+  }
+  if (ir_cut1 < 1 || ir_cut1 > n_r || ir_cut2 < 1 || ir_cut2 > n_r)
+    return mqct_fail(MQCT_ERR_ARG, "expansion_build: cutoff R index out of range");
+  const int64_t T = total_size;
+  std::vector<int2> pairs((size_t)T);
+  {
+    int64_t k = 0;
+    for (int s1 = 0; s1 < h->n_states; s1++)
+      for (int s2 = 0; s2 <= s1; s2++) {
+        if (h->m12[s1] != h->m12[s2]) continue;
+        if (ind_mat[2 * k] != s1 + 1 || ind_mat[2 * k + 1] != s2 + 1)
+          return mqct_fail(MQCT_ERR_ARG, "expansion_build: ind_mat is not the INI_ARRAYS order");
+        pairs[k] = make_int2(s1, s2);
+        k++;
+      }
+  }
+  // Clebsch-Gordan table over every argument the angular sums use
+  int ca = 0, cc = 0;
+  for (int c = 0; c < h->n_channels; c++) ca = std::max(ca, std::max(h->j1_ch[c], h->j2_ch[c]));
+  for (int s = 0; s < h->n_states; s++) cc = std::max(cc, h->j12[s]);
+  for (int i = 0; i < h->n_terms; i++) {
+    const int *a = &h->atop[5 * i];
+    if (h->coll_type == 0) {
+      ca = std::max(ca, std::max(a[0], a[2]));
+      cc = std::max(cc, a[4]);
+    } else {
+      ca = std::max(ca, a[0]);
+    }
+  }
+  cc = std::max(cc, ca);
+  const int w = 2 * ca + 1;
+  const size_t ncg = (size_t)(ca + 1) * (ca + 1) * (cc + 1) * w * w;
+  if (ncg > ((size_t)1 << 26)) return mqct_fail(MQCT_ERR_ARG, "expansion_build: angular momenta too large for the coefficient table");
+  std::vector<double> cgt(ncg, 0.0);
+  for (int j1 = 0; j1 <= ca; j1++)
+    for (int j2 = 0; j2 <= ca; j2++)
+      for (int J = std::abs(j1 - j2); J <= std::min(j1 + j2, cc); J++)
+        for (int m1 = -j1; m1 <= j1; m1++)
+          for (int m2 = -j2; m2 <= j2; m2++) {
+            if (std::abs(m1 + m2) > J) continue;
+            cgt[(((size_t)j1 * (ca + 1) + j2) * (cc + 1) + J) * w * w + (size_t)(m1 + ca) * w + (m2 + ca)] =
+                mq_cg(j1, j2, J, m1, m2, m1 + m2);
+          }
+  std::vector<int> chj(2 * (size_t)h->n_channels);
+  for (int c = 0; c < h->n_channels; c++) {
+    chj[c] = h->j1_ch[c];
+    chj[h->n_channels + c] = h->j2_ch[c];
+  }
+  const int ntp = (h->n_terms + 3) / 4 * 4;
+  if ((rc = upload(h, h->d_exppairs, pairs.data(), pairs.size()))) return rc;
+  if ((rc = upload(h, h->d_cgt, cgt.data(), cgt.size()))) return rc;
+  if ((rc = upload(h, h->d_expch, chj.data(), chj.size()))) return rc;
+  if (h->d_expA.ensure((size_t)T * ntp * 8) || h->d_mat.ensure((size_t)T * n_r * 8) ||
+      h->d_der.ensure((size_t)T * n_r * 8) || h->d_skip.ensure((size_t)T))
+    return mqct_fail(MQCT_ERR_NOMEM, "expansion_build: output does not fit in HBM");
+  h->built_T = T;
+  AngCtx x;
+  x.coll_type = h->coll_type;
+  x.n_terms = h->n_terms;
+  x.ntp = ntp;
+  x.T = T;
+  x.pairs = h->d_exppairs.as<int2>();
+  x.j12 = h->d_j12.as<int>();
+  x.m12 = h->d_m12.as<int>();
+  x.chan = h->d_chan.as<int>() + h->n_states;
+  x.chj = h->d_expch.as<int>();
+  x.n_ch = h->n_channels;
+  x.ldvec = h->ldvec;
+  x.eig1 = h->eig1.empty() ? nullptr : h->d_eig1.as<double>();
+  x.eig2 = h->eig2.empty() ? nullptr : h->d_eig2.as<double>();
+  x.atop = h->d_atop.as<int>();
+  x.cgt = h->d_cgt.as<double>();
+  x.ca = ca;
+  x.cc = cc;
+  HIP_TRY(hipEventRecord(h->ev[0], h->stream));
+  angular_kernel<<<nblk(T * ntp, 256), 256, 0, h->stream>>>(x, h->d_expA.as<double>());
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(h->ev[1], h->stream));
+  exp_matrix_kernel<<<nblk((T + 15) / 16, 4), 256, 0, h->stream>>>(
+      T, ntp, h->n_terms, n_r, h->d_expA.as<double>(), h->d_terms.as<double>(), conv_unit_e,
+      h->d_mat.as<double>());
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(h->ev[2], h->stream));
+  if ((rc = mqct_run_post(h, mij_zero_cut, ir_cut1, ir_cut2, nullptr))) return rc;
+  HIP_TRY(hipEventRecord(h->ev[3], h->stream));
+  HIP_TRY(hipEventSynchronize(h->ev[3]));
+  float ms[3];
+  for (int i = 0; i < 3; i++) HIP_TRY(hipEventElapsedTime(&ms[i], h->ev[i], h->ev[i + 1]));
+  h->exp_timing[2] = ms[0];
+  h->exp_timing[3] = ms[1];
+  // the build-timing slots: [0] angular coefficients, [1] none, [2] GEMM,
+  // [3] cutoff + spline, [4] total
+  h->timing[0] = ms[0];
+  h->timing[1] = 0.0;
+  h->timing[2] = ms[1];
+  h->timing[3] = ms[2];
+  h->timing[4] = ms[0] + ms[1] + ms[2];
+  h->built = true;
+  if (mat_el || mat_el_der || skipped) {
+    rc = mqct_hip_matrix_get(h, mat_el, mat_el_der);
+    if (rc) return rc;
+    if (skipped) HIP_TRY(hipMemcpy(skipped, h->d_skip.p, (size_t)T, hipMemcpyDeviceToHost));
+  }
+  return MQCT_OK;
+}
+
+extern "C" int mqct_hip_expansion_timing(mqct_hip_t *h, double *ms4) {
+  if (!h || !ms4) return mqct_fail(MQCT_ERR_ARG, "expansion_timing: null argument");
+  for (int i = 0; i < 4; i++) ms4[i] = h->exp_timing[i];
+  return MQCT_OK;
+}

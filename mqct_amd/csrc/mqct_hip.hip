@@ -37,15 +37,6 @@ void DevBuf::release() {
   bytes = 0;
 }
 
-template <class T>
-static int upload(mqct_hip *h, DevBuf &b, const T *src, size_t n) {
-  if (b.ensure(n * sizeof(T))) return mqct_fail(MQCT_ERR_NOMEM, "device alloc");
-  if (n)
-    HIP_TRY(hipMemcpyAsync(b.p, src, n * sizeof(T), hipMemcpyHostToDevice,
-                           h->stream));
-  return MQCT_OK;
-}
-
 // Clebsch-Gordan through the Racah 3j sum (restates suppl_routines.f:246-310
 // in double-argument form).
 static double lf(int n) { return mq_logfact(n); }
@@ -322,11 +313,14 @@ extern "C" int mqct_hip_set_grid(mqct_hip_t *h, const mqct_grid_t *g) {
   return MQCT_OK;
 }
 
-static int set_r(mqct_hip *h, int n_r, const double *R) {
+int mqct_set_r(mqct_hip *h, int n_r, const double *R) {
   if (n_r < 2 || !R) return mqct_fail(MQCT_ERR_ARG, "R grid needs >= 2 points");
   for (int i = 1; i < n_r; i++)
     if (!(R[i] > R[i - 1]))
       return mqct_fail(MQCT_ERR_ARG, "R grid must increase");
+  bool same = (n_r == h->n_r);
+  for (int i = 0; same && i < n_r; i++) same = (R[i] == h->R[i]);
+  if (!same) h->terms_nr = 0;  // expansion coefficients were on another R grid
   h->n_r = n_r;
   h->R.assign(R, R + n_r);
   return upload(h, h->d_R, R, n_r);
@@ -337,7 +331,7 @@ extern "C" int mqct_hip_set_vgrid(mqct_hip_t *h, int n_r, const double *R_com,
   if (!h || !V) return mqct_fail(MQCT_ERR_ARG, "set_vgrid: null argument");
   if (!h->grid_ok) return mqct_fail(MQCT_ERR_STATE, "set_vgrid: call set_grid first");
   HIP_TRY(hipSetDevice(h->device));
-  int rc = set_r(h, n_r, R_com);
+  int rc = mqct_set_r(h, n_r, R_com);
   if (rc) return rc;
   int64_t n = (int64_t)n_r * h->G;
   for (int64_t i = 0; i < n; i++)
@@ -360,7 +354,7 @@ extern "C" int mqct_hip_eval_pes(mqct_hip_t *h, int n_r, const double *R_com,
   if (mqct_pes_par_len(pes_kind, n_par, par) < 0)
     return mqct_fail(MQCT_ERR_ARG, "eval_pes: parameter array too short for its header");
   HIP_TRY(hipSetDevice(h->device));
-  int rc = set_r(h, n_r, R_com);
+  int rc = mqct_set_r(h, n_r, R_com);
   if (rc) return rc;
   if ((rc = upload(h, h->d_par, par, n_par))) return rc;
   if ((rc = mqct_eval_pes_dev(h, pes_kind, n_par))) return rc;

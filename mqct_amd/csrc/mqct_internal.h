@@ -100,6 +100,16 @@ struct mqct_hip {
   // M(R) - M(R_max) (matrix_ini.f:2226-2230, 6603-6608)
   bool mij_shift = true;
 
+  // ---- PES expansion (expansion.hip) ----
+  int n_terms = 0;
+  std::vector<int> atop;             // A_TOP rows, [n_terms][5]
+  DevBuf d_atop, d_expch, d_cgt;     // terms, per-channel j1/j2, CG table
+  DevBuf d_terms;                    // expansion_terms, [n_terms][n_r]
+  DevBuf d_expA;                     // angular coefficients, [T][n_terms (padded)]
+  DevBuf d_expF, d_expT1, d_expT2, d_expcgl, d_exppart, d_exppairs;
+  int terms_nr = 0;                  // R points of d_terms (0: none)
+  double exp_timing[4] = {0, 0, 0, 0};
+
   MlpState *mlp = nullptr;
   DynState *dyn = nullptr;
 };
@@ -183,6 +193,20 @@ __host__ __device__ inline double mq_plgndr(int l, int m, double x) {
 
 // Clebsch-Gordan <j1 m1 j2 m2 | j3 m3> (host only; built into tables).
 double mq_cg(int j1, int j2, int j3, int m1, int m2, int m3);
+
+// Asynchronous host-to-device copy into a handle-owned buffer (grown as
+// needed) on the handle's stream.
+template <class T>
+inline int upload(mqct_hip *h, DevBuf &b, const T *src, size_t n) {
+  if (b.ensure(n * sizeof(T))) return mqct_fail(MQCT_ERR_NOMEM, "device alloc");
+  if (n)
+    HIP_TRY(hipMemcpyAsync(b.p, src, n * sizeof(T), hipMemcpyHostToDevice,
+                           h->stream));
+  return MQCT_OK;
+}
+
+// R grid of the handle (>= 2 increasing points), host copy and device copy.
+int mqct_set_r(mqct_hip *h, int n_r, const double *R);
 
 // Kernel launchers (defined in the .hip files).
 int mqct_build_psi(mqct_hip *h);

@@ -95,6 +95,25 @@ void or_splint(const double *xa, const double *ya, const double *y2a, int n,
                double x, double *y, double *yprime);
 int  or_nearest_r(const double *R, int n, double r0);
 
+/* ---- PES expansion (mqct_oracle_exp.c) --------------------------------- */
+/* Term rows are A_TOP(1:5, t); expansion coefficients terms[t * n_r + ir]. */
+void or_expansion_term(int coll_type, const int *t, double b1, double g1,
+                       double a2, double b2, double g2, double *re,
+                       double *im);
+void or_expansion_term_grid(const or_matrix_ctx *c, const int *t, double *out);
+void or_calc_expansion_terms(const or_matrix_ctx *c, int n_terms,
+                             const int *a_top, double *out);
+void or_expansion_matrix_element(const or_matrix_ctx *c, int n_terms,
+                                 const int *a_top, const double *terms,
+                                 const int *ind_mat, int64_t k, double *out);
+void or_expansion_matrix_make(const or_matrix_ctx *c, int n_terms,
+                              const int *a_top, const double *terms,
+                              int64_t total_size, const int *ind_mat,
+                              const double *R, double conv_unit_e,
+                              double mij_zero_cut_hartree, int ir_cut1,
+                              int ir_cut2, double *mat_el, double *mat_el_der,
+                              uint8_t *skipped);
+
 /* ---- synthetic potentials --------------------------------------------- */
 double or_pes_legendre(const double *par, double R, double theta);
 double or_pes_realsh(const double *par, double R, double beta, double gamma);

@@ -117,6 +117,13 @@ def lib():
         L.or_pes_sites.argtypes = [_dp, _d, _d, _d, _d, _d, _d]
         L.or_pes_sites.restype = _d
         L.or_vgrid.argtypes = [_i, _i, _dp, ctypes.POINTER(MatrixCtx), _dp, _dp]
+        L.or_expansion_term.argtypes = [_i, _ip, _d, _d, _d, _d, _d, _dp, _dp]
+        L.or_expansion_term_grid.argtypes = [ctypes.POINTER(MatrixCtx), _ip, _dp]
+        L.or_calc_expansion_terms.argtypes = [ctypes.POINTER(MatrixCtx), _i, _ip, _dp]
+        L.or_expansion_matrix_element.argtypes = [ctypes.POINTER(MatrixCtx), _i, _ip, _dp, _ip, ctypes.c_int64,
+                                                  _dp]
+        L.or_expansion_matrix_make.argtypes = [ctypes.POINTER(MatrixCtx), _i, _ip, _dp, ctypes.c_int64, _ip, _dp,
+                                               _d, _d, _i, _i, _dp, _dp, _u8p]
         L.or_logfact_init.argtypes = [_i]
         L.or_resize.argtypes = [ctypes.POINTER(DynCtx)]
         L.or_at_prepare.argtypes = [ctypes.POINTER(DynCtx)]
@@ -313,6 +320,56 @@ class MatrixOracle:
         lib().or_matrix_make(ctypes.byref(self.c), T, ip(ind), dp(self.R), conv_unit_e, mij_zero_cut, ir1, ir2,
                              dp(M), dp(D), S.ctypes.data_as(_u8p))
         return M, D, S
+
+
+    # -- PES expansion (mqct_oracle_exp.c) --------------------------------
+    def expansion_terms(self, a_top):
+        """CALC_EXPANSION_TERMS: [n_terms][n_r] coefficients of the V grid."""
+        a = _a_top(a_top)
+        out = np.zeros((len(a), len(self.R)))
+        lib().or_calc_expansion_terms(ctypes.byref(self.c), len(a), ip(a), dp(out))
+        return out
+
+    def term_grid(self, term):
+        """Real part of the term's function on the grid (V-grid order)."""
+        c = self.c
+        G = c.n_beta * c.n_gamma * c.n_alpha2 * c.n_beta2 * c.n_gamma2
+        out = np.zeros(G)
+        t = _a_top([term])
+        lib().or_expansion_term_grid(ctypes.byref(c), ip(t), dp(out))
+        return out
+
+    def expansion_element(self, a_top, terms, ind_mat, k):
+        """EXPANSION_MATRIX_ELEMENT of pair k (0-based) at every R point."""
+        a, t, ind = _a_top(a_top), f64(terms), i32(ind_mat)
+        out = np.zeros(len(self.R))
+        lib().or_expansion_matrix_element(ctypes.byref(self.c), len(a), ip(a), dp(t), ip(ind), int(k), dp(out))
+        return out
+
+    def expansion_matrix_make(self, a_top, terms, ind_mat, conv_unit_e, mij_zero_cut, ir1, ir2):
+        """MATRIX_MAKE with expansion_defined: M, Mat_el_der, skipped."""
+        a, t, ind = _a_top(a_top), f64(terms), i32(ind_mat)
+        T, n = len(ind), len(self.R)
+        M, D = np.zeros((T, n)), np.zeros((T, n))
+        S = np.zeros(T, np.uint8)
+        lib().or_expansion_matrix_make(ctypes.byref(self.c), len(a), ip(a), dp(t), T, ip(ind), dp(self.R),
+                                       conv_unit_e, mij_zero_cut, ir1, ir2, dp(M), dp(D), S.ctypes.data_as(_u8p))
+        return M, D, S
+
+
+def _a_top(a_top):
+    """Term rows padded to A_TOP's five slots."""
+    a = np.zeros((len(a_top), 5), np.int32)
+    for i, row in enumerate(a_top):
+        a[i, :len(row)] = row
+    return a
+
+
+def expansion_term(coll_type, term, b1, g1=0.0, a2=0.0, b2=0.0, g2=0.0):
+    t = _a_top([term])
+    re, im = _d(), _d()
+    lib().or_expansion_term(coll_type, ip(t), b1, g1, a2, b2, g2, ctypes.byref(re), ctypes.byref(im))
+    return re.value, im.value
 
 
 def vgrid(coll_type, pes_kind, par, ctx: MatrixOracle, R):

@@ -1,0 +1,153 @@
+"""Pins of the oracle's PES-expansion restatement (oracle/mqct_oracle_exp.c)
+and of the host side of the expansion path (mqct_amd/expansion.py).
+
+What fixes each piece:
+  * the term list: the reference's own generated EXPANSION_TERMS_LIST.DAT
+    (EXAMPLES_IO/SYS_TYPE_4/H2O_He_CalcExp, L_MAX=10, M_MAX=6, AXL_SYM1=2);
+  * the projection (CALC_EXPANSION_TERMS): on a grid that integrates the
+    products exactly, a potential equal to one term's function projects to
+    1 on that term (1/2 when the term has NJU != 0, whose function is a
+    cosine) and 0 on every other term -- normalisation and orthogonality;
+  * the matrix elements (EXPANSION_MATRIX_ELEMENT): for a potential that is
+    a finite sum of term functions, the matrix built from its projection
+    equals the quadrature-branch matrix of the same potential (the
+    INTEGRATOR_TEMP restatement, itself pinned by closed forms and the
+    reference's MTRX.DAT).  Types 1 and 0 agree term by term.  Type 4 agrees
+    for even NJU; for odd NJU the reference's two formulas differ by the
+    sign (-1)^NJU: EXPANSION_MATRIX_ELEMENT multiplies by (-1)^NJU
+    (pot_comp.f:7029-7030) on top of the (-1)^NJU that EXPANSION_TERM
+    (pot_comp.f:3764-3766) puts into the projection.  The restatement keeps
+    the reference's formula, so the pin is M_exp = sum_t (-1)^NJU_t M_quad(t);
+    molecules with a C2 axis (H2O: AXL_SYM1=2) have only even NJU;
+  * PES_EXPAN_TERMS.DAT: the fixed-column read of the reference's own file
+    (EXAMPLES_IO/SYS_TYPE_0/H2O_H2O_Identical) and a write/read round trip.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle.oracle as O
+from mqct_amd import expansion as X
+from mqct_amd import workloads as W
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden", "expansion")
+
+
+def test_type4_term_list_matches_reference_calcexp():
+    with open(os.path.join(GOLD, "h2o_he_calcexp_EXPANSION_TERMS_LIST.DAT")) as f:
+        text = f.read()
+    ref = X.parse_terms_list(text)
+    mine = X.terms_type4(10, 6, du1=2, dlamda1=1)
+    assert mine == ref and len(ref) == 32
+    assert X.format_terms_list(mine).split() == text.split()
+
+
+def test_type0_term_generator_rules():
+    t = X.terms_type0(2, 2)
+    assert all(n1 >= 0 for _, n1, _, _, _ in t)
+    assert not any(n1 == 0 and n2 < 0 for _, n1, _, n2, _ in t)
+    assert all((l1 + l2 + l) % 2 == 0 and abs(l1 - l2) <= l <= l1 + l2 for l1, _, l2, _, l in t)
+    assert len(set(t)) == len(t)
+    assert (1, 1, 1, -1, 0) in t and (1, 0, 1, -1, 0) not in t
+
+
+def _single_term_projection(coll_type, wl, terms):
+    o = O.oracle_for_workload(wl, np.zeros(len(wl.R) * wl.grid.G))
+    worst = 0.0
+    for s, ts in enumerate(terms):
+        f = o.term_grid(ts)
+        V = np.concatenate([f * (1.0 + 0.5 * ir) for ir in range(len(wl.R))])
+        o2 = O.oracle_for_workload(wl, V)
+        P = o2.expansion_terms(terms)
+        nju0 = (ts[1] == 0) if coll_type == 4 else (coll_type == 1 or (ts[1] == 0 and ts[3] == 0))
+        want = np.zeros_like(P)
+        want[s] = (1.0 if nju0 else 0.5) * (1.0 + 0.5 * np.arange(len(wl.R)))
+        worst = max(worst, np.max(np.abs(P - want)))
+    return worst
+
+
+def test_projection_closed_form_type1():
+    wl = W.diatom_atom(jmax=2, n_beta=24, n_r=2)
+    assert _single_term_projection(1, wl, X.terms_type1(6)) < 1e-13
+
+
+def test_projection_closed_form_type4():
+    wl = W.asym_top_atom(jmax=2, n_beta=28, n_gamma=16, n_r=2)
+    assert _single_term_projection(4, wl, X.terms_type4(4)) < 1e-12
+
+
+def test_projection_closed_form_type0():
+    wl = W.two_asym_tops(1, 1, n_ang_each=(6, 16, 16, 6, 6), n_r=2)
+    terms = [(0, 0, 0, 0, 0), (1, 0, 1, 0, 2), (1, 1, 1, -1, 0), (1, 1, 1, 1, 2), (2, 1, 1, 0, 1)]
+    assert _single_term_projection(0, wl, terms) < 1e-12
+
+
+def _expansion_vs_quadrature(wl, terms, sign, stride, rng):
+    """M from the projection of V = sum_t c_t(R) f_t against the quadrature
+    matrix of sign_t c_t f_t summed; returns (max |diff|, max |M|)."""
+    o = O.oracle_for_workload(wl, np.zeros(len(wl.R) * wl.grid.G))
+    F = np.array([o.term_grid(t) for t in terms])
+    c = rng.normal(size=(len(terms), len(wl.R)))
+    V = np.einsum("tr,tg->rg", c, F).ravel()
+    Vs = np.einsum("tr,tg->rg", c * np.asarray(sign)[:, None], F).ravel()
+    oe = O.oracle_for_workload(wl, V)
+    oq = O.oracle_for_workload(wl, Vs)
+    P = oe.expansion_terms(terms)
+    b = wl.basis
+    diff = scale = 0.0
+    for k in range(0, b.total_size, stride):
+        me = oe.expansion_element(terms, P, b.ind_mat, k)
+        mq = np.array([oq.element(b.ind_mat, k, i) for i in range(len(wl.R))])
+        diff = max(diff, np.max(np.abs(me - mq)))
+        scale = max(scale, np.max(np.abs(mq)))
+    return diff, scale
+
+
+def test_type1_expansion_matches_quadrature():
+    wl = W.diatom_atom(jmax=4, n_beta=24, n_r=3)
+    terms = X.terms_type1(6)
+    d, s = _expansion_vs_quadrature(wl, terms, [1.0] * len(terms), 3, np.random.default_rng(3))
+    assert s > 0.1 and d < 1e-11 * s
+
+
+def test_type4_expansion_matches_quadrature_with_odd_nju_sign():
+    wl = W.asym_top_atom(jmax=3, n_beta=28, n_gamma=16, n_r=3)
+    terms = X.terms_type4(4)
+    sign = [(-1.0) ** n for _, n in terms]
+    d, s = _expansion_vs_quadrature(wl, terms, sign, 7, np.random.default_rng(4))
+    assert s > 0.1 and d < 1e-11 * s
+    # without the sign the two differ: odd NJU terms do contribute
+    d2, _ = _expansion_vs_quadrature(wl, terms, [1.0] * len(terms), 7, np.random.default_rng(4))
+    assert d2 > 0.1 * s
+
+
+def test_type0_expansion_matches_quadrature():
+    wl = W.two_asym_tops(1, 1, n_ang_each=(6, 16, 16, 6, 6), n_r=2)
+    terms = [(0, 0, 0, 0, 0), (1, 0, 0, 0, 1), (1, 0, 1, 0, 2), (2, 0, 1, 0, 1), (1, 1, 1, 0, 0),
+             (1, 1, 1, -1, 0), (1, 1, 1, 1, 2), (2, 2, 0, 0, 2), (2, 1, 1, 1, 1), (2, 2, 2, -2, 2)]
+    d, s = _expansion_vs_quadrature(wl, terms, [1.0] * len(terms), 37, np.random.default_rng(5))
+    assert s > 1e-3 and d < 1e-11 * s
+
+
+def test_reference_pes_expan_terms_file_reads():
+    R, C = X.read_pes_expan_terms(os.path.join(GOLD, "h2o_h2o_identical_PES_EXPAN_TERMS.DAT"), 152)
+    assert list(R) == [6.00, 8.09]
+    assert C.shape == (152, 2)
+    assert C[0, 0] == -0.166182059235E+02 and C[1, 0] == -0.637068209530E+01
+    assert C[0, 1] == -0.136502164329E+02 and C[3, 1] == -0.407457989480E+02
+
+
+def test_pes_expan_terms_round_trip(tmp_path):
+    rng = np.random.default_rng(6)
+    terms = X.terms_type4(3)
+    R = np.round(np.linspace(4.5, 20.0, 7), 2)
+    C = rng.normal(size=(len(terms), len(R))) * 10.0 ** rng.integers(-8, 4, size=(len(terms), len(R)))
+    C[0, 0] = 0.0
+    p = str(tmp_path / "PES_EXPAN_TERMS.DAT")
+    X.write_pes_expan_terms(p, 4, R, terms, C)
+    R2, C2 = X.read_pes_expan_terms(p, len(terms))
+    assert np.array_equal(R2, R)
+    assert np.max(np.abs(C2 - C) / np.maximum(np.abs(C), 1e-300)) < 1e-11
+    line = open(p).read().splitlines()[1]
+    assert len(line) == 7 + 22 * len(terms)
