@@ -76,6 +76,23 @@ int mqct_hip_set_basis(mqct_hip_t *h, int coll_type, int n_states,
                        int n_channels, const int *j1_ch, const int *j2_ch,
                        int ldvec, const double *eig1, const double *eig2);
 
+/* IDENTICAL= and IDENTICAL_PES= of a type-0 basis, after set_basis.
+ * parity_state (+-1, the exchange parity P of each state) and
+ * parity_state_bk ((-1)^j12 times the channel's inversion and kappa signs)
+ * are INI_ARRAYS' arrays (matrix_ini.f:486-528); same_level[c] is 1 when
+ * both partners of channel c are in the same level (j, Ka, Kc).  With
+ * parity_state the pairs are those of equal m12 and equal P
+ * (matrix_ini.f:556-630) and the basis is of identical partners, which the
+ * reference requires to come with identical_pes; with parity_state NULL
+ * the partners are distinguishable and identical_pes alone marks the term
+ * list as that of an exchange-symmetric PES (EXPANSION_MATRIX_ELEMENT's
+ * second symmetry term).  Identical partners are built through the
+ * expansion (mqct_hip_expansion_build); mqct_hip_matrix_build refuses
+ * them.  Clears the expansion terms. */
+int mqct_hip_set_identical(mqct_hip_t *h, int identical_pes,
+                           const int *parity_state, const int *parity_state_bk,
+                           const int *same_level);
+
 /* Angular quadrature: GRID_INTEGR module state (iotest.f:62-90) as
  * INI_POT_STORAGE fills it (pot_comp.f:374-382, 586-611, 1628-1709).  Nodes
  * x* on [-1,1] with weights w*, mapped as angle = x*m + x*l * x (or angle =
@@ -153,7 +170,8 @@ int mqct_hip_mij_spline(mqct_hip_t *h, int n_r, const double *R_com,
  * (CALC_EXPANSION_TERMS, pot_comp.f:4695-5920), and the matrix elements
  * from the coefficients (EXPANSION_MATRIX_ELEMENT, pot_comp.f:6465-8751, as
  * MATRIX_MAKE calls it with expansion_defined, matrix_ini.f:1950-1956).
- * Distinguishable partners (types 1, 4 and 0 without IDENTICAL).          */
+ * Types 1 and 4, and type 0 for distinguishable partners, IDENTICAL_PES
+ * and identical partners (mqct_hip_set_identical).                        */
 
 /* a_top: n_terms rows of five ints, A_TOP(1:5, t): type 1 (L, 0, 0, 0, 0),
  * type 4 (L, NJU, 0, 0, 0), type 0 (L1, NJU1, L2, NJU2, L).  Rejected as

@@ -83,6 +83,7 @@ SIGNATURES = {
     "mqct_hip_set_mij_shift": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "mqct_hip_mij_spline": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, c_dbl_p, ctypes.c_int64, c_dbl_p,
                                            c_dbl_p]),
+    "mqct_hip_set_identical": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, c_int_p, c_int_p, c_int_p]),
     "mqct_hip_set_expansion_terms": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, c_int_p]),
     "mqct_hip_expansion_project": (ctypes.c_int, [ctypes.c_void_p, c_dbl_p]),
     "mqct_hip_expansion_build": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, c_dbl_p, c_dbl_p, ctypes.c_int64,

@@ -130,6 +130,11 @@ class Basis:
     chann_indx: np.ndarray = None
     ind_mat: np.ndarray = None        # [T][2], 1-based
     Ej: np.ndarray = None             # cm^-1 per state
+    # identical partners (type 0, IDENTICAL=YES): exchange parity per state
+    # and the parity_state_bk sign the expansion's exchange terms carry
+    identical: bool = False
+    parity_state: np.ndarray = None
+    parity_state_bk: np.ndarray = None
     extra: dict = field(default_factory=dict)
 
     @property
@@ -233,10 +238,52 @@ def channels_type0(jmin1, jmax1, jmin2, jmax2, A1, B1, C1, A2, B2, C2, keep=None
     )
 
 
-def ini_arrays(b: Basis):
+def channels_type0_list(rows, A1, B1, C1, A2=None, B2=None, C2=None):
+    """Asym. top + asym. top channels from an explicit CHNLS_LIST of
+    (j1, ka1, kc1, j2, ka2, kc2) rows, in the list's order."""
+    A2, B2, C2 = (A1, B1, C1) if A2 is None else (A2, B2, C2)
+    ld = 2 * max(max(r[0], r[3]) for r in rows) + 1
+    out = []
+    for j1, ka1, kc1, j2, ka2, kc2 in rows:
+        E1, v1 = eigen_vec(A1, B1, C1, j1, ka1, kc1)
+        E2, v2 = eigen_vec(A2, B2, C2, j2, ka2, kc2)
+        a = np.zeros(ld)
+        a[:2 * j1 + 1] = v1
+        b = np.zeros(ld)
+        b[:2 * j2 + 1] = v2
+        out.append((j1, ka1, kc1, j2, ka2, kc2, E1 + E2, a, b))
+    col = lambda i: np.array([r[i] for r in out], np.int32)
+    return Basis(0, col(0), np.array([r[6] for r in out]), col(1), col(2), col(3), col(4), col(5),
+                 eig1=np.array([r[7] for r in out]), eig2=np.array([r[8] for r in out]))
+
+
+def inversion_sign(vec, J):
+    """The sign of c_K c_{-K} for the first K = -J..0 where it is not zero
+    (pot_comp.f:70-104): +1 when the level is even under K -> -K, -1 when
+    odd."""
+    for i in range(J + 1):
+        p = vec[i] * vec[2 * J - i]
+        if p != 0.0:
+            return 1 if p > 0.0 else -1
+        if abs(vec[i]) + abs(vec[2 * J - i]) != 0.0:
+            raise ValueError("STOP: ASYMETRIC TOP DIAG WRONG")
+    return 1
+
+
+def kappa(ka, kc):
+    """bikram_kappa (matrix_ini.f:9147-9180): (Ka - Kc) mod 2."""
+    return abs(ka - kc) % 2
+
+
+def ini_arrays(b: Basis, identical=False):
     """States (channel order, then j12, then m12 ascending) and the pair list
-    st1 >= st2 with equal m12 (matrix_ini.f:255-630); distinguishable
-    partners only."""
+    st1 >= st2 with equal m12 (matrix_ini.f:255-630).  identical (type 0,
+    IDENTICAL=YES): per channel the exchange parities p = +1, -1 outermost
+    (only +1 when both partners are in the same level, where odd j12 states
+    take -1), and pairs of equal m12 and equal exchange parity
+    (matrix_ini.f:486-528, 556-630)."""
+    if identical:
+        return _ini_arrays_identical(b)
     j12, m12, ich, cidx = [], [], [], []
     for i in range(b.n_channels):
         if b.coll_type in (1, 4):
@@ -268,6 +315,44 @@ def ini_arrays(b: Basis):
     pairs = []
     for s1 in range(len(m)):
         s2 = np.nonzero(m[: s1 + 1] == m[s1])[0]
+        pairs.append(np.stack([np.full(len(s2), s1 + 1), s2 + 1], 1))
+    b.ind_mat = np.ascontiguousarray(np.concatenate(pairs).astype(np.int32))
+    return b
+
+
+def _ini_arrays_identical(b: Basis):
+    if b.coll_type != 0:
+        raise ValueError("ERROR: EXCHANGE SYMMETRY DEFINED FOR NON-IDENTICAL PARTICLES")
+    j12, m12, ich, cidx, par, pbk = [], [], [], [], [], []
+    for i in range(b.n_channels):
+        ja, jb = int(b.j1_ch[i]), int(b.j2_ch[i])
+        same = ja == jb and b.ka1_ch[i] == b.ka2_ch[i] and b.kc1_ch[i] == b.kc2_ch[i]
+        p_inv = inversion_sign(b.eig1[i], ja) * inversion_sign(b.eig2[i], jb)
+        k12 = (-1) ** (kappa(b.ka1_ch[i], b.kc1_ch[i]) + kappa(b.ka2_ch[i], b.kc2_ch[i]))
+        first = None
+        for p in range(1, (1 if same else 2) + 1):
+            for js in range(abs(ja - jb), ja + jb + 1):
+                for m in range(-js, js + 1):
+                    j12.append(js)
+                    m12.append(m)
+                    ich.append(i + 1)
+                    pbk.append((-1) ** js * p_inv * k12)
+                    par.append(-1 if (same and js % 2 == 1) else (-1) ** (p - 1))
+                    if m == 0 and js == abs(ja - jb) and p == 1:
+                        first = len(j12)
+        cidx.append(first)
+    b.identical = True
+    b.j12 = np.array(j12, np.int32)
+    b.m12 = np.array(m12, np.int32)
+    b.indx_chann = np.array(ich, np.int32)
+    b.chann_indx = np.array(cidx, np.int32)
+    b.parity_state = np.array(par, np.int32)
+    b.parity_state_bk = np.array(pbk, np.int32)
+    b.Ej = np.asarray(b.E_ch)[b.indx_chann - 1]
+    m, pr = b.m12, b.parity_state
+    pairs = []
+    for s1 in range(len(m)):
+        s2 = np.nonzero((m[: s1 + 1] == m[s1]) & (pr[: s1 + 1] == pr[s1]))[0]
         pairs.append(np.stack([np.full(len(s2), s1 + 1), s2 + 1], 1))
     b.ind_mat = np.ascontiguousarray(np.concatenate(pairs).astype(np.int32))
     return b

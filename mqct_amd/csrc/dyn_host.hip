@@ -143,7 +143,7 @@ extern "C" int mqct_hip_use_built_matrix(mqct_hip_t *h, const double *Ej, double
   long long k = 0;
   for (int s1 = 0; s1 < h->n_states; s1++)
     for (int s2 = 0; s2 <= s1; s2++)
-      if (h->m12[s1] == h->m12[s2]) {
+      if (mqct_pair_ok(h, s1, s2)) {
         ind[2 * k] = s1 + 1;
         ind[2 * k + 1] = s2 + 1;
         k++;

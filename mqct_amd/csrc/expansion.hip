@@ -19,7 +19,8 @@
 //   matrix      M = conv * A V_t as an f64 MFMA GEMM (16 pairs x 128 R
 //               points per wave), then the shared cutoff and clamped spline.
 // Term rows are A_TOP(1:5, t): type 1 (L), type 4 (L, NJU), type 0
-// (L1, NJU1, L2, NJU2, L).  Distinguishable partners only.
+// (L1, NJU1, L2, NJU2, L); type 0 also with IDENTICAL_PES and identical
+// partners (mqct_hip_set_identical).
 #include "mqct_internal.h"
 
 #include <algorithm>
@@ -213,6 +214,9 @@ struct AngCtx {
   const int *atop;
   const double *cgt;    // CG table
   int ca, cc;           // first/second argument max, third argument max
+  int ident, ipes;      // identical partners; exchange-symmetric PES terms
+  const int *par, *pbk; // per state: exchange parity, parity_state_bk
+  const int *same;      // per channel: both partners in one level
 };
 
 __device__ inline double cgtab(const AngCtx &x, int j1, int j2, int J, int m1, int m2) {
@@ -274,57 +278,109 @@ __global__ void angular_kernel(AngCtx x, double *__restrict__ A) {
       }
     }
   } else {
+    // symmetry 1 always; 2 with IDENTICAL_PES (the term with l1 <-> l2);
+    // 3 and 4 with identical partners (the p-side levels exchanged),
+    // weighted by the initial state's exchange parities
     const int j1p = x.chj[chp], j2p = x.chj[x.n_ch + chp];
     const int j1pp = x.chj[chpp], j2pp = x.chj[x.n_ch + chpp];
     const int j12p = x.j12[stp], j12pp = x.j12[stpp];
     const int l1 = a[0], l2 = a[2], l = a[4];
-    int n1 = a[1], n2 = a[3];
-    if (triang(j1pp, l1, j1p) && triang(j2pp, l2, j2p)) {
-      const double *c1p = x.eig1 + (size_t)chp * x.ldvec + j1p;
-      const double *c1pp = x.eig1 + (size_t)chpp * x.ldvec + j1pp;
-      const double *c2p = x.eig2 + (size_t)chp * x.ldvec + j2p;
-      const double *c2pp = x.eig2 + (size_t)chpp * x.ldvec + j2pp;
-      double cgjk = 0.0;
-      const int npl = 2 - (n1 == 0 && n2 == 0);
+    const int n1o = a[1], n2o = a[3];
+    int pp = 1;
+    if (x.ident) {
+      int bk1 = x.pbk[stp];
+      if (stp == stpp && x.same[chp]) bk1 = (j12p & 1) ? -1 : 1;
+      pp = x.par[stp] * bk1;
+    }
+    const double *c1p = x.eig1 + (size_t)chp * x.ldvec;
+    const double *c1pp = x.eig1 + (size_t)chpp * x.ldvec + j1pp;
+    const double *c2p = x.eig2 + (size_t)chp * x.ldvec;
+    const double *c2pp = x.eig2 + (size_t)chpp * x.ldvec + j2pp;
+    const int npl = 2 - (n1o == 0 && n2o == 0);
+    bool sym[4] = {false, false, false, false};
+    double cgjk[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int sI = 0; sI < 4; sI++) {
+      if (sI == 1 && !x.ipes) continue;
+      if (sI >= 2 && !x.ident) continue;
+      if ((sI == 1 || sI == 3) && l1 == l2 && abs(n1o) == abs(n2o)) continue;
+      const bool st = (sI == 0 || sI == 2);
+      const int la = st ? l1 : l2, lb = st ? l2 : l1;
+      const int ja = sI < 2 ? j1p : j2p, jb = sI < 2 ? j2p : j1p;
+      const double *va = (sI < 2 ? c1p : c2p) + ja;
+      const double *vb = (sI < 2 ? c2p : c1p) + jb;
+      if (!(triang(j1pp, la, ja) && triang(j2pp, lb, jb))) continue;
+      sym[sI] = true;
       for (int pl = 1; pl <= npl; pl++) {
+        int n1 = n1o, n2 = n2o;
         double sc = 1.0;
-        if (pl == 2) { sc = sgnp(l1 + l2 + l + n1 + n2); n1 = -n1; n2 = -n2; }
+        if (pl == 2) { sc = sgnp(l1 + l2 + l + n1o + n2o); n1 = -n1o; n2 = -n2o; }
+        if (!st) sc *= sgnp(l1 + l2);
+        const int na = st ? n1 : n2, nb = st ? n2 : n1;
         double db1 = 0.0, db2 = 0.0;
-        for (int k1p = -j1p; k1p <= j1p; k1p++) {
-          int k1pp = k1p - n1;
-          if (k1pp < -j1pp || k1pp > j1pp) continue;
-          db1 += cgtab(x, j1pp, l1, j1p, k1pp, n1) * c1p[k1p] * c1pp[k1pp];
+        for (int kp = -ja; kp <= ja; kp++) {
+          int kpp = kp - na;
+          if (kpp < -j1pp || kpp > j1pp) continue;
+          db1 += cgtab(x, j1pp, la, ja, kpp, na) * va[kp] * c1pp[kpp];
         }
-        for (int k2p = -j2p; k2p <= j2p; k2p++) {
-          int k2pp = k2p - n2;
-          if (k2pp < -j2pp || k2pp > j2pp) continue;
-          db2 += cgtab(x, j2pp, l2, j2p, k2pp, n2) * c2p[k2p] * c2pp[k2pp];
+        for (int kp = -jb; kp <= jb; kp++) {
+          int kpp = kp - nb;
+          if (kpp < -j2pp || kpp > j2pp) continue;
+          db2 += cgtab(x, j2pp, lb, jb, kpp, nb) * vb[kp] * c2pp[kpp];
         }
-        cgjk += db1 * db2 * sc;
+        cgjk[sI] += db1 * db2 * sc * (sI >= 2 ? pp : 1);
       }
+    }
+    const bool any = sym[0] || sym[1] || sym[2] || sym[3];
+    if (any && (x.ident || sym[0] || sym[1])) {
+      const int mx = l1 < l2 ? l1 : l2;
+      // with identical partners the exchanged pair state needs mp over
+      // -j2'..j2' as well (pot_comp.f:8589 stops at j1')
+      const int mpx = x.ident && j2p > j1p ? j2p : j1p;
       double m5 = 0.0;
-      if (cgjk != 0.0) {
-        const int mx = l1 < l2 ? l1 : l2;
-        for (int mp = -j1p; mp <= j1p; mp++) {
-          if (m - mp < -j2p || m - mp > j2p) continue;
-          const double cg_p = cgtab(x, j1p, j2p, j12p, mp, m - mp);
-          double m1 = 0.0;
-          for (int me = -mx; me <= mx; me++) {
-            int mpp = mp - me;
-            if (mpp < -j1pp || mpp > j1pp) continue;
-            if (m - mpp < -j2pp || m - mpp > j2pp) continue;
-            double cg_pp = cgtab(x, j1pp, j2pp, j12pp, mpp, m - mpp);
-            double ca = cgtab(x, j1pp, l1, j1p, mpp, me);
-            double cb = cgtab(x, j2pp, l2, j2p, m - mpp, -me);
-            m1 += cgtab(x, l1, l2, l, me, -me) * cg_pp * ca * cb;
+      for (int mp = -mpx; mp <= mpx; mp++) {
+        double cg_p = 0.0, cg_x = 0.0;
+        if (m - mp < -j2p || m - mp > j2p) {
+          if (!x.ident) continue;
+        } else {
+          cg_p = cgtab(x, j1p, j2p, j12p, mp, m - mp);
+        }
+        if (x.ident) {
+          if ((m - mp < -j1p || m - mp > j1p) && (m - mp < -j2p || m - mp > j2p)) continue;
+          cg_x = cgtab(x, j2p, j1p, j12p, mp, m - mp);
+        }
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int me = -mx; me <= mx; me++) {
+          int mpp = mp - me;
+          if (mpp < -j1pp || mpp > j1pp) continue;
+          if (m - mpp < -j2pp || m - mpp > j2pp) continue;
+          const double cg_pp = cgtab(x, j1pp, j2pp, j12pp, mpp, m - mpp);
+          const double cl = cgtab(x, l1, l2, l, me, -me);
+          if (sym[0])
+            acc[0] += cl * cg_pp * cgtab(x, j1pp, l1, j1p, mpp, me) * cgtab(x, j2pp, l2, j2p, m - mpp, -me);
+          if (sym[1])
+            acc[1] += cl * cg_pp * cgtab(x, j1pp, l2, j1p, mpp, me) * cgtab(x, j2pp, l1, j2p, m - mpp, -me);
+          if (mp >= -j2p && mp <= j2p) {
+            if (sym[2])
+              acc[2] += cl * cg_pp * cgtab(x, j1pp, l1, j2p, mpp, me) * cgtab(x, j2pp, l2, j1p, m - mpp, -me);
+            if (sym[3])
+              acc[3] += cl * cg_pp * cgtab(x, j1pp, l2, j2p, mpp, me) * cgtab(x, j2pp, l1, j1p, m - mpp, -me);
           }
-          m5 += m1 * cg_p * cgjk;
+        }
+        if (sym[0]) m5 += acc[0] * cg_p * cgjk[0];
+        if (sym[1]) m5 += acc[1] * cg_p * cgjk[1];
+        if (x.ident) {
+          if (sym[2]) m5 += acc[2] * cg_x * cgjk[2];
+          if (sym[3]) m5 += acc[3] * cg_x * cgjk[3];
         }
       }
       const double coef1 = sqrt((2.0 * j1pp + 1.0) / (2.0 * j1p + 1.0));
       const double coef2 = sqrt((2.0 * j2pp + 1.0) / (2.0 * j2p + 1.0));
       const double coef3 = sqrt((2.0 * l1 + 1.0) * (2.0 * l2 + 1.0)) / (8.0 * MQCT_PI * MQCT_PI);
       out = coef1 * coef2 * coef3 * m5;
+      if (x.ident) {
+        // pot_comp.f:8699-8708: 1/sqrt(1 + delta) per side (same level)
+        out = out / sqrt(1.0 + (x.same[chp] ? 1.0 : 0.0)) / sqrt(1.0 + (x.same[chpp] ? 1.0 : 0.0));
+      }
     }
   }
   A[idx] = out;
@@ -371,7 +427,7 @@ __global__ __launch_bounds__(256) void exp_matrix_kernel(
   }
 }
 
-int check_terms(int coll_type, int n, const int *a) {
+int check_terms(int coll_type, bool ipes, int n, const int *a) {
   // the checks CALC_EXPANSION_TERMS / EXPANSION_MATRIX_ELEMENT make before
   // using a term list (pot_comp.f:5060-5072, 5563-5612, 6978-6995)
   auto row = [&](int i) {
@@ -395,13 +451,22 @@ int check_terms(int coll_type, int n, const int *a) {
         return mqct_fail(MQCT_ERR_ARG, "The values of M2 should be positive only for M1 = 0. Please check the expansion term: " + row(i));
       if (t[1] > t[0] || std::abs(t[3]) > t[2])
         return mqct_fail(MQCT_ERR_ARG, "set_expansion_terms: |NJU| above L in term " + row(i));
+      if (ipes && t[0] < t[2])
+        return mqct_fail(MQCT_ERR_ARG, "The values of L2 should be less or equal to L1. Please check the expansion term: " + row(i));
+      if (ipes && t[0] == t[2] && std::abs(t[1]) < std::abs(t[3]))
+        return mqct_fail(MQCT_ERR_ARG, "The values of M1 should be greater or equal to abs(M2) for L1 = L2. Please check the expansion term: " + row(i));
+      // a term given twice, also as its planar mirror or (IDENTICAL_PES)
+      // with the partners exchanged (pot_comp.f:5591-5612, 5652-5684)
       for (int j = 0; j < n; j++) {
         if (j == i) continue;
         const int *u = a + 5 * j;
-        for (int pl = 0; pl < (t[1] == 0 && t[3] == 0 ? 1 : 2); pl++) {
-          int s = pl ? -1 : 1;
-          if (u[0] == t[0] && u[1] == s * t[1] && u[2] == t[2] && u[3] == s * t[3] && u[4] == t[4])
-            return mqct_fail(MQCT_ERR_ARG, "Found two same terms. Please check the expansion terms: " + row(i) + " " + row(j));
+        for (int sw = 0; sw < (ipes ? 2 : 1); sw++) {
+          int L1 = sw ? t[2] : t[0], N1 = sw ? t[3] : t[1], L2 = sw ? t[0] : t[2], N2 = sw ? t[1] : t[3];
+          for (int pl = 0; pl < (N1 == 0 && N2 == 0 ? 1 : 2); pl++) {
+            int s = pl ? -1 : 1;
+            if (u[0] == L1 && u[1] == s * N1 && u[2] == L2 && u[3] == s * N2 && u[4] == t[4])
+              return mqct_fail(MQCT_ERR_ARG, "Found two same terms. Please check the expansion terms: " + row(i) + " " + row(j));
+          }
         }
       }
     }
@@ -415,7 +480,7 @@ int check_terms(int coll_type, int n, const int *a) {
 extern "C" int mqct_hip_set_expansion_terms(mqct_hip_t *h, int n_terms, const int *a_top) {
   if (!h || n_terms <= 0 || !a_top) return mqct_fail(MQCT_ERR_ARG, "set_expansion_terms: empty term list");
   if (!h->basis_ok) return mqct_fail(MQCT_ERR_STATE, "set_expansion_terms: call set_basis first");
-  int rc = check_terms(h->coll_type, n_terms, a_top);
+  int rc = check_terms(h->coll_type, h->identical || h->identical_pes, n_terms, a_top);
   if (rc) return rc;
   HIP_TRY(hipSetDevice(h->device));
   h->n_terms = n_terms;
@@ -544,7 +609,7 @@ extern "C" int mqct_hip_expansion_build(mqct_hip_t *h, int n_r, const double *R_
     int64_t k = 0;
     for (int s1 = 0; s1 < h->n_states; s1++)
       for (int s2 = 0; s2 <= s1; s2++) {
-        if (h->m12[s1] != h->m12[s2]) continue;
+        if (!mqct_pair_ok(h, s1, s2)) continue;
         if (ind_mat[2 * k] != s1 + 1 || ind_mat[2 * k + 1] != s2 + 1)
           return mqct_fail(MQCT_ERR_ARG, "expansion_build: ind_mat is not the INI_ARRAYS order");
         pairs[k] = make_int2(s1, s2);
@@ -609,6 +674,11 @@ extern "C" int mqct_hip_expansion_build(mqct_hip_t *h, int n_r, const double *R_
   x.cgt = h->d_cgt.as<double>();
   x.ca = ca;
   x.cc = cc;
+  x.ident = h->identical ? 1 : 0;
+  x.ipes = (h->identical || h->identical_pes) ? 1 : 0;
+  x.par = h->identical ? h->d_xpar.as<int>() : nullptr;
+  x.pbk = h->identical ? h->d_xpar.as<int>() + h->n_states : nullptr;
+  x.same = h->identical ? h->d_same.as<int>() : nullptr;
   HIP_TRY(hipEventRecord(h->ev[0], h->stream));
   angular_kernel<<<nblk(T * ntp, 256), 256, 0, h->stream>>>(x, h->d_expA.as<double>());
   HIP_TRY(hipGetLastError());

@@ -124,6 +124,8 @@ extern "C" int mqct_hip_destroy(mqct_hip_t *h) {
 }
 
 // ---------------------------------------------------------------------------
+static int build_blocks(mqct_hip *h);
+
 extern "C" int mqct_hip_set_basis(mqct_hip_t *h, int coll_type, int n_states,
                                   const int *j12, const int *m12,
                                   const int *indx_chann, int n_channels,
@@ -171,9 +173,34 @@ extern "C" int mqct_hip_set_basis(mqct_hip_t *h, int coll_type, int n_states,
   else h->eig1.clear();
   if (eig2) h->eig2.assign(eig2, eig2 + (size_t)n_channels * ldvec);
   else h->eig2.clear();
-  // m-blocks in ascending m, states ascending inside a block
-  std::map<int, std::vector<int>> blocks;
-  for (int s = 0; s < n_states; s++) blocks[m12[s]].push_back(s);
+  h->identical = h->identical_pes = false;
+  h->parity_state.clear();
+  h->parity_state_bk.clear();
+  h->same_level.clear();
+  int rc = build_blocks(h);
+  if (rc) return rc;
+  if (!h->eig1.empty() && (rc = upload(h, h->d_eig1, h->eig1.data(), h->eig1.size()))) return rc;
+  if (!h->eig2.empty() && (rc = upload(h, h->d_eig2, h->eig2.data(), h->eig2.size()))) return rc;
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  h->basis_ok = true;
+  h->psi_fresh = false;
+  h->built = false;
+  h->n_terms = 0;
+  h->terms_nr = 0;
+  for (int &v : h->cvar) v = -1;
+  return MQCT_OK;
+}
+
+// Blocks of the pair structure: states of equal m12 (and, for identical
+// partners, equal exchange parity) in ascending m12 then parity, states
+// ascending inside a block; kbase in INI_ARRAYS' pair order.
+static int build_blocks(mqct_hip *h) {
+  const int n_states = h->n_states;
+  const int *m12 = h->m12.data();
+  const int *indx_chann = h->indx_chann.data();
+  std::map<std::pair<int, int>, std::vector<int>> blocks;
+  for (int s = 0; s < n_states; s++)
+    blocks[std::make_pair(m12[s], h->identical ? -h->parity_state[s] : 0)].push_back(s);
   h->pos_state.clear();
   h->block_start.clear();
   std::vector<int> rank(n_states), block_of;
@@ -215,12 +242,54 @@ extern "C" int mqct_hip_set_basis(mqct_hip_t *h, int coll_type, int n_states,
   if ((rc = upload(h, h->d_pos_state, h->pos_state.data(), n_states))) return rc;
   if ((rc = upload(h, h->d_kbase, kb.data(), kb.size()))) return rc;
   if ((rc = upload(h, h->d_chan, chan.data(), chan.size()))) return rc;
-  if (!h->eig1.empty() && (rc = upload(h, h->d_eig1, h->eig1.data(), h->eig1.size()))) return rc;
-  if (!h->eig2.empty() && (rc = upload(h, h->d_eig2, h->eig2.data(), h->eig2.size()))) return rc;
+  return MQCT_OK;
+}
+
+extern "C" int mqct_hip_set_identical(mqct_hip_t *h, int identical_pes,
+                                      const int *parity_state,
+                                      const int *parity_state_bk,
+                                      const int *same_level) {
+  if (!h) return mqct_fail(MQCT_ERR_ARG, "set_identical: null handle");
+  if (!h->basis_ok || h->coll_type != 0)
+    return mqct_fail(MQCT_ERR_STATE, "ERROR: EXCHANGE SYMMETRY DEFINED FOR NON-IDENTICAL PARTICLES");
+  const bool ident = parity_state != nullptr;
+  if (ident && (!parity_state_bk || !same_level))
+    return mqct_fail(MQCT_ERR_ARG, "set_identical: parity_state_bk and same_level required");
+  if (ident && !identical_pes)
+    return mqct_fail(MQCT_ERR_ARG, "ERROR: INDISTINGUISHABLE CALCULATIONS SHOULD BE DONE USING IDENTICAL PES");
+  HIP_TRY(hipSetDevice(h->device));
+  const int n = h->n_states;
+  if (ident) {
+    for (int s = 0; s < n; s++)
+      if ((parity_state[s] != 1 && parity_state[s] != -1) ||
+          (parity_state_bk[s] != 1 && parity_state_bk[s] != -1))
+        return mqct_fail(MQCT_ERR_ARG, "set_identical: parities must be +1 or -1");
+    h->parity_state.assign(parity_state, parity_state + n);
+    h->parity_state_bk.assign(parity_state_bk, parity_state_bk + n);
+    h->same_level.assign(same_level, same_level + h->n_channels);
+  } else {
+    h->parity_state.clear();
+    h->parity_state_bk.clear();
+    h->same_level.clear();
+  }
+  h->identical = ident;
+  h->identical_pes = identical_pes != 0;
+  int rc = build_blocks(h);
+  if (rc) return rc;
+  if (ident) {
+    std::vector<int> par(2 * (size_t)n);
+    for (int s = 0; s < n; s++) {
+      par[s] = h->parity_state[s];
+      par[n + s] = h->parity_state_bk[s];
+    }
+    if ((rc = upload(h, h->d_xpar, par.data(), par.size()))) return rc;
+    if ((rc = upload(h, h->d_same, h->same_level.data(), h->same_level.size()))) return rc;
+  }
   HIP_TRY(hipStreamSynchronize(h->stream));
-  h->basis_ok = true;
   h->psi_fresh = false;
   h->built = false;
+  h->n_terms = 0;
+  h->terms_nr = 0;
   for (int &v : h->cvar) v = -1;
   return MQCT_OK;
 }
@@ -494,12 +563,15 @@ extern "C" int mqct_hip_matrix_build(mqct_hip_t *h, int precision,
     return mqct_fail(MQCT_ERR_ARG, "ERROR: total_size does not match the basis (INI_ARRAYS pair count)");
   if (ir_cut1 < 1 || ir_cut1 > h->n_r || ir_cut2 < 1 || ir_cut2 > h->n_r)
     return mqct_fail(MQCT_ERR_ARG, "matrix_build: cutoff R index out of range");
+  if (h->identical)
+    return mqct_fail(MQCT_ERR_ARG, "matrix_build: the quadrature build of identical partners is not implemented; "
+                                   "build their matrix through the expansion (IDENTICAL_PES)");
   if (ind_mat) {
     // must be INI_ARRAYS' order: st1 ascending, st2 ascending, equal m12
     int64_t k = 0;
     for (int s1 = 0; s1 < h->n_states; s1++)
       for (int s2 = 0; s2 <= s1; s2++) {
-        if (h->m12[s1] != h->m12[s2]) continue;
+        if (!mqct_pair_ok(h, s1, s2)) continue;
         if (ind_mat[2 * k] != s1 + 1 || ind_mat[2 * k + 1] != s2 + 1)
           return mqct_fail(MQCT_ERR_ARG, "matrix_build: ind_mat is not the INI_ARRAYS order");
         k++;

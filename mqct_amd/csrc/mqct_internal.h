@@ -100,6 +100,12 @@ struct mqct_hip {
   // M(R) - M(R_max) (matrix_ini.f:2226-2230, 6603-6608)
   bool mij_shift = true;
 
+  // ---- identical partners (type 0): IDENTICAL= / IDENTICAL_PES= ----
+  bool identical = false, identical_pes = false;
+  std::vector<int> parity_state, parity_state_bk, same_level;
+  DevBuf d_xpar;                     // [parity_state][parity_state_bk] per state
+  DevBuf d_same;                     // per channel: both partners in one level
+
   // ---- PES expansion (expansion.hip) ----
   int n_terms = 0;
   std::vector<int> atop;             // A_TOP rows, [n_terms][5]
@@ -207,6 +213,13 @@ inline int upload(mqct_hip *h, DevBuf &b, const T *src, size_t n) {
 
 // R grid of the handle (>= 2 increasing points), host copy and device copy.
 int mqct_set_r(mqct_hip *h, int n_r, const double *R);
+
+// States s1 >= s2 (0-based) form a pair of the matrix: equal m12 and, for
+// identical partners, equal exchange parity (matrix_ini.f:553-630).
+inline bool mqct_pair_ok(const mqct_hip *h, int s1, int s2) {
+  return h->m12[s1] == h->m12[s2] &&
+         (!h->identical || h->parity_state[s1] == h->parity_state[s2]);
+}
 
 // Kernel launchers (defined in the .hip files).
 int mqct_build_psi(mqct_hip *h);

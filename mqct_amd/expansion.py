@@ -159,11 +159,16 @@ class Expansion:
     """The expansion path of one handle whose basis (and, for project(), grid
     and V grid) a MatrixBuilder has set."""
 
-    def __init__(self, dev, wl, terms):
+    def __init__(self, dev, wl, terms, identical_pes=False):
+        """identical_pes (IDENTICAL_PES=YES): the terms are those of an
+        exchange-symmetric PES (L2 <= L1); implied for a basis of identical
+        partners, which MatrixBuilder has declared to the handle."""
         self.dev, self.lib, self.h = dev, dev.lib, dev.h
         self.wl = wl
         self.terms = [tuple(t) for t in terms]
         self.a_top = a_top_rows(self.terms)
+        if identical_pes and not getattr(wl.basis, "identical", False):
+            L.check(self.lib.mqct_hip_set_identical(self.h, 1, None, None, None), "set_identical")
         L.check(self.lib.mqct_hip_set_expansion_terms(self.h, len(self.terms), L.iptr(self.a_top)),
                 "set_expansion_terms")
 

@@ -91,6 +91,12 @@ class MatrixBuilder:
         L.check(self.lib.mqct_hip_set_basis(self.h, b.coll_type, b.n_states, L.iptr(j12), L.iptr(m12),
                                             L.iptr(ich), b.n_channels, L.iptr(j1), L.iptr(j2),
                                             b.ldvec, L.dptr(e1), L.dptr(e2)), "set_basis")
+        if getattr(b, "identical", False):
+            same = ((b.j1_ch == b.j2_ch) & (b.ka1_ch == b.ka2_ch) & (b.kc1_ch == b.kc2_ch)).astype(np.int32)
+            ps, pb = L.as_i32(b.parity_state), L.as_i32(b.parity_state_bk)
+            self._keep += [ps, pb, same]
+            L.check(self.lib.mqct_hip_set_identical(self.h, 1, L.iptr(ps), L.iptr(pb), L.iptr(same)),
+                    "set_identical")
         gs, keep = grid_struct(wl.grid)
         L.check(self.lib.mqct_hip_set_grid(self.h, ctypes.byref(gs)), "set_grid")
         R = L.as_f64(wl.R)

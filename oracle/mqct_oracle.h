@@ -78,6 +78,14 @@ typedef struct {
                                  type 4: [n_r][n_beta][n_gamma]
                                  type 0: [n_r][a2][b1][b2][g1][g2] */
   int abs_mode;               /* 1: sum |integrand| (condition scale) */
+  /* type 0 identical partners (IDENTICAL=YES) and IDENTICAL_PES=YES */
+  int identical, identical_pes;
+  const int *parity_state;    /* per state, +-1 (matrix_ini.f:504-528) */
+  const int *parity_state_bk; /* per state, +-1 */
+  const int *ka1_ch, *kc1_ch, *ka2_ch, *kc2_ch;  /* per channel */
+  /* planted mistakes for the tests that show a check fails (0 = none):
+   * bit 0 runs the identical-partner m sum over -j1'..j1' only */
+  int plant;
 } or_matrix_ctx;
 
 double or_integrator_temp(const or_matrix_ctx *c, const int *ind_mat,

@@ -14,8 +14,8 @@
  * Term rows are A_TOP(1:5, t): type 1 (L), type 4 (L, NJU), type 0
  * (L1, NJU1, L2, NJU2, L); unused slots are 0.  Expansion coefficients are
  * expansion_terms(n_r, n_terms) in the reference's column-major layout,
- * i.e. terms[t * n_r + ir] here.  Distinguishable partners only (the
- * IDENTICAL branch of type 0, pot_comp.f:8413-8540, is not restated).
+ * i.e. terms[t * n_r + ir] here.  Type 0 covers distinguishable
+ * partners, IDENTICAL_PES=YES and identical partners (IDENTICAL=YES).
  */
 #include <math.h>
 #include <stdlib.h>
@@ -220,62 +220,129 @@ static double angular_parts(const or_matrix_ctx *c, int n_terms,
     }
     return coef1;
   }
-  /* type 0, distinguishable partners: pot_comp.f:8257-8412, 8541-8716 */
+  /* type 0: pot_comp.f:8257-8716.  Symmetry 1 always; symmetry 2 with
+   * IDENTICAL_PES (the swapped term l1 <-> l2); symmetries 3 and 4 with
+   * identical partners (the swapped levels), weighted by the exchange
+   * parity of the initial state. */
   int j12p = c->j12[stp - 1], j12pp = c->j12[stpp - 1];
   int j1p = c->j1_ch[chp - 1], j2p = c->j2_ch[chp - 1];
   int j1pp = c->j1_ch[chpp - 1], j2pp = c->j2_ch[chpp - 1];
+  const int ident = c->identical, ipes = c->identical_pes || c->identical;
   double coef1 = sqrt((2.0 * j1pp + 1.0) / (2.0 * j1p + 1.0));
   double coef2 = sqrt((2.0 * j2pp + 1.0) / (2.0 * j2p + 1.0));
+  int par_p = 1, bk1 = 1;
+  if (ident) {
+    par_p = c->parity_state[stp - 1];
+    bk1 = c->parity_state_bk[stp - 1];
+    if (stp == stpp && j1p == j2p && c->ka1_ch[chp - 1] == c->ka2_ch[chp - 1] &&
+        c->kc1_ch[chp - 1] == c->kc2_ch[chp - 1])
+      bk1 = (int)sgn_pow(j12p);
+  }
   for (int i = 0; i < n_terms; i++) {
     const int *t = a_top + 5 * i;
-    int l1 = t[0], l2 = t[2], l = t[4];
-    int n1 = t[1], n2 = t[3];
+    const int l1 = t[0], l2 = t[2], l = t[4];
+    const int n1o = t[1], n2o = t[3];
     double coef3 = sqrt((2.0 * l1 + 1.0) * (2.0 * l2 + 1.0)) / (8.0 * PI * PI);
-    if (!(triang(j1pp, l1, j1p) && triang(j2pp, l2, j2p))) continue;
-    double cgjk = 0.0;
-    int nplanar = 2 - (n1 == 0 && n2 == 0);
-    for (int pl = 1; pl <= nplanar; pl++) {
-      double sc = 1.0;
-      if (pl == 2) {
-        sc = sgn_pow(l1 + l2 + l + n1 + n2);
-        n1 = -n1;
-        n2 = -n2;
+    const int npl = 2 - (n1o == 0 && n2o == 0);
+    int sym[4] = {0, 0, 0, 0};
+    double cgjk[4] = {0, 0, 0, 0};
+    /* the four K sums: (molecule of the p-side level, term index) per
+     * symmetry: 1 (l1 on 1, l2 on 2), 2 (l2 on 1, l1 on 2), 3 and 4 the
+     * same with the p-side levels of the two molecules exchanged */
+    for (int s = 0; s < 4; s++) {
+      if (s == 1 && !ipes) continue;
+      if (s >= 2 && !ident) continue;
+      if ((s == 1 || s == 3) && l1 == l2 && abs(n1o) == abs(n2o)) continue;
+      const int la = (s == 0 || s == 2) ? l1 : l2;   /* on molecule 1 (pp side) */
+      const int lb = (s == 0 || s == 2) ? l2 : l1;   /* on molecule 2 (pp side) */
+      const int ja = s < 2 ? j1p : j2p;               /* p-side level paired with 1 */
+      const int jb = s < 2 ? j2p : j1p;
+      const int wa = s < 2 ? 1 : 2, wb = s < 2 ? 2 : 1;
+      if (!(triang(j1pp, la, ja) && triang(j2pp, lb, jb))) continue;
+      sym[s] = 1;
+      for (int pl = 1; pl <= npl; pl++) {
+        int n1 = n1o, n2 = n2o;
+        double sc = 1.0;
+        if (pl == 2) {
+          sc = sgn_pow(l1 + l2 + l + n1o + n2o);
+          n1 = -n1o;
+          n2 = -n2o;
+        }
+        if (s == 1 || s == 3) sc *= sgn_pow(l1 + l2);
+        const int na = (s == 0 || s == 2) ? n1 : n2;
+        const int nb = (s == 0 || s == 2) ? n2 : n1;
+        double db1 = 0.0, db2 = 0.0;
+        for (int kp = -ja; kp <= ja; kp++) {
+          int kpp = kp - na;
+          if (abs(kpp) > j1pp) continue;
+          db1 += or_cg(j1pp, la, ja, kpp, na, kp) * eigc(c, wa, chp, ja, kp) *
+                 eigc(c, 1, chpp, j1pp, kpp);
+        }
+        for (int kp = -jb; kp <= jb; kp++) {
+          int kpp = kp - nb;
+          if (abs(kpp) > j2pp) continue;
+          db2 += or_cg(j2pp, lb, jb, kpp, nb, kp) * eigc(c, wb, chp, jb, kp) *
+                 eigc(c, 2, chpp, j2pp, kpp);
+        }
+        cgjk[s] += db1 * db2 * sc * (s >= 2 ? par_p * bk1 : 1);
       }
-      double db1 = 0.0, db2 = 0.0;
-      for (int k1p = -j1p; k1p <= j1p; k1p++) {
-        int k1pp = k1p - n1;
-        if (abs(k1pp) > j1pp) continue;
-        db1 += or_cg(j1pp, l1, j1p, k1pp, n1, k1p) * eigc(c, 1, chp, j1p, k1p) *
-               eigc(c, 1, chpp, j1pp, k1pp);
-      }
-      for (int k2p = -j2p; k2p <= j2p; k2p++) {
-        int k2pp = k2p - n2;
-        if (abs(k2pp) > j2pp) continue;
-        db2 += or_cg(j2pp, l2, j2p, k2pp, n2, k2p) * eigc(c, 2, chp, j2p, k2p) *
-               eigc(c, 2, chpp, j2pp, k2pp);
-      }
-      cgjk += db1 * db2 * sc;
     }
+    if (!(sym[0] || sym[1] || sym[2] || sym[3])) continue;
+    if (!ident && !(sym[0] || sym[1])) continue;
     double m5 = 0.0;
     int mx = l1 < l2 ? l1 : l2;
-    for (int mp = -j1p; mp <= j1p; mp++) {
-      if (abs(m - mp) > j2p) continue;
-      double cg_p = or_cg(j1p, j2p, j12p, mp, m - mp, m);
-      double m1 = 0.0;
+    /* pot_comp.f:8589 runs mp over -j1'..j1' only; the exchanged pair
+     * state |j2' mp; j1' m-mp> needs -j2'..j2' as well, and the reference's
+     * own H2O+H2O IDENTICAL MTRX.DAT holds those terms, so with identical
+     * partners mp covers -max(j1', j2')..max(j1', j2') (the direct terms
+     * vanish for |mp| > j1' through their Clebsch-Gordan coefficients) */
+    const int mpx = ident && j2p > j1p && !(c->plant & 1) ? j2p : j1p;
+    for (int mp = -mpx; mp <= mpx; mp++) {
+      double cg_p = 0.0, cg_x = 0.0;
+      if (abs(m - mp) > j2p) {
+        if (!ident) continue;
+      } else {
+        cg_p = or_cg(j1p, j2p, j12p, mp, m - mp, m);
+      }
+      if (ident) {
+        if (abs(m - mp) > j1p && abs(m - mp) > j2p) continue;
+        cg_x = or_cg(j2p, j1p, j12p, mp, m - mp, m);
+      }
+      double acc[4] = {0, 0, 0, 0};
       for (int me = -mx; me <= mx; me++) {
         int mpp = mp - me;
         if (abs(mpp) > j1pp) continue;
         if (abs(m - mpp) > j2pp) continue;
         double cg_pp = or_cg(j1pp, j2pp, j12pp, mpp, m - mpp, m);
-        double a = or_cg(j1pp, l1, j1p, mpp, me, mp);
-        double b = or_cg(j2pp, l2, j2p, m - mpp, -me, m - mp);
-        m1 += or_cg(l1, l2, l, me, -me, 0) * cg_pp * a * b;
+        double cl = or_cg(l1, l2, l, me, -me, 0);
+        if (sym[0])
+          acc[0] += cl * cg_pp * or_cg(j1pp, l1, j1p, mpp, me, mp) * or_cg(j2pp, l2, j2p, m - mpp, -me, m - mp);
+        if (sym[1])
+          acc[1] += cl * cg_pp * or_cg(j1pp, l2, j1p, mpp, me, mp) * or_cg(j2pp, l1, j2p, m - mpp, -me, m - mp);
+        if (abs(mp) <= j2p) {
+          if (sym[2])
+            acc[2] += cl * cg_pp * or_cg(j1pp, l1, j2p, mpp, me, mp) * or_cg(j2pp, l2, j1p, m - mpp, -me, m - mp);
+          if (sym[3])
+            acc[3] += cl * cg_pp * or_cg(j1pp, l2, j2p, mpp, me, mp) * or_cg(j2pp, l1, j1p, m - mpp, -me, m - mp);
+        }
       }
-      m5 += m1 * cg_p * cgjk;
+      if (sym[0]) m5 += acc[0] * cg_p * cgjk[0];
+      if (sym[1]) m5 += acc[1] * cg_p * cgjk[1];
+      if (ident) {
+        if (sym[2]) m5 += acc[2] * cg_x * cgjk[2];
+        if (sym[3]) m5 += acc[3] * cg_x * cgjk[3];
+      }
     }
     ang[i] = coef3 * m5;
   }
-  return coef1 * coef2;
+  double f = coef1 * coef2;
+  if (ident) {
+    /* pot_comp.f:8699-8708 */
+    int dp = c->ka1_ch[chp - 1] == c->ka2_ch[chp - 1] && c->kc1_ch[chp - 1] == c->kc2_ch[chp - 1];
+    int dpp = c->ka1_ch[chpp - 1] == c->ka2_ch[chpp - 1] && c->kc1_ch[chpp - 1] == c->kc2_ch[chpp - 1];
+    f = f / sqrt(1.0 + (j1p == j2p) * dp) / sqrt(1.0 + (j1pp == j2pp) * dpp);
+  }
+  return f;
 }
 
 /* EXPANSION_MATRIX_ELEMENT for pair k at every R point. */

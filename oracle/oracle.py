@@ -44,6 +44,10 @@ class MatrixCtx(ctypes.Structure):
         ("n_r", _i),
         ("V", _dp),
         ("abs_mode", _i),
+        ("identical", _i), ("identical_pes", _i),
+        ("parity_state", _ip), ("parity_state_bk", _ip),
+        ("ka1_ch", _ip), ("kc1_ch", _ip), ("ka2_ch", _ip), ("kc2_ch", _ip),
+        ("plant", _i),
     ]
 
 
@@ -296,6 +300,17 @@ class MatrixOracle:
         self.V = k(V, f64)
         c.V = dp(self.V)
         self.c = c
+
+    def set_identical(self, basis, identical=True, identical_pes=True):
+        """IDENTICAL= / IDENTICAL_PES= of a type-0 basis (mqct_amd.basis)."""
+        c = self.c
+        c.identical, c.identical_pes = int(identical), int(identical_pes or identical)
+        keep = lambda a: ip(self.keep.append(i32(a)) or self.keep[-1])
+        if identical:
+            c.parity_state = keep(basis.parity_state)
+            c.parity_state_bk = keep(basis.parity_state_bk)
+        c.ka1_ch, c.kc1_ch = keep(basis.ka1_ch), keep(basis.kc1_ch)
+        c.ka2_ch, c.kc2_ch = keep(basis.ka2_ch), keep(basis.kc2_ch)
 
     def element(self, ind_mat, k, i_r):
         ind = i32(ind_mat)

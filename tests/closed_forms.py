@@ -134,3 +134,37 @@ def h2ohe_reference_system():
     assert np.array_equal(b.ind_mat, np.array(fx["pairs"])[:, 1:3])
     g = B.grid_type4(vg["grid"]["n_beta"], vg["grid"]["n_gamma"])
     return b, g, np.array(vg["R"]), np.array(vg["V"]), np.array(vg["mij"])
+
+
+def h2o_h2o_identical_reference_system():
+    """The reference's SYS_TYPE 0 example EXAMPLES_IO/SYS_TYPE_0/
+    H2O_H2O_Identical as data (copied under tests/golden/expansion): the
+    input's CHNLS_LIST, rotational constants and 152 IDENTICAL_PES terms,
+    its PES_EXPAN_TERMS.DAT (2 R points, kcal/mol) and the states, pairs and
+    Mij (Hartree, 12 digits) of its MTRX.DAT.  Returns a dict."""
+    import os
+    import re
+    from mqct_amd import basis as B
+    from mqct_amd import expansion as X
+    d = os.path.join(os.path.dirname(__file__), "golden", "expansion")
+    inp = open(os.path.join(d, "h2o_h2o_identical.inp")).read()
+    num = lambda k: float(re.search(k + r"\s*=\s*([-0-9.]+)", inp).group(1))
+    A, Bc, C = num("A1"), num("B1"), num("C1")
+    i0 = inp.index("CHNLS_LIST=")
+    rows = re.findall(r"^\s*(\d+),(\d+),(\d+),(\d+),(\d+),(\d+),\s*$", inp[i0:inp.index("INIT_CHNL")], re.M)
+    rows = [tuple(int(v) for v in r) for r in rows]
+    i1 = inp.index("TERMS =")
+    terms = X.parse_terms_list(inp[inp.index("\n", i1) + 1:inp.index("$END", i1)])
+    R, coeffs = X.read_pes_expan_terms(os.path.join(d, "h2o_h2o_identical_PES_EXPAN_TERMS.DAT"), len(terms))
+    lines = open(os.path.join(d, "h2o_h2o_identical_MTRX.DAT")).read().splitlines()
+    ns = int(lines[2].split("=")[1])
+    T = int(lines[3].split("=")[1])
+    st = np.array([[int(x) for x in ln.split()] for ln in lines[6:6 + ns]])
+    pr = np.array([[int(x) for x in ln.split()] for ln in lines[7 + ns:7 + ns + T]])
+    el = lines[7 + ns + T + 1 + len(R) + 1:]
+    mij = np.zeros((T, len(R)))
+    for ln in el:
+        k, ir, v = ln.split()
+        mij[int(k) - 1, int(ir) - 1] = float(v)
+    return {"rows": rows, "A": A, "B": Bc, "C": C, "terms": terms, "R": R, "coeffs": coeffs,
+            "states": st, "pairs": pr[:, 1:], "mij": mij, "conv_unit_e": 1.0 / 627.509469}

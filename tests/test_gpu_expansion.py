@@ -150,3 +150,77 @@ def test_c3_basis_sampled_against_oracle(dev):
     print(f"c3 basis: {b.total_size} pairs x {len(terms)} terms: angular {t['angular_ms']:.1f} ms, "
           f"GEMM {t['gemm_ms']:.1f} ms; sampled diff {worst:.2e} of {scale:.2e}")
     assert scale > 0 and worst <= 1e-12 * scale
+
+
+def _golden_identical_workload(signs):
+    import closed_forms as CF
+    from mqct_amd import basis as B
+    g = CF.h2o_h2o_identical_reference_system()
+    b = B.channels_type0_list(g["rows"], g["A"], g["B"], g["C"])
+    for i, r in enumerate(g["rows"]):
+        b.eig1[i] *= signs[r[:3]]
+        b.eig2[i] *= signs[r[3:]]
+    B.ini_arrays(b, identical=True)
+    base = W.two_asym_tops(1, 1, n_ang=2, n_r=2)
+    wl = W.Workload("h2o_h2o_identical", b, base.grid, np.asarray(g["R"]), base.pes_kind, base.pes_par, 9.00764,
+                    conv_unit_e=g["conv_unit_e"])
+    return g, wl
+
+
+def test_identical_h2o_h2o_matches_reference_mtrx_dat(dev):
+    """The reference's IDENTICAL=YES H2O+H2O example through the HIP
+    expansion build: all 644 Mij of its MTRX.DAT within 1e-9 of the largest
+    (eigenvector signs as the CPU pin fits them), the oracle's values within
+    1e-13 of the largest, and the cutoff mask equal to the file's zero
+    elements."""
+    signs = {(0, 0, 0): 1.0, (1, 1, 1): -1.0, (2, 0, 2): 1.0}
+    g, wl = _golden_identical_workload(signs)
+    MatrixBuilder(dev, wl)
+    ex = X.Expansion(dev, wl, g["terms"])
+    M, D, S = ex.build(coeffs=g["coeffs"], R=g["R"])
+    scale = np.max(np.abs(g["mij"]))
+    d = np.max(np.abs(M - g["mij"]))
+    b = wl.basis
+    grid = {"n_beta": 1, "n_gamma": 1, "n_alpha2": 1, "n_beta2": 1, "n_gamma2": 1}
+    o = O.MatrixOracle(0, b.j12, b.m12, b.indx_chann, b.j1_ch, grid, g["R"], np.zeros(1), eig1=b.eig1, eig2=b.eig2,
+                       j2_ch=b.j2_ch)
+    o.set_identical(b)
+    Mo = np.array([o.expansion_element(g["terms"], g["coeffs"], b.ind_mat, k) for k in range(b.total_size)]) \
+        * g["conv_unit_e"]
+    do = np.max(np.abs(M - Mo))
+    print(f"identical H2O+H2O on the GPU: vs MTRX.DAT {d:.2e}, vs oracle {do:.2e} of {scale:.2e}")
+    # the file's 26 elements that vanish by symmetry are the ones the cutoff skips
+    assert np.array_equal(S.astype(bool), (g["mij"] == 0.0).all(axis=1)) and S.sum() == 26
+    assert d <= 1e-9 * scale and do <= 1e-13 * scale
+
+
+def test_identical_basis_refused_by_quadrature_build(dev):
+    signs = {(0, 0, 0): 1.0, (1, 1, 1): 1.0, (2, 0, 2): 1.0}
+    _, wl = _golden_identical_workload(signs)
+    mb = MatrixBuilder(dev, wl)
+    with pytest.raises(L.MqctError, match="identical partners"):
+        mb.build("fp64")
+
+
+def test_identical_pes_terms_match_oracle(dev):
+    """IDENTICAL_PES=YES with distinguishable partners: the second symmetry
+    term (the term with the partners' expansions exchanged) on the GPU
+    against the oracle, 1e-12 of the largest element."""
+    wl = W.two_asym_tops(2, 1, n_ang=2, n_r=24)
+    terms = [t for t in X.terms_type0(2, 2, l_max=2) if t[0] > t[2] or (t[0] == t[2] and abs(t[1]) >= abs(t[3]))]
+    R = np.asarray(wl.R)
+    C = np.random.default_rng(41).normal(size=(len(terms), len(R))) * (300.0 * np.exp(-0.9 * (R - 4.0)))[None, :]
+    MatrixBuilder(dev, wl)
+    ex = X.Expansion(dev, wl, terms, identical_pes=True)
+    M, D, S = ex.build(coeffs=C, R=R)
+    o = O.oracle_for_workload(wl, np.zeros(len(R) * wl.grid.G))
+    o.set_identical(wl.basis, identical=False, identical_pes=True)
+    b = wl.basis
+    Mo, Do, So = o.expansion_matrix_make(terms, C, b.ind_mat, wl.conv_unit_e, wl.mij_zero_cut, *wl.ir_cut)
+    o.set_identical(wl.basis, identical=False, identical_pes=False)
+    M1 = np.array([o.expansion_element(terms, C, b.ind_mat, k) for k in range(0, b.total_size, 97)]) * wl.conv_unit_e
+    d = np.max(np.abs(M - Mo)) / np.max(np.abs(Mo))
+    print(f"IDENTICAL_PES: {len(terms)} terms, element diff {d:.2e}")
+    assert d <= 1e-12 and np.array_equal(S, So)
+    # the second symmetry term contributes
+    assert np.max(np.abs(M1 - Mo[::97])) > 1e-3 * np.max(np.abs(Mo))
