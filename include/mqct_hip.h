@@ -275,6 +275,13 @@ int mqct_hip_propagate(mqct_hip_t *h, int mode, int n_traj,
                        mqct_traj_stat_t *stat_out);
 int mqct_hip_propagate_timing(mqct_hip_t *h, double *ms);
 
+/* Progress of long batched propagations: when seconds > 0, the host loop
+ * of the batched form prints the number of trajectories still running to
+ * stderr at most every `seconds`, from the device counter it reads between
+ * graph replays anyway (the reference prints per-trajectory progress to its
+ * log instead, propagator.f).  0 (the default) prints nothing. */
+int mqct_hip_set_progress(mqct_hip_t *h, double seconds);
+
 /* Kernel timing of the batched form (a diagnostic with no counterpart in
  * the reference).  on=1: the following batched propagations record HIP
  * events on their stream around every launch of the interval GEMM (inside
@@ -405,6 +412,34 @@ int mqct_hip_write_mtrx_uf(const char *path, int coll_type, int n_channels,
 int mqct_hip_read_mtrx_uf(const char *path, int64_t *hdr, int *indx_chann,
                           int *j12, int *m12, int *chan_qn, int *ind_mat,
                           double *R_com, double *mat_el);
+
+/* The parallel-IO matrix directory of a split build (MATRIX_TRUNCATED,
+ * UNFORMAT=NO or YES): MATRIX_INFO.DAT with the state tables
+ * (bk_print_matrix_info, matrix_ini.f:6836-7144; types 1, 4, 0, with the
+ * exchange parity column for identical partners), and the matrix as
+ * n_files pairs MIJ_<k_st>_<k_fn>.DAT / Ind_<k_st>_<k_fn>.DAT over the
+ * reference's rank split of 1..total_size, holding the elements whose |M|
+ * at the two cutoff R points exceeds mij_zero_cut (Hartree), with
+ * MATRIX_NONZERO_INDEX.DAT (the cut in cm^-1) and MATRIX_FILE_INDEX.DAT
+ * (bk_print_matrix, matrix_ini.f:7147-7244; 1545-1575; 9053-9145).
+ * mqct_hip_read_mij_dir is bk_read_matrix (matrix_ini.f:7246-7525) for one
+ * reader: the non-zero elements in file order with their ind_mat pairs
+ * (mat_el[n_nonzero][n_r], ind_mat[n_nonzero][2]); with mat_el and ind_mat
+ * NULL only n_nonzero and the cut are returned.  MIJ_SHIFT and the spline
+ * that follow the read are mqct_hip_mij_spline's. */
+int mqct_hip_write_mij_info(const char *dir, int unformatted, int coll_type,
+                            int n_channels, int n_states, int64_t total_size,
+                            int n_r, const int *indx_chann, const int *j12,
+                            const int *m12, const int *chan_qn,
+                            const int *parity_state, const int *ind_mat,
+                            const double *R_com);
+int mqct_hip_write_mij_dir(const char *dir, int unformatted, int n_files,
+                           int64_t total_size, int n_r, const int *ind_mat,
+                           const double *mat_el, double mij_zero_cut,
+                           int ir_cut1, int ir_cut2);
+int mqct_hip_read_mij_dir(const char *dir, int unformatted, int n_r,
+                          int64_t cap, int64_t *n_nonzero, double *cutoff_cm,
+                          int *ind_mat, double *mat_el);
 
 /* The classical paths of the first AT run, in the reference's files:
  * dir/AT_<j12>_<m12>_<l>_<parity> (one Fortran unformatted record of 8

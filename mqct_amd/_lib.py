@@ -83,6 +83,15 @@ SIGNATURES = {
     "mqct_hip_set_mij_shift": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "mqct_hip_mij_spline": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, c_dbl_p, ctypes.c_int64, c_dbl_p,
                                            c_dbl_p]),
+    "mqct_hip_write_mij_info": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                               ctypes.c_int, ctypes.c_int64, ctypes.c_int, c_int_p, c_int_p,
+                                               c_int_p, c_int_p, c_int_p, c_int_p, c_dbl_p]),
+    "mqct_hip_write_mij_dir": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int64,
+                                              ctypes.c_int, c_int_p, c_dbl_p, ctypes.c_double, ctypes.c_int,
+                                              ctypes.c_int]),
+    "mqct_hip_read_mij_dir": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int64,
+                                             c_i64_p, c_dbl_p, c_int_p, c_dbl_p]),
+    "mqct_hip_set_progress": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double]),
     "mqct_hip_set_identical": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, c_int_p, c_int_p, c_int_p]),
     "mqct_hip_set_expansion_terms": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, c_int_p]),
     "mqct_hip_expansion_project": (ctypes.c_int, [ctypes.c_void_p, c_dbl_p]),

@@ -163,6 +163,46 @@ def read_mtrx_uf(path):
             "indx_chann": ich, "j12": j12, "m12": m12, "chan_qn": qn, "ind_mat": ind, "R": R, "mat_el": M}
 
 
+def write_mij_dir(path, wl, mat_el, n_files, unformatted=False, parity_state=None):
+    """The parallel-IO matrix directory (MATRIX_TRUNCATED) of a split build:
+    MATRIX_INFO.DAT, then the elements above MIJ_CUTOFF in n_files
+    MIJ_/Ind_ file pairs with MATRIX_NONZERO_INDEX.DAT and
+    MATRIX_FILE_INDEX.DAT (matrix_ini.f:1545-1575, 6836-7244, 9053-9145)."""
+    import os
+    lib = L.load()
+    b = wl.basis
+    os.makedirs(path, exist_ok=True)
+    ps = L.as_i32(parity_state) if parity_state is not None else (
+        L.as_i32(b.parity_state) if getattr(b, "identical", False) else None)
+    L.check(lib.mqct_hip_write_mij_info(path.encode(), int(unformatted), b.coll_type, b.n_channels, b.n_states,
+                                        b.total_size, len(wl.R), L.iptr(L.as_i32(b.indx_chann)),
+                                        L.iptr(L.as_i32(b.j12)), L.iptr(L.as_i32(b.m12)), L.iptr(b.chan_qn()),
+                                        L.iptr(ps), L.iptr(L.as_i32(b.ind_mat)), L.dptr(L.as_f64(wl.R))),
+            "write_mij_info")
+    ir1, ir2 = wl.ir_cut
+    M = L.as_f64(mat_el)
+    L.check(lib.mqct_hip_write_mij_dir(path.encode(), int(unformatted), n_files, b.total_size, len(wl.R),
+                                       L.iptr(L.as_i32(b.ind_mat)), L.dptr(M), wl.mij_zero_cut, ir1, ir2),
+            "write_mij_dir")
+
+
+def read_mij_dir(path, n_r, unformatted=False):
+    """bk_read_matrix for one reader: (ind_mat[n_nonzero][2], mat_el
+    [n_nonzero][n_r], the file's cut-off in cm^-1), the elements in file
+    order; mij_spline then applies MIJ_SHIFT and the spline."""
+    lib = L.load()
+    nnz = np.zeros(1, np.int64)
+    cut = np.zeros(1)
+    L.check(lib.mqct_hip_read_mij_dir(path.encode(), int(unformatted), n_r, 0, nnz.ctypes.data_as(L.c_i64_p),
+                                      L.dptr(cut), None, None), "read_mij_dir sizes")
+    n = int(nnz[0])
+    ind = np.zeros((n, 2), np.int32)
+    M = np.zeros((n, n_r))
+    L.check(lib.mqct_hip_read_mij_dir(path.encode(), int(unformatted), n_r, n, nnz.ctypes.data_as(L.c_i64_p),
+                                      L.dptr(cut), L.iptr(ind), L.dptr(M)), "read_mij_dir")
+    return ind, M, float(cut[0])
+
+
 def mij_spline(dev, R, mat_el):
     """READ_MIJ_USER's post-read step (matrix_ini.f:6603-6621) on the GPU:
     MIJ_SHIFT per the handle's setting, then the clamped spline.  Returns
