@@ -115,6 +115,13 @@ def main():
                     help="steps of the bounded plain-kernel run that gives its right-hand-side rate")
     ap.add_argument("--c3-dt", type=float, default=0.5,
                     help="TIME_STEP (a.u.) of the two-tops rk4 propagation")
+    ap.add_argument("--c3-dt2", type=float, default=0.0,
+                    help="also propagate the FP64 two-tops build at this TIME_STEP and report the cross-section "
+                         "change (a step-size convergence check; 0 skips it)")
+    ap.add_argument("--c3-chunk", type=int, default=1024,
+                    help="two-tops trajectories per propagate call (each call is logged when it ends)")
+    ap.add_argument("--c3-only", action="store_true",
+                    help="run only the two-asymmetric-tops section (no headline line)")
     ap.add_argument("--sur-samples", type=int, default=4_000_000)
     ap.add_argument("--sur-steps", type=int, default=30000)
     ap.add_argument("--sur-batch", type=int, default=8192)
@@ -137,6 +144,11 @@ def main():
     from mqct_amd.propagate import Propagator, at_first_step
 
     dev = Device(0)
+    if args.c3_only:
+        c3 = two_tops_section(dev, args)
+        dev.close()
+        print(json.dumps({"two_asym_tops": c3}), flush=True)
+        return 0
     n = args.n_traj
     bb, E = W.at_batch(n)
     form = {"fp64": 1, "fp32": 2, "fp32p": 3}[args.knots]
@@ -344,12 +356,6 @@ def main():
             if prec not in build["asym_top_atom"]:
                 mb.build(prec, copy_out=False)
                 build["asym_top_atom"][prec] = mb.timing()
-        w3 = W.two_asym_tops()
-        m3 = MatrixBuilder(dev, w3)
-        build["two_asym_tops"] = {}
-        for prec in ("fp64", "fp32", "bf16x3"):
-            m3.build(prec, copy_out=False)
-            build["two_asym_tops"][prec] = m3.timing()
         log("matrix build total ms: " + ", ".join(
             "%s %s %.1f" % (w, p, build[w][p]["total_ms"]) for w in build for p in build[w]))
     build = {w: {p: {k: round(float(v), 3) for k, v in t.items()} for p, t in d.items()} for w, d in build.items()}
@@ -359,6 +365,9 @@ def main():
     c3 = None
     if not args.no_c3:
         c3 = two_tops_section(dev, args)
+        build["two_asym_tops"] = {p: c3[p]["build_ms"] for p in ("fp64", "fp32", "bf16x3")}
+        if "expansion" in c3:
+            build["two_asym_tops"]["expansion"] = c3["expansion"]["build_ms"]
 
     # -- CPU baseline: the oracle on one core, one trajectory of the batch ----
     cpu = None
@@ -439,6 +448,8 @@ def two_tops_section(dev, args):
     from mqct_amd.propagate import Propagator, at_first_step
     from mqct_amd.surrogate import train_surrogate
     n = args.c3_traj
+    # long propagations report the trajectories still running every minute
+    L.check(dev.lib.mqct_hip_set_progress(dev.h, 60.0), "set_progress")
     wl = W.two_asym_tops_c3()
     bb, E = W.two_tops_at_batch(1024)
     sel = np.linspace(0, 1023, n).astype(int) if n < 1024 else np.arange(1024)
@@ -458,13 +469,24 @@ def two_tops_section(dev, args):
             rows[-1]["dt"] = args.c3_dt
         return rows
 
-    def propagate(batched=1):
+    def propagate(batched=1, dt=None, tag=""):
         prop = Propagator(dev, wl.basis, wl.R, mass_amu=wl.mass_amu, from_build=True)
         prop.load_paths(paths)
         rows = rows_for(prop)
-        t0 = time.perf_counter()
-        q, st = prop.run(L.MQCT_PROP_AT_RK4, rows, batched=batched)
-        el = time.perf_counter() - t0
+        if dt is not None:
+            for r in rows:
+                r["dt"] = dt
+        qs, sts, el = [], [], 0.0
+        for c0 in range(0, n, args.c3_chunk):
+            t0 = time.perf_counter()
+            q, st = prop.run(L.MQCT_PROP_AT_RK4, rows[c0:c0 + args.c3_chunk], batched=batched)
+            el += time.perf_counter() - t0
+            qs.append(q)
+            sts.append(st)
+            if n > args.c3_chunk:
+                log("two tops %s: trajectories %d-%d done, %.1f s so far" % (tag, c0, c0 + len(q) - 1, el))
+        q = np.concatenate(qs)
+        st = {k: np.concatenate([s[k] for s in sts]) for k in sts[0]}
         return prop, q, st, n / el
 
     mb = MatrixBuilder(dev, wl)
@@ -483,12 +505,19 @@ def two_tops_section(dev, args):
             log("two tops fp32: build %.1f ms (contract %.1f), elem err max %.2e" % (
                 tm["total_ms"], tm["contract_ms"], rec["elem_err"]["max"]))
             continue
-        prop, q, st, tps = propagate(1)
+        prop, q, st, tps = propagate(1, tag=prec)
         sig = xsec_functional(prop, q, l, E)
         rec.update({"batched_traj_per_s": round(tps, 3), "norm_err_max": float(st["norm_err"].max()),
                     "rhs_mean": float(st["n_rhs"].mean())})
         if prec == "fp64":
             M64, sig64 = M, sig
+            if args.c3_dt2 > 0:
+                p2, q2, s2, tps2 = propagate(1, dt=args.c3_dt2, tag="fp64 dt %g" % args.c3_dt2)
+                e2, nch2 = rel_err(xsec_functional(p2, q2, l, E), sig)
+                rec["dt_study"] = {"dt": args.c3_dt2, "traj_per_s": round(tps2, 3), "xsec_rel_change": e2,
+                                   "channels": nch2, "norm_err_max": float(s2["norm_err"].max())}
+                log("two tops fp64 at dt %g: %.3f traj/s, cross sections differ by %.2e from dt %g" % (
+                    args.c3_dt2, tps2, e2, args.c3_dt))
             # the plain kernel on a bounded run: right-hand sides per second
             pr = Propagator(dev, wl.basis, wl.R, mass_amu=wl.mass_amu, from_build=True)
             pr.load_paths(paths)
@@ -530,7 +559,7 @@ def two_tops_section(dev, args):
     fill_s = time.perf_counter() - t0
     M, D, S = mb2.build("bf16x3")
     tm = {k: round(float(v), 3) for k, v in mb2.timing().items()}
-    prop, q, st, tps = propagate(1)
+    prop, q, st, tps = propagate(1, tag="surrogate")
     sig = xsec_functional(prop, q, l, E)
     e, nch = rel_err(sig, sig64)
     stats.update({"grid_fill_s": round(fill_s, 4), "plain_pes_grid_fill_gpu_s": round(plain_fill_s, 4),

@@ -29,7 +29,9 @@
 #include "dyn_internal.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <type_traits>
 #include <vector>
@@ -1768,7 +1770,19 @@ int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *t
     }
     ~GraphGuard() { reset(); }
   } gg;
+  auto t_prog = std::chrono::steady_clock::now();
+  const auto t_start = t_prog;
   for (int rep = 0; rep < max_rounds && active > 0; rep++) {
+    if (h->progress_s > 0.0) {
+      const auto now = std::chrono::steady_clock::now();
+      if (std::chrono::duration<double>(now - t_prog).count() >= h->progress_s) {
+        std::fprintf(stderr, "[mqct] batched propagation: %d of %d trajectories running after %lld iterations, %.0f s\n",
+                     active, n_traj, (long long)it_done,
+                     std::chrono::duration<double>(now - t_start).count());
+        std::fflush(stderr);
+        t_prog = now;
+      }
+    }
     if (int rc = shape_for(shape)) return rc;
     if (active == 0) break;
     if (prof) {

@@ -638,6 +638,12 @@ extern "C" int mqct_hip_build_timing(mqct_hip_t *h, double *ms5) {
   return MQCT_OK;
 }
 
+extern "C" int mqct_hip_set_progress(mqct_hip_t *h, double seconds) {
+  if (!h || !(seconds >= 0.0)) return mqct_fail(MQCT_ERR_ARG, "set_progress: bad argument");
+  h->progress_s = seconds;
+  return MQCT_OK;
+}
+
 extern "C" int mqct_hip_set_mij_shift(mqct_hip_t *h, int on) {
   if (!h) return mqct_fail(MQCT_ERR_ARG, "set_mij_shift: null handle");
   h->mij_shift = on != 0;

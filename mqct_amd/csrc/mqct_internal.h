@@ -99,6 +99,8 @@ struct mqct_hip {
   // MIJ_SHIFT= keyword (default YES, iotest.f:3754): the propagation sees
   // M(R) - M(R_max) (matrix_ini.f:2226-2230, 6603-6608)
   bool mij_shift = true;
+  // progress lines of long batched propagations on stderr (0: none)
+  double progress_s = 0.0;
 
   // ---- identical partners (type 0): IDENTICAL= / IDENTICAL_PES= ----
   bool identical = false, identical_pes = false;
