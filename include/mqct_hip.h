@@ -240,8 +240,9 @@ typedef struct {
   int numb_oscl_prds;
   int numb_orb_prds;
   int max_steps;       /* safety cap on outer steps (<=0: none); the plain
-                        * form ends the trajectory there with fail = 2 (the
-                        * batched AT form runs every path to its end) */
+                        * form and the batched CC form end the trajectory
+                        * there with fail = 2 (the batched AT form runs
+                        * every path to its end) */
 } mqct_traj_t;
 
 typedef struct {
@@ -263,8 +264,11 @@ int mqct_hip_load_paths(mqct_hip_t *h, int n_paths, const int64_t *path_off,
  * (AT modes fill the 2N amplitudes and leave the classical slots 0);
  * stat_out (optional): per-trajectory statistics.
  * batched=0: the one-trajectory-per-workgroup plain FP64 form;
- * batched=1: the interval-bucketed batched form (AT modes) with FP64
- * spline coefficients; batched=2: the same with the coefficients stored
+ * batched=1: the interval-bucketed batched form with FP64 spline
+ * coefficients -- AT modes, and CC modes, whose GEMM also forms dM/dR v
+ * for the mean-field force and Coriolis sums (DERIVS_BK, mqct_eqtns.f:
+ * 112-499) and whose step kernel runs DOTRAJECT's dt intervals and end
+ * checks (propagator.f:3873-4050); batched=2: the same with the coefficients stored
  * in FP32 (amplitudes, products and time stepping stay FP64); batched=3:
  * FP32 coefficients and FP32 products -- the pipelined interval-GEMM forms
  * multiply them by the weighted amplitudes rounded to FP32 on f32 MFMA
@@ -317,10 +321,11 @@ int mqct_hip_batched_tile(mqct_hip_t *h, int shape, int *form, int *entries,
                           int *k_rows, int *row_tiles);
 /* The chosen shapes (indices into the candidate list, -1 none yet) of
  * levels 0..n-1 for the FP64 (fp32=0) or FP32 (fp32=1) knot matrices, or
- * the FP32 knot matrices with FP32 products (fp32=2). */
+ * the FP32 knot matrices with FP32 products (fp32=2), of the AT modes;
+ * fp32 = 3, 4, 5: the same three forms of the CC modes. */
 int mqct_hip_batched_tile_levels(mqct_hip_t *h, int fp32, int n, int *shapes);
 /* Sets the shapes of levels 0..n-1 (n <= 24; -1: chosen by timing) for the
- * product form fp32 (0, 1, 2 as in batched_tile_levels), e.g. to repeat a
+ * product form fp32 (0..5 as in batched_tile_levels), e.g. to repeat a
  * measured run's choices in a profiled one.  A batched run refuses a set
  * shape that does not fit its matrix (work-queue forms: at most 16383
  * knots). */

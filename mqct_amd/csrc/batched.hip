@@ -26,6 +26,13 @@
 // running trajectories between replays.  The knot matrices are kept in
 // FP64 or, as the FP32 coefficient form, in FP32 (converted to FP64 in the
 // MFMA operand).
+//
+// The CC modes (DERIVS_BK, mqct_eqtns.f:112-499) use the same GEMM with
+// two more entries per knot whose weights are the spline's R-derivatives,
+// so each trajectory gets M v and dM/dR v; their own step kernel forms the
+// mean-field force and the Coriolis sums from them, integrates the
+// classical variables beside the amplitudes and runs DOTRAJECT's dt
+// intervals with their end checks (propagator.f:3873-4050).
 #include "dyn_internal.h"
 
 #include <algorithm>
@@ -57,16 +64,33 @@ struct Ctl {
   int last, rec;                  // last: rk4 step that ends the path; rec: path-record hint
 };
 
-// A bucket entry: trajectory, knot side (0: the knot below R, 1: above)
-// and its two spline weights for that knot, so the GEMM reads one record
-// per entry.
+// A bucket entry: trajectory, product slot (0: the knot below R, 1: above;
+// CC also 2, 3: the R-derivative weights at those knots) and its two
+// spline weights for that knot, so the GEMM reads one record per entry.
 struct Entry {
   int t, side;
   double wy, wy2;
 };
 
+// CC-MQCT (DERIVS_BK, mqct_eqtns.f:205-500; DOTRAJECT's dt intervals,
+// propagator.f:3873-4050): per trajectory, the classical part of the stage
+// being evaluated and the interval loop's state.  The classical parts of y
+// and of the k slots follow the 2N amplitudes in each slot.
+struct CtlCC {
+  double xc[8];           // stage's R, P_R, Theta, P_Theta, Phi, P_Phi, 0, 0
+  double tcur;            // start of the current dt interval
+  double dt_step;         // odeint: first trial step of the next interval
+  double tmp_R1, tmp_R2;  // R at the ends of the last two intervals
+  int vib, counter, nsteps, fail;
+};
+
 struct BDev {
   int N, nblk, n_r, n_traj, mode;
+  int sw;                     // product vectors per trajectory: 2 (AT), 4 (CC: also dM/dR v)
+  int bcap;                   // bucket entries per knot: n_traj (AT), 2 n_traj (CC)
+  int theta_on;               // CC: the Theta equations are integrated
+  const mqct_traj_t *traj;    // CC: the trajectories' end conditions
+  CtlCC *cc;                  // CC: classical state per trajectory
   const double *R;            // knots; R[-1] holds the reduced mass
   const void *KM;             // [n_r][2][S2] dense block matrices Y, Y2 (double or float)
   long long S2;
@@ -89,15 +113,16 @@ struct BDev {
   const int *rg_row0;         // [nrg] first row of the group in its block
   const double *E;            // per position, Hartree
   const int2 *nb;             // per position: positions of state-1 / state+1, -1 none
+  const int *m_pos;           // per position: m12 (CC Coriolis with Theta)
   const double2 *sq;          // per position: their Coriolis factors
   const double *prec, *pder;  // paths
   const long long *poff;
-  double *S;                  // [n_traj][NSLOT][2N]
+  double *S;                  // [n_traj][NSLOT][2N] (CC: [n_traj][NSLOT][2N + 8])
   double2 *V;                 // [n_traj][N]
-  double2 *W;                 // [n_traj][2][N] (float2 [n_traj][2][N] after FP32 products)
+  double2 *W;                 // [n_traj][sw][N] (float2 [n_traj][sw][N] after FP32 products)
   Ctl *ctl;
   int *cnt;                   // [2][n_r]
-  Entry *bucket;              // [2][n_r][n_traj]
+  Entry *bucket;              // [2][n_r][bcap]
   int *active;                // running trajectories
   unsigned long long *nbucketed;  // right-hand sides with R inside the grid
   unsigned long long *ntouched;   // (GEMM launch, knot) pairs with a non-empty bucket
@@ -212,11 +237,11 @@ __device__ void prepare_stage(const BDev &d, int t, Ctl &c, const double *xr, co
     if (!c.zero) {
       if (d.nbucketed) atomicAdd(d.nbucketed, 1ULL);  // counted only in profiled runs
       int *cnt = d.cnt + par * nr;
-      Entry *bk = d.bucket + (long long)par * nr * d.n_traj;
+      Entry *bk = d.bucket + (long long)par * nr * d.bcap;
       const int p0 = atomicAdd(&cnt[klo], 1);
-      bk[(long long)klo * d.n_traj + p0] = Entry{t, 0, aa, c.c1};
+      bk[(long long)klo * d.bcap + p0] = Entry{t, 0, aa, c.c1};
       const int p1 = atomicAdd(&cnt[khi], 1);
-      bk[(long long)khi * d.n_traj + p1] = Entry{t, 1, bb, c.c2};
+      bk[(long long)khi * d.bcap + p1] = Entry{t, 1, bb, c.c2};
     }
   }
 }
@@ -261,7 +286,7 @@ __device__ void lds_super_tile(const BDev &d, int par, int r, int b, int row0, i
     tile_on[q] = rt0 < nb;
     rr[q] = rt0 + (lane & 15);
   }
-  const Entry *bk = d.bucket + ((long long)par * d.n_r + r) * d.n_traj;
+  const Entry *bk = d.bucket + ((long long)par * d.n_r + r) * d.bcap;
   const int nct = (2 * nE + 15) / 16;
   const int nEpad = nct * 8;  // entries staged (zero columns up to the tile edge)
   __syncthreads();
@@ -335,7 +360,7 @@ __device__ void lds_super_tile(const BDev &d, int par, int r, int b, int row0, i
       if (ct >= nct) break;
       const int col = 16 * ct + (lane & 15), e = col >> 1;
       if (e >= nE) continue;
-      double *wb = reinterpret_cast<double *>(d.W + ((long long)sh.t[e] * 2 + sh.side[e]) * d.N + p0) + (col & 1);
+      double *wb = reinterpret_cast<double *>(d.W + ((long long)sh.t[e] * d.sw + sh.side[e]) * d.N + p0) + (col & 1);
 #pragma unroll
       for (int i = 0; i < 4; i++) {
         const int orow = rt0 + (lane >> 4) + 4 * i;
@@ -383,7 +408,7 @@ __global__ __launch_bounds__(BS) void gemm_direct_kernel(BDev d, int par) {
   const int rr = rt0 + (lane & 15);
   const KT *Y = static_cast<const KT *>(d.KM) + ((long long)r * 2) * d.S2 + d.blk_off[b];
   const KT *Y2 = static_cast<const KT *>(d.KM) + ((long long)r * 2 + 1) * d.S2 + d.blk_off[b];
-  const Entry *bk = d.bucket + ((long long)par * d.n_r + r) * d.n_traj;
+  const Entry *bk = d.bucket + ((long long)par * d.n_r + r) * d.bcap;
   for (int e0 = 0; e0 < ne; e0 += 8 * NCT) {
     const double *vp[NCT];
     double wy[NCT], wy2[NCT];
@@ -420,7 +445,7 @@ __global__ __launch_bounds__(BS) void gemm_direct_kernel(BDev d, int par) {
 #pragma unroll
     for (int ct = 0; ct < NCT; ct++) {
       if (!ok[ct]) continue;
-      double *wb = reinterpret_cast<double *>(d.W + ((long long)tt[ct] * 2 + sd[ct]) * d.N + p0) + comp;
+      double *wb = reinterpret_cast<double *>(d.W + ((long long)tt[ct] * d.sw + sd[ct]) * d.N + p0) + comp;
 #pragma unroll
       for (int i = 0; i < 4; i++) {
         const int orow = rt0 + (lane >> 4) + 4 * i;
@@ -478,7 +503,7 @@ __global__ __launch_bounds__(BS) void gemm_stream_kernel(BDev d, int par) {
   const int q = lane >> 4, col = lane & 15, comp = col & 1;
   const FV *A = reinterpret_cast<const FV *>(static_cast<const KT *>(d.KT) + (long long)r * d.KS + d.tile_off[g]) +
                 lane;
-  const Entry *bk = d.bucket + ((long long)par * d.n_r + r) * d.n_traj;
+  const Entry *bk = d.bucket + ((long long)par * d.n_r + r) * d.bcap;
   for (int e0 = 0; e0 < ne; e0 += 8 * NCT) {
     const double *vp[NCT];
     double wy[NCT], wy2[NCT];
@@ -520,7 +545,7 @@ __global__ __launch_bounds__(BS) void gemm_stream_kernel(BDev d, int par) {
 #pragma unroll
     for (int ct = 0; ct < NCT; ct++) {
       if (!ok[ct]) continue;
-      double *wb = reinterpret_cast<double *>(d.W + ((long long)tt[ct] * 2 + sd[ct]) * d.N + p0) + comp;
+      double *wb = reinterpret_cast<double *>(d.W + ((long long)tt[ct] * d.sw + sd[ct]) * d.N + p0) + comp;
 #pragma unroll
       for (int i = 0; i < 4; i++) {
         const int orow = rt0 + (lane >> 4) + 4 * i;
@@ -618,7 +643,7 @@ __global__ __launch_bounds__(BS) void gemm_queue_kernel(BDev d, int par) {
     const int ns = (nb + KB - 1) / KB;
     const FV *A =
         reinterpret_cast<const FV *>(static_cast<const KT *>(d.KT) + (long long)r * d.KS + d.tile_off[g]) + lane;
-    const Entry *bk = d.bucket + ((long long)par * nr + r) * d.n_traj;
+    const Entry *bk = d.bucket + ((long long)par * nr + r) * d.bcap;
     const double *vp[NCT];
     double wy[NCT], wy2[NCT];
     int tt[NCT], sd[NCT];
@@ -659,7 +684,7 @@ __global__ __launch_bounds__(BS) void gemm_queue_kernel(BDev d, int par) {
 #pragma unroll
     for (int ct = 0; ct < NCT; ct++) {
       if (!ok[ct]) continue;
-      double *wb = reinterpret_cast<double *>(d.W + ((long long)tt[ct] * 2 + sd[ct]) * d.N + p0) + comp;
+      double *wb = reinterpret_cast<double *>(d.W + ((long long)tt[ct] * d.sw + sd[ct]) * d.N + p0) + comp;
 #pragma unroll
       for (int i = 0; i < 4; i++) {
         const int orow = rt0 + (lane >> 4) + 4 * i;
@@ -748,7 +773,7 @@ __device__ void pipe_item(const BDev &d, int par, int r, int b, int tg, int c,
   // fragments and amplitudes are zero there
   const int nbp = (nb + KB - 1) / KB * KB, nfr = 2 * nbp / KB, nsl = (2 * nbp + GK - 1) / GK;
   const int e0 = c * GE, nE = min(GE, cnt[r] - e0), nct = (2 * nE + 15) / 16;
-  const Entry *bk = d.bucket + ((long long)par * nr + r) * d.n_traj;
+  const Entry *bk = d.bucket + ((long long)par * nr + r) * d.bcap;
   __syncthreads();  // the previous item is done with the entry table and Bs
   if (tid < GE) {
     int t = 0, side = 0;
@@ -862,7 +887,7 @@ __device__ void pipe_item(const BDev &d, int par, int r, int b, int tg, int c,
       if (ct >= nct) break;
       const int cc = 16 * ct + col, e = cc >> 1;
       if (e >= nE) continue;
-      const long long wo = ((long long)sh.t[e] * 2 + sh.side[e]) * d.N + p0;
+      const long long wo = ((long long)sh.t[e] * d.sw + sh.side[e]) * d.N + p0;
       // FP32 products are stored as they were accumulated, in FP32
       if constexpr (M32) {
         float *wb = reinterpret_cast<float *>(reinterpret_cast<float2 *>(d.W) + wo) + (cc & 1);
@@ -1271,6 +1296,430 @@ __global__ __launch_bounds__(TB) void step_kernel(BDev d, int par, int w32) {
   if (tid == 0) d.ctl[t] = c;
 }
 
+// ---------------------------------------------------------------------------
+// CC-MQCT in the batched form.  The GEMM gives per trajectory M(R) v from
+// the two knots of R (product slots 0, 1) and dM/dR v (slots 2, 3: the same
+// knot matrices with the R-derivatives of the spline weights, as splint_bk's
+// derivative branch, mqct_eqtns.f:575-607).  With W = M v and W' = dM/dR v
+// the pair sums of DERIVS_BK (mqct_eqtns.f:263-359) are sums over states:
+// M is real symmetric and couples equal m only, so a pair sum over i >= j
+// with d_f = 2 off the diagonal is the full double sum, and
+//   mean-field force  -sum d_f dM_ij Re(v_i* v_j)          = -sum_j Re(v_j* W'_j)
+//   dP_Phi/dt  sin(Th) sum d_f/2 M_ij (Im(v_j+-* v_i) terms) = -sin(Th) sum_j Im(W_j* (P v)_j)
+//   P_Theta part      sum d_f/2 M_ij (Re terms)             =  sum_j Re(W_j* (Q v)_j)
+// with (P v)_j = c+_j v_{j+1} + c-_j v_{j-1}, (Q v)_j = c+_j v_{j+1} - c-_j v_{j-1}
+// and c+-_j = sqrt(j(j+1) - m(m +- 1)), zero at m = +-j.
+
+// The next stage of a CC trajectory: its rotated amplitudes v = exp(-i E
+// tau) x into V; thread 0 finds the knots of the stage's R (cc.xc[0]) and
+// appends four entries: the spline weights at the knot below and above, and
+// their R-derivatives -1/h, -(3A^2-1)h/6 and 1/h, (3B^2-1)h/6.
+template <int PT, int TB = BS>
+__device__ void prepare_stage_cc(const BDev &d, int t, Ctl &c, const CtlCC &cc, const double *xr,
+                                 const double *xi, int par, const double *Ej) {
+  const int tid = threadIdx.x, N = d.N;
+  const double tau = stage_time(c);
+  double2 *v = d.V + (long long)t * N;
+#pragma unroll
+  for (int j = 0; j < PT; j++) {
+    const int i = tid + j * TB;
+    if (i < N) {
+      double sn, cs;
+      sincos(tau * Ej[j], &sn, &cs);
+      v[i] = make_double2(xr[j] * cs + xi[j] * sn, xi[j] * cs - xr[j] * sn);
+    }
+  }
+  if (tid == 0) {
+    const int nr = d.n_r;
+    const double R = cc.xc[0];
+    c.tau = tau;
+    c.zero = R > d.R[nr - 1];
+    const int klo = interval(d.R, 1, nr, R, c.klo), khi = klo + 1;
+    const double hk = d.R[khi] - d.R[klo];
+    const double aa = (d.R[khi] - R) / hk, bb = (R - d.R[klo]) / hk;
+    const double cc5 = (hk * hk) / 6.0, cc6 = cc5 / hk;
+    c.klo = klo;
+    c.aa = aa;
+    c.bb = bb;
+    c.c1 = (aa * aa * aa - aa) * cc5;
+    c.c2 = (bb * bb * bb - bb) * cc5;
+    if (!c.zero) {
+      if (d.nbucketed) atomicAdd(d.nbucketed, 2ULL);  // twice an AT right-hand side's products
+      int *cnt = d.cnt + par * nr;
+      Entry *bk = d.bucket + (long long)par * nr * d.bcap;
+      const int p0 = atomicAdd(&cnt[klo], 2);
+      bk[(long long)klo * d.bcap + p0] = Entry{t, 0, aa, c.c1};
+      bk[(long long)klo * d.bcap + p0 + 1] = Entry{t, 2, -1.0 / hk, -(3.0 * aa * aa - 1.0) * cc6};
+      const int p1 = atomicAdd(&cnt[khi], 2);
+      bk[(long long)khi * d.bcap + p1] = Entry{t, 1, bb, c.c2};
+      bk[(long long)khi * d.bcap + p1 + 1] = Entry{t, 3, 1.0 / hk, (3.0 * bb * bb - 1.0) * cc6};
+    }
+  }
+}
+
+// The CC step: completes the stage the GEMM evaluated (amplitude and
+// classical derivatives, DERIVS_BK), advances rk4 or odeint/rkqs/rkck over
+// all 2N + 8 variables (the classical eight kept by thread 0), ends the dt
+// interval of DOTRAJECT with its checks (propagator.f:3995-4050: turning
+// points of R, orbiting periods of Phi, time limit, R past R_start, step
+// cap), and prepares the next stage.
+template <int PT, int TB = BS>
+__global__ __launch_bounds__(TB) void cc_step_kernel(BDev d, int par, int w32) {
+  extern __shared__ double sys[];  // 2N: the completed stage's vector (Coriolis neighbours)
+  __shared__ double red[8];
+  __shared__ Ctl s_ctl;            // the control after an interval's end, from thread 0
+  const int t = blockIdx.x, tid = threadIdx.x, N = d.N, n2 = 2 * N, ns = n2 + 8;
+  if (t == 0 && tid == 0) *d.qctr = 0;  // item counter of the next GEMM
+  Ctl c = d.ctl[t];
+  if (c.status != 0) return;
+  CtlCC cc = d.cc[t];
+  double *S = d.S + (long long)t * NSLOT * ns;
+  double *y = S;
+  const int s = c.stage;
+  const double mu = d.R[-1];
+  const double *xc = cc.xc;  // the completed stage's classical part
+  const double sin_t = sin(xc[2]), cos_t = cos(xc[2]);
+  const double dPhidT = xc[5] / xc[0] / xc[0] / mu / (sin_t * sin_t);
+  const double dThetadT = d.theta_on ? xc[3] / xc[0] / xc[0] / mu : 0.0;
+  double kr[PT], ki[PT], Ej[PT], yr[PT], yi[PT];
+  double fsum = 0.0, psum = 0.0, tsum = 0.0;  // -Re(v* W'), Im(W* P v), Re(W* Q v) partial sums
+  {
+    const double2 *vv = d.V + (long long)t * N;
+    const long long w0 = (long long)t * 4 * N;
+    double2 vj[PT], wj[PT], dj[PT], fj[PT], uj[PT], lj[PT];
+    int2 nj[PT];
+#pragma unroll
+    for (int j = 0; j < PT; j++) {
+      const int i = tid + j * TB;
+      Ej[j] = yr[j] = yi[j] = 0.0;
+      vj[j] = wj[j] = dj[j] = fj[j] = uj[j] = lj[j] = make_double2(0.0, 0.0);
+      nj[j] = make_int2(-1, -1);
+      if (i < N) {
+        Ej[j] = d.E[i];
+        vj[j] = vv[i];
+        nj[j] = d.nb[i];
+        fj[j] = d.sq[i];
+        if (!c.zero) {
+          double2 p[4];
+#pragma unroll
+          for (int q = 0; q < 4; q++) {
+            if (w32) {
+              const float2 a = reinterpret_cast<const float2 *>(d.W)[w0 + q * N + i];
+              p[q] = make_double2(a.x, a.y);
+            } else {
+              p[q] = d.W[w0 + q * N + i];
+            }
+          }
+          wj[j] = make_double2(p[0].x + p[1].x, p[0].y + p[1].y);
+          dj[j] = make_double2(p[2].x + p[3].x, p[2].y + p[3].y);
+          if (nj[j].x >= 0) lj[j] = vv[nj[j].x];
+          if (nj[j].y >= 0) uj[j] = vv[nj[j].y];
+        }
+        yr[j] = y[i];
+        yi[j] = y[i + N];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < PT; j++) {
+      const int i = tid + j * TB;
+      kr[j] = ki[j] = 0.0;
+      if (i < N) {
+        double sn, cs;
+        sincos(c.tau * Ej[j], &sn, &cs);
+        const double2 vi = vj[j];
+        sys[i] = vi.x * cs - vi.y * sn;
+        sys[i + N] = vi.y * cs + vi.x * sn;
+        if (!c.zero) {
+          const double2 w = wj[j], f = fj[j];
+          kr[j] = sn * w.x + cs * w.y;
+          ki[j] = -(cs * w.x - sn * w.y);
+          fsum -= vi.x * dj[j].x + vi.y * dj[j].y;
+          // f = (c-, c+): the factors of state-1 and state+1
+          const double px = f.y * uj[j].x + f.x * lj[j].x, py = f.y * uj[j].y + f.x * lj[j].y;
+          const double qx = f.y * uj[j].x - f.x * lj[j].x, qy = f.y * uj[j].y - f.x * lj[j].y;
+          psum += w.x * py - w.y * px;
+          tsum += w.x * qx + w.y * qy;
+        }
+      }
+    }
+    __syncthreads();
+    // state-diagonal Coriolis coupling of the stage vector (mqct_eqtns.f:389-466)
+#pragma unroll
+    for (int j = 0; j < PT; j++) {
+      const int i = tid + j * TB;
+      if (i < N) {
+        const int2 nbr = nj[j];
+        const double2 f = fj[j];
+        if (d.theta_on) {
+          const double m = d.m_pos[i];
+          kr[j] += dPhidT * m * cos_t * sys[i];
+          ki[j] -= dPhidT * m * cos_t * sys[i + N];
+          if (nbr.x >= 0) {
+            kr[j] += (dPhidT * sin_t * sys[nbr.x + N] - dThetadT * sys[nbr.x]) * f.x / 2.0;
+            ki[j] -= (dPhidT * sin_t * sys[nbr.x] - dThetadT * sys[nbr.x + N]) * f.x / 2.0;
+          }
+          if (nbr.y >= 0) {
+            kr[j] += (dPhidT * sin_t * sys[nbr.y + N] - dThetadT * sys[nbr.y]) * f.y / 2.0;
+            ki[j] -= (dPhidT * sin_t * sys[nbr.y] - dThetadT * sys[nbr.y + N]) * f.y / 2.0;
+          }
+        } else {
+          const double hd = dPhidT / 2.0;
+          if (nbr.x >= 0) {
+            kr[j] += hd * f.x * sys[nbr.x + N];
+            ki[j] -= hd * f.x * sys[nbr.x];
+          }
+          if (nbr.y >= 0) {
+            kr[j] += hd * f.y * sys[nbr.y + N];
+            ki[j] -= hd * f.y * sys[nbr.y];
+          }
+        }
+      }
+    }
+  }
+  if (!c.zero) {
+    fsum = block_sum<TB>(fsum, red);
+    psum = block_sum<TB>(psum, red);
+    if (d.theta_on) tsum = block_sum<TB>(tsum, red);
+  }
+  // classical derivatives (mqct_eqtns.f:473-495), thread 0
+  double kc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  if (tid == 0) {
+    kc[0] = xc[1] / mu;
+    kc[4] = xc[5] / xc[0] / xc[0] / mu / (sin_t * sin_t);
+    kc[1] = fsum + xc[3] / xc[0] / xc[0] / mu * xc[3] / xc[0] +
+            xc[5] / xc[0] / xc[0] / mu * xc[5] / xc[0] / (sin_t * sin_t);
+    kc[5] = -sin_t * psum;
+    if (d.theta_on) {
+      kc[2] = xc[3] / xc[0] / xc[0] / mu;
+      kc[3] = tsum + xc[5] / xc[0] / xc[0] / mu * xc[5] / (sin_t * sin_t * sin_t) * cos_t;
+    }
+  }
+  c.nrhs++;
+  const double TINY = 1.e-30;
+  double xr[PT], xi[PT];            // the next stage's vector
+  double ycl[8], xcn[8];            // thread 0: classical y and the next stage's classical part
+  if (tid == 0)
+    for (int q = 0; q < 8; q++) ycl[q] = y[n2 + q];
+  auto store_k = [&]() {
+    double *k = S + (1 + s) * ns;
+#pragma unroll
+    for (int j = 0; j < PT; j++) {
+      const int i = tid + j * TB;
+      if (i < N) {
+        k[i] = kr[j];
+        k[i + N] = ki[j];
+      }
+    }
+    if (tid == 0)
+      for (int q = 0; q < 8; q++) k[n2 + q] = kc[q];
+  };
+  auto stage_from_k = [&](double f) {
+#pragma unroll
+    for (int j = 0; j < PT; j++) {
+      const int i = tid + j * TB;
+      if (i < N) {
+        xr[j] = yr[j] + f * kr[j];
+        xi[j] = yi[j] + f * ki[j];
+      }
+    }
+    if (tid == 0)
+      for (int q = 0; q < 8; q++) xcn[q] = ycl[q] + f * kc[q];
+  };
+  bool iv_end = false;  // the dt interval ended with this stage (uniform over the block)
+  if (c.rk4) {
+    if (s < 3) {
+      store_k();
+      c.stage = s + 1;
+      stage_from_k(s < 2 ? 0.5 * c.h : c.h);
+    } else {
+      const double h6 = c.h / 6.0;
+      const double *k1 = S + ns, *k2 = S + 2 * ns, *k3 = S + 3 * ns;
+#pragma unroll
+      for (int j = 0; j < PT; j++) {
+        const int i = tid + j * TB;
+        if (i < N) {
+          xr[j] = yr[j] + h6 * (k1[i] + kr[j] + 2.0 * (k2[i] + k3[i]));
+          xi[j] = yi[j] + h6 * (k1[i + N] + ki[j] + 2.0 * (k2[i + N] + k3[i + N]));
+          y[i] = xr[j];
+          y[i + N] = xi[j];
+        }
+      }
+      if (tid == 0)
+        for (int q = 0; q < 8; q++) {
+          const int ii = n2 + q;
+          xcn[q] = ycl[q] + h6 * (k1[ii] + kc[q] + 2.0 * (k2[ii] + k3[ii]));
+          y[ii] = xcn[q];
+        }
+      c.nstp++;
+      c.stage = 0;
+      // odeint's MAXSTP-th step ends the trajectory as failed (suppl_routines.f:1385-1391)
+      if (d.mode == MQCT_PROP_CC_ODEINT && tid == 0) cc.fail = 1;
+      iv_end = true;
+    }
+  } else if (s == 0) {
+    store_k();
+    if (c.nstp == MAXSTP - 1) {
+      c.rk4 = 1;
+      c.h = copysign(fabs(c.x2 - c.x), c.x2 - c.x1);
+      c.stage = 1;
+      stage_from_k(0.5 * c.h);
+    } else {
+      c.h0 = c.h;
+      if ((c.x + c.h - c.x2) * (c.x + c.h - c.x1) > 0.0) c.h = c.x2 - c.x;
+      c.stage = 1;
+      stage_from_k(CK_B[1][0] * c.h);
+    }
+  } else if (s < 5) {
+    store_k();
+    c.stage = s + 1;
+#pragma unroll
+    for (int j = 0; j < PT; j++) {
+      const int i = tid + j * TB;
+      if (i < N) {
+        double ar = 0.0, ai = 0.0;
+        for (int q = 0; q < s; q++) {
+          ar += CK_B[s + 1][q] * S[(1 + q) * ns + i];
+          ai += CK_B[s + 1][q] * S[(1 + q) * ns + i + N];
+        }
+        ar += CK_B[s + 1][s] * kr[j];
+        ai += CK_B[s + 1][s] * ki[j];
+        xr[j] = yr[j] + c.h * ar;
+        xi[j] = yi[j] + c.h * ai;
+      }
+    }
+    if (tid == 0)
+      for (int cv = 0; cv < 8; cv++) {
+        double a = 0.0;
+        for (int q = 0; q < s; q++) a += CK_B[s + 1][q] * S[(1 + q) * ns + n2 + cv];
+        a += CK_B[s + 1][s] * kc[cv];
+        xcn[cv] = ycl[cv] + c.h * a;
+      }
+  } else {
+    const double C1 = 37.0 / 378.0, C3 = 250.0 / 621.0, C4 = 125.0 / 594.0, C6 = 512.0 / 1771.0;
+    const double DC1 = C1 - 2825.0 / 27648.0, DC3 = C3 - 18575.0 / 48384.0,
+                 DC4 = C4 - 13525.0 / 55296.0, DC5 = -277.0 / 14336.0, DC6 = C6 - .25;
+    const double *k1 = S + ns, *k3 = S + 3 * ns, *k4 = S + 4 * ns, *k5 = S + 5 * ns;
+    double em = 0.0;
+    auto one = [&](int ii, double k6, double yv) {
+      const double yerr = c.h * (DC1 * k1[ii] + DC3 * k3[ii] + DC4 * k4[ii] + DC5 * k5[ii] + DC6 * k6);
+      const double ysc = fabs(yv) + fabs(c.h0 * k1[ii]) + TINY;
+      em = fmax(em, fabs(yerr / ysc));
+      return yv + c.h * (C1 * k1[ii] + C3 * k3[ii] + C4 * k4[ii] + C6 * k6);
+    };
+#pragma unroll
+    for (int j = 0; j < PT; j++) {
+      const int i = tid + j * TB;
+      if (i < N) {
+        xr[j] = one(i, kr[j], yr[j]);
+        xi[j] = one(i + N, ki[j], yi[j]);
+      }
+    }
+    if (tid == 0)
+      for (int q = 0; q < 8; q++) xcn[q] = one(n2 + q, kc[q], ycl[q]);
+    em = block_max<TB>(em, red) / c.eps;
+    const bool accept = (c.h <= c.hmin) || !(em > 1.0);
+    if (!accept) {
+      const double htemp = 0.9 * c.h * pow(em, -.25);
+      const double a = fabs(htemp), b = 0.1 * fabs(c.h);
+      c.h = copysign(a > b ? a : b, c.h);
+      c.stage = 1;  // retry with the same k1
+#pragma unroll
+      for (int j = 0; j < PT; j++) {
+        const int i = tid + j * TB;
+        if (i < N) {
+          xr[j] = yr[j] + CK_B[1][0] * c.h * k1[i];
+          xi[j] = yi[j] + CK_B[1][0] * c.h * k1[i + N];
+        }
+      }
+      if (tid == 0)
+        for (int q = 0; q < 8; q++) xcn[q] = ycl[q] + CK_B[1][0] * c.h * k1[n2 + q];
+    } else {
+      if (c.h > c.hmin) c.hnext = (em > 1.89e-4) ? 0.9 * c.h * pow(em, -.2) : 5.0 * c.h;
+#pragma unroll
+      for (int j = 0; j < PT; j++) {
+        const int i = tid + j * TB;
+        if (i < N) {
+          y[i] = xr[j];
+          y[i + N] = xi[j];
+        }
+      }
+      if (tid == 0)
+        for (int q = 0; q < 8; q++) y[n2 + q] = xcn[q];
+      c.x += c.h;
+      c.nstp++;
+      c.stage = 0;
+      if ((c.x - c.x2) * (c.x2 - c.x1) >= 0.0) {
+        iv_end = true;  // odeint reached the interval's end; its hnext is the next first step
+      } else {
+        if (fabs(c.hnext) < c.hmin) c.hnext = copysign(c.hmin, c.x2 - c.x1);
+        c.h = c.hnext;
+      }
+    }
+  }
+  if (iv_end) {
+    // DOTRAJECT after odeint / rk4 over [tcur, tcur + dt]
+    double nrm = 0.0;
+#pragma unroll
+    for (int j = 0; j < PT; j++)
+      if (tid + j * TB < N) nrm += xr[j] * xr[j] + xi[j] * xi[j];
+    nrm = block_sum<TB>(nrm, red);
+    if (tid == 0) {
+      const mqct_traj_t &p = d.traj[t];
+      c.norm_err = fmax(c.norm_err, fabs(1.0 - nrm));
+      cc.nsteps += c.nstp;
+      if (d.mode == MQCT_PROP_CC_ODEINT) cc.dt_step = c.hnext;
+      bool run = cc.fail == 0;
+      if (run) {
+        const double uR = xcn[0], uphi = xcn[4];
+        if (cc.tcur > p.dt && cc.counter > 2)
+          if ((cc.tmp_R2 - cc.tmp_R1) < 0.0 && (cc.tmp_R1 - uR) > 0.0) cc.vib++;
+        cc.tmp_R2 = cc.tmp_R1;
+        cc.tmp_R1 = uR;
+        cc.tcur += p.dt;
+        const int period_cnt = (int)(fabs(uphi) / 2.0 / MQCT_PI);
+        if (p.numb_orb_prds <= period_cnt) run = false;
+        if (cc.vib >= p.numb_oscl_prds) run = false;
+        if (p.time_lim > 0.0 && cc.tcur >= p.time_lim) run = false;
+        if (uR > p.R_start) run = false;
+        if (p.max_steps > 0 && cc.nsteps >= p.max_steps) {
+          run = false;
+          cc.fail = 2;
+        }
+      }
+      if (!run) {
+        c.status = 1;
+      } else {
+        cc.counter++;
+        c.x = c.x1 = cc.tcur;
+        c.x2 = cc.tcur + p.dt;
+        c.nstp = 0;
+        c.stage = 0;
+        c.rk4 = d.mode == MQCT_PROP_CC_RK4;
+        c.h = c.rk4 ? p.dt : cc.dt_step;
+        c.hnext = c.h;
+      }
+      s_ctl = c;
+    }
+    __syncthreads();
+    c = s_ctl;
+  }
+  if (c.status != 0) {
+    if (tid == 0) {
+      atomicSub(d.active, 1);
+      d.ctl[t] = c;
+      d.cc[t] = cc;
+    }
+    return;
+  }
+  if (tid == 0)
+    for (int q = 0; q < 8; q++) cc.xc[q] = xcn[q];
+  prepare_stage_cc<PT, TB>(d, t, c, cc, xr, xi, par, Ej);
+  if (tid == 0) {
+    d.ctl[t] = c;
+    d.cc[t] = cc;
+  }
+}
+
 // dense block matrices of every knot from the R-major pair arrays
 template <typename KT>
 __global__ void knot_matrices_kernel(long long T, int n_r, const double *__restrict__ Mt,
@@ -1367,29 +1816,87 @@ __global__ __launch_bounds__(BS) void init_kernel(BDev d, const mqct_traj_t *__r
   if (tid == 0) d.ctl[t] = c;
 }
 
+// CC initial conditions (propagator.f:2431-2566): the initial state, R =
+// R_start, P_R = -sqrt(k^2 - l(l+1)/R^2), Theta = pi/2, P_Theta = 0, Phi =
+// 0, P_Phi = sqrt(l(l+1)); the first interval [0, dt].
+template <int PT>
+__global__ __launch_bounds__(BS) void cc_init_kernel(BDev d, const int *__restrict__ state_pos) {
+  const int t = blockIdx.x, tid = threadIdx.x, N = d.N, n2 = 2 * N, ns = n2 + 8;
+  double *S = d.S + (long long)t * NSLOT * ns;
+  const mqct_traj_t p = d.traj[t];
+  const int s0 = state_pos[p.s_st - 1];
+  double xr[PT], xi[PT], Ej[PT];
+#pragma unroll
+  for (int j = 0; j < PT; j++) {
+    const int i = tid + j * BS;
+    xr[j] = xi[j] = Ej[j] = 0.0;
+    if (i < N) {
+      Ej[j] = d.E[i];
+      xr[j] = (i == s0) ? 1.0 : 0.0;
+      S[i] = xr[j];
+      S[i + N] = 0.0;
+    }
+  }
+  const double mu = d.R[-1];
+  const double k_vec = sqrt(mu * 2.0 * p.E_coll), l_orb = sqrt(p.l_real * (p.l_real + 1.0));
+  CtlCC cc{};
+  cc.xc[0] = p.R_start;
+  cc.xc[1] = -sqrt(k_vec * k_vec - l_orb * l_orb / (p.R_start * p.R_start));
+  cc.xc[2] = acos(0.0);
+  cc.xc[3] = l_orb * sin(0.0);
+  cc.xc[4] = 0.0;
+  cc.xc[5] = l_orb * cos(0.0);
+  if (tid < 8) S[n2 + tid] = cc.xc[tid];
+  cc.dt_step = p.dt;
+  cc.counter = 1;
+  Ctl c{};
+  c.x = c.x1 = 0.0;
+  c.x2 = p.dt;
+  c.rk4 = d.mode == MQCT_PROP_CC_RK4;
+  c.h = p.dt;
+  c.hnext = c.h;
+  c.hmin = p.hmin;
+  c.eps = p.eps;
+  prepare_stage_cc<PT>(d, t, c, cc, xr, xi, 0, Ej);
+  if (tid == 0) {
+    d.ctl[t] = c;
+    d.cc[t] = cc;
+  }
+}
+
 __global__ __launch_bounds__(BS) void out_kernel(BDev d, const int *__restrict__ pos_state,
                                                  double *__restrict__ q, mqct_traj_stat_t *__restrict__ st) {
   __shared__ double red[8];
   const int t = blockIdx.x, tid = threadIdx.x, N = d.N, n2 = 2 * N;
-  const double *y = d.S + (long long)t * NSLOT * n2;
+  const int ns = d.cc ? n2 + 8 : n2;  // CC slots carry the classical eight
+  const double *y = d.S + (long long)t * NSLOT * ns;
   double *o = q + (long long)t * (n2 + 8);
   for (int p = tid; p < N; p += BS) {
     const int s = pos_state[p];
     o[s] = y[p];
     o[s + N] = y[p + N];
   }
-  for (int i = tid; i < 8; i += BS) o[n2 + i] = 0.0;
+  for (int i = tid; i < 8; i += BS) o[n2 + i] = d.cc ? y[n2 + i] : 0.0;
   double nrm = 0.0;
   for (int i = tid; i < N; i += BS) nrm += y[i] * y[i] + y[i + N] * y[i + N];
   nrm = block_sum(nrm, red);
   if (tid == 0) {
     const Ctl &c = d.ctl[t];
     mqct_traj_stat_t s;
-    s.t_end = (d.mode == MQCT_PROP_AT_ODEINT) ? c.x2 : c.x;
-    s.n_steps = c.nstp;
-    s.n_rhs = c.nrhs;
-    s.fail = 0;
-    s.norm_err = (d.mode == MQCT_PROP_AT_ODEINT) ? fabs(1.0 - nrm) : c.norm_err;
+    if (d.cc) {
+      const CtlCC &cc = d.cc[t];
+      s.t_end = cc.tcur;
+      s.n_steps = cc.nsteps;
+      s.n_rhs = c.nrhs;
+      s.fail = cc.fail;
+      s.norm_err = c.norm_err;
+    } else {
+      s.t_end = (d.mode == MQCT_PROP_AT_ODEINT) ? c.x2 : c.x;
+      s.n_steps = c.nstp;
+      s.n_rhs = c.nrhs;
+      s.fail = 0;
+      s.norm_err = (d.mode == MQCT_PROP_AT_ODEINT) ? fabs(1.0 - nrm) : c.norm_err;
+    }
     st[t] = s;
   }
 }
@@ -1401,6 +1908,9 @@ int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *t
   DynState *ds = h->dyn;
   const int N = ds->N, n_r = ds->n_r;
   const long long T = ds->T;
+  // CC modes: classical variables in the slots, four product vectors per
+  // trajectory (M v and dM/dR v at two knots), two bucket entries per knot
+  const bool ccm = mode == MQCT_PROP_CC_RK4 || mode == MQCT_PROP_CC_ODEINT;
   (void)hipGetLastError();  // a failed earlier call must not be reported here
   // block order of the states (m12 ascending, states ascending)
   std::vector<int> pos_state, state_pos(N), blk_start, pos_blk(N);
@@ -1432,10 +1942,11 @@ int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *t
                      "propagate: the batched form keeps a stage vector of 2N doubles in LDS; N too large");
   // per position: energies, Coriolis neighbours (state-1 / state+1) and factors
   std::vector<double> E(N), sqv(2 * N);
-  std::vector<int> nbv(2 * N);
+  std::vector<int> nbv(2 * N), mpos(N);
   for (int p = 0; p < N; p++) {
     const int s = pos_state[p];
     E[p] = ds->E[s];
+    mpos[p] = ds->m12[s];
     const double j = ds->j12[s], m = ds->m12[s];
     const double m1 = m + 1.0, m0 = m - 1.0;
     sqv[2 * p] = std::sqrt(j * (j + 1.0) - m0 * (m0 + 1.0));
@@ -1443,8 +1954,9 @@ int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *t
     nbv[2 * p] = s > 0 ? state_pos[s - 1] : -1;
     nbv[2 * p + 1] = s < N - 1 ? state_pos[s + 1] : -1;
   }
-  const int n2 = 2 * N;
-  DevBuf d_ps, d_sp, d_bs, d_bo, d_pb, d_E, d_nb, d_sq, d_R, d_tr;
+  const int n2 = 2 * N, ns = ccm ? n2 + 8 : n2, sw = ccm ? 4 : 2;
+  const long long bcap = ccm ? 2LL * n_traj : n_traj;
+  DevBuf d_ps, d_sp, d_bs, d_bo, d_pb, d_E, d_nb, d_sq, d_R, d_tr, d_mp;
   auto up = [&](DevBuf &b, const void *src, size_t bytes) -> int {
     if (b.ensure(bytes)) return 1;
     return hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, h->stream) != hipSuccess;
@@ -1455,7 +1967,7 @@ int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *t
   if (up(d_ps, pos_state.data(), N * 4) || up(d_sp, state_pos.data(), N * 4) ||
       up(d_bs, blk_start.data(), blk_start.size() * 4) || up(d_bo, blk_off.data(), blk_off.size() * 8) ||
       up(d_pb, pos_blk.data(), N * 4) || up(d_E, E.data(), N * 8) || up(d_nb, nbv.data(), N * 8) ||
-      up(d_sq, sqv.data(), N * 16) ||
+      up(d_sq, sqv.data(), N * 16) || up(d_mp, mpos.data(), N * 4) ||
       up(d_R, Rmu.data(), (n_r + 1) * 8) || up(d_tr, traj, (size_t)n_traj * sizeof(mqct_traj_t)))
     return mqct_fail(MQCT_ERR_NOMEM, "batched: tables");
   // knot matrices: made once per matrix and storage precision
@@ -1547,12 +2059,13 @@ int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *t
   // trajectory state: slots, rotated amplitudes, GEMM outputs, controls,
   // double-buffered buckets, counters
   auto al = [](size_t x) { return (x + 255) / 256 * 256; };
-  const size_t off_V = al((size_t)n_traj * NSLOT * n2 * 8);
+  const size_t off_V = al((size_t)n_traj * NSLOT * ns * 8);
   const size_t off_W = off_V + al((size_t)n_traj * N * 16);
-  const size_t off_C = off_W + al((size_t)n_traj * 2 * N * 16);
-  const size_t off_cnt = off_C + al((size_t)n_traj * sizeof(Ctl));
+  const size_t off_C = off_W + al((size_t)n_traj * sw * N * 16);
+  const size_t off_CC = off_C + al((size_t)n_traj * sizeof(Ctl));
+  const size_t off_cnt = off_CC + (ccm ? al((size_t)n_traj * sizeof(CtlCC)) : 0);
   const size_t off_bk = off_cnt + al((size_t)2 * n_r * 4);
-  const size_t off_act = off_bk + al((size_t)2 * n_r * n_traj * sizeof(Entry));
+  const size_t off_act = off_bk + al((size_t)2 * n_r * bcap * sizeof(Entry));
   const size_t total = off_act + 256;
   if (ds->d_bat.ensure(total)) return mqct_fail(MQCT_ERR_NOMEM, "batched: trajectory state");
   char *base = ds->d_bat.as<char>();
@@ -1562,6 +2075,12 @@ int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *t
   d.n_r = n_r;
   d.n_traj = n_traj;
   d.mode = mode;
+  d.sw = sw;
+  d.bcap = (int)bcap;
+  d.theta_on = ds->theta_on;
+  d.traj = d_tr.as<mqct_traj_t>();
+  d.cc = ccm ? reinterpret_cast<CtlCC *>(base + off_CC) : nullptr;
+  d.m_pos = d_mp.as<int>();
   d.R = d_R.as<double>() + 1;
   d.KM = ds->d_KM.p;
   d.S2 = S2;
@@ -1609,6 +2128,19 @@ int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *t
   // the step kernel reads FP32 products after the FP32-product GEMM forms
   auto w32_of = [&](int sh) -> int { return prod_fp32 && sh >= 0 && kShapes[sh].form >= kPipe ? 1 : 0; };
   auto step = [&](int par, int w32) {
+    if (ccm) {
+      if (stb == 512) {
+        if (N <= 4 * 512) cc_step_kernel<4, 512><<<n_traj, 512, lds_step, h->stream>>>(d, par, w32);
+        else cc_step_kernel<8, 512><<<n_traj, 512, lds_step, h->stream>>>(d, par, w32);
+      } else if (PT == 4) {
+        cc_step_kernel<4><<<n_traj, BS, lds_step, h->stream>>>(d, par, w32);
+      } else if (PT == 8) {
+        cc_step_kernel<8><<<n_traj, BS, lds_step, h->stream>>>(d, par, w32);
+      } else {
+        cc_step_kernel<MAXPT><<<n_traj, BS, lds_step, h->stream>>>(d, par, w32);
+      }
+      return;
+    }
     if (stb == 512) {
       if (N <= 4 * 512) step_kernel<4, 512><<<n_traj, 512, lds_step, h->stream>>>(d, par, w32);
       else step_kernel<8, 512><<<n_traj, 512, lds_step, h->stream>>>(d, par, w32);
@@ -1618,13 +2150,22 @@ int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *t
     else if (PT == 8) step_kernel<8><<<n_traj, BS, lds_step, h->stream>>>(d, par, w32);
     else step_kernel<MAXPT><<<n_traj, BS, lds_step, h->stream>>>(d, par, w32);
   };
-  if (PT == 4) init_kernel<4><<<n_traj, BS, 0, h->stream>>>(d, d_tr.as<mqct_traj_t>(), d_sp.as<int>());
-  else if (PT == 8) init_kernel<8><<<n_traj, BS, 0, h->stream>>>(d, d_tr.as<mqct_traj_t>(), d_sp.as<int>());
-  else init_kernel<MAXPT><<<n_traj, BS, 0, h->stream>>>(d, d_tr.as<mqct_traj_t>(), d_sp.as<int>());
+  if (ccm) {
+    if (PT == 4) cc_init_kernel<4><<<n_traj, BS, 0, h->stream>>>(d, d_sp.as<int>());
+    else if (PT == 8) cc_init_kernel<8><<<n_traj, BS, 0, h->stream>>>(d, d_sp.as<int>());
+    else cc_init_kernel<MAXPT><<<n_traj, BS, 0, h->stream>>>(d, d_sp.as<int>());
+  } else if (PT == 4) {
+    init_kernel<4><<<n_traj, BS, 0, h->stream>>>(d, d_tr.as<mqct_traj_t>(), d_sp.as<int>());
+  } else if (PT == 8) {
+    init_kernel<8><<<n_traj, BS, 0, h->stream>>>(d, d_tr.as<mqct_traj_t>(), d_sp.as<int>());
+  } else {
+    init_kernel<MAXPT><<<n_traj, BS, 0, h->stream>>>(d, d_tr.as<mqct_traj_t>(), d_sp.as<int>());
+  }
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(h->stream));
   const dim3 kb(n_r, nblk);
-  const int pi_ = prod_fp32 ? 2 : (knots_fp32 ? 1 : 0);
+  // tile shapes are kept per product form, for AT (0..2) and CC (3..5)
+  const int pi_ = (prod_fp32 ? 2 : (knots_fp32 ? 1 : 0)) + (ccm ? 3 : 0);
   ds->last_form = pi_;
   auto gemm = [&](int sh, int par) {
     if (knots_fp32) launch_gemm<float>(sh, kb, maxnb, h->stream, d, par, prod_fp32);
@@ -1887,7 +2428,7 @@ extern "C" int mqct_hip_batched_tile(mqct_hip_t *h, int shape, int *form, int *e
 }
 
 extern "C" int mqct_hip_batched_tile_levels(mqct_hip_t *h, int fp32, int n, int *shapes) {
-  if (!h || !h->dyn || !shapes || n < 0 || fp32 < 0 || fp32 > 2)
+  if (!h || !h->dyn || !shapes || n < 0 || fp32 < 0 || fp32 >= DynState::kForms)
     return mqct_fail(MQCT_ERR_ARG, "batched_tile_levels: bad argument");
   for (int l = 0; l < n; l++) shapes[l] = l < DynState::kLevels ? h->dyn->gemm_shape[fp32][l] : -1;
   return MQCT_OK;
@@ -1895,7 +2436,7 @@ extern "C" int mqct_hip_batched_tile_levels(mqct_hip_t *h, int fp32, int n, int 
 
 extern "C" int mqct_hip_batched_set_tile_levels(mqct_hip_t *h, int fp32, int n, const int *shapes) {
   if (!h || !h->dyn) return mqct_fail(MQCT_ERR_STATE, "batched_set_tile_levels: no matrix loaded");
-  if (!shapes || n < 0 || n > DynState::kLevels || fp32 < 0 || fp32 > 2)
+  if (!shapes || n < 0 || n > DynState::kLevels || fp32 < 0 || fp32 >= DynState::kForms)
     return mqct_fail(MQCT_ERR_ARG, "batched_set_tile_levels: bad argument");
   for (int l = 0; l < n; l++)
     if (shapes[l] < -1 || shapes[l] >= kNShapes) return mqct_fail(MQCT_ERR_ARG, "batched_set_tile_levels: unknown shape");

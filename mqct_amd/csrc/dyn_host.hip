@@ -242,10 +242,7 @@ extern "C" int mqct_hip_propagate(mqct_hip_t *h, int mode, int n_traj,
   }
   HIP_TRY(hipSetDevice(h->device));
   if (batched < 0 || batched > 3) return mqct_fail(MQCT_ERR_ARG, "propagate: batched must be 0, 1, 2 or 3");
-  if (batched) {
-    if (!at) return mqct_fail(MQCT_ERR_ARG, "propagate: the batched form serves the AT modes");
-    return mqct_dyn_run_batched(h, mode, n_traj, traj, q_out, stat_out, batched >= 2, batched == 3);
-  }
+  if (batched) return mqct_dyn_run_batched(h, mode, n_traj, traj, q_out, stat_out, batched >= 2, batched == 3);
   return mqct_dyn_run_plain(h, mode, n_traj, traj, q_out, stat_out);
 }
 

@@ -23,14 +23,15 @@ struct DynState {
   DevBuf d_KT;       // the same matrices in MFMA-fragment order (16-row tiles)
   bool kt_valid = false, kt_fp32 = false;
   // interval-GEMM tile shape per product form (FP64 knots, FP32 knots, FP32
-  // knots with FP32 products) and
+  // knots with FP32 products; 0..2 for the AT modes, 3..5 for CC) and
   // level of the running-trajectory count (floor(log2(active))), -1 not yet
   // chosen; a shape fixed by the caller (>= 0) overrides them
+  static constexpr int kForms = 6;
   static constexpr int kLevels = 24;
-  int gemm_shape[3][kLevels];
+  int gemm_shape[kForms][kLevels];
   int gemm_fixed = -1;
   int step_threads = 0;  // batched step kernel: 0 automatic, 256 or 512
-  int gemm_first[3] = {-1, -1, -1};  // shape the last run started with
+  int gemm_first[kForms] = {-1, -1, -1, -1, -1, -1};  // shape the last run started with
   int last_form = 0;                  // product form of the last run
   DynState() {
     for (auto &p : gemm_shape)
