@@ -7,8 +7,9 @@
 !
 ! Input: namelist &demo on the file named by the first argument.
 ! Output: MTRX_UF.DAT (Fortran), MTRX_UF_C.DAT (library), demo_out.bin
-! (stream: the MIJ_SHIFTed Mat_el the trajectories used, Mat_el_der, then
-! the final sys_var of each trajectory).
+! (stream: the MIJ_SHIFTed Mat_el the trajectories used, Mat_el_der, the
+! final sys_var of each trajectory, then the expansion-built Mat_el and
+! Mat_el_der and the projected expansion_terms).
 program mqct_fortran_demo
   use iso_c_binding
   use mqct_hip_bridge
@@ -24,6 +25,8 @@ program mqct_fortran_demo
   integer(c_int64_t) :: total_size
   integer, allocatable :: j12(:), m12(:), indx_chann(:), j_ch(:), ind_mat(:, :), chan_qn(:)
   real(8), allocatable :: xb(:), wb(:), R_COM(:), Mat_el(:, :), Mat_el_der(:, :), Ej(:), sys_var(:, :)
+  real(8), allocatable :: exp_terms(:, :), Mat_exp(:, :), Der_exp(:, :)
+  integer, allocatable :: a_top(:, :)
   type(mqct_traj_t), allocatable :: traj(:)
   type(mqct_traj_stat_t), allocatable :: stats(:)
   type(c_ptr) :: h
@@ -78,6 +81,17 @@ program mqct_fortran_demo
       n_beta, xb, wb, pi / 2d0, pi / 2d0, n_r, R_COM, total_size, ind_mat, conv_unit_e, &
       1d-12 * conv_unit_e, ir1, ir2, MQCT_PREC_FP64, Mat_el, Mat_el_der, &
       pes_kind=MQCT_PES_LEGENDRE, pes_par=par(1:npar))
+
+  ! the same matrix through the PES expansion: the terms L = 0..6 of the
+  ! Legendre potential projected from the handle's V grid
+  ! (CALC_EXPANSION_TERMS), then EXPANSION_MATRIX_ELEMENT
+  allocate(a_top(5, 7), exp_terms(n_r, 7), Mat_exp(n_r, total_size), Der_exp(n_r, total_size))
+  a_top = 0
+  do i = 1, 7
+    a_top(1, i) = i - 1
+  end do
+  call mqct_gpu_expansion_matrix(h, 7, a_top, n_r, R_COM, exp_terms, .true., total_size, ind_mat, &
+      conv_unit_e, 1d-12 * conv_unit_e, ir1, ir2, Mat_exp, Der_exp)
 
   ! MTRX_UF.DAT from Fortran
   open(newunit=u, file="MTRX_UF.DAT", form="unformatted", action="write", status="replace")
@@ -138,7 +152,7 @@ program mqct_fortran_demo
   call mqct_check(mqct_hip_destroy(h), "destroy")
 
   open(newunit=u, file="demo_out.bin", form="unformatted", access="stream", action="write", status="replace")
-  write(u) Mat_el, Mat_el_der, sys_var
+  write(u) Mat_el, Mat_el_der, sys_var, Mat_exp, Der_exp, exp_terms
   close(u)
   write(*, '(A,I0,A,I0,A,I0,A)') "demo: ", states_size, " states, ", total_size, " pairs, ", n_traj, &
       " trajectories"

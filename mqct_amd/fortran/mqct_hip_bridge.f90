@@ -1,11 +1,14 @@
 ! mqct_hip_bridge.f90 -- iso_c_binding interface of libmqct_hip.so for the
 ! MQCT Fortran driver.
 !
-! The interfaces follow include/mqct_hip.h one to one.  The two drop-in
+! The interfaces follow include/mqct_hip.h one to one.  The drop-in
 ! wrappers at the end take the driver's own arrays:
 !   mqct_gpu_matrix_make  replaces the k/R loop of MATRIX_MAKE
 !                         (matrix_ini.f:1916-2016) and the V-grid fill of
 !                         INI_POT_STORAGE (pot_comp.f:333-1788);
+!   mqct_gpu_expansion_matrix replaces its expansion branch
+!                         (matrix_ini.f:1950-1956, EXPANSION_MATRIX_ELEMENT)
+!                         with CALC_EXPANSION_TERMS' projection if asked;
 !   mqct_gpu_trajectories replaces the per-trajectory DOTRAJECT stepping
 !                         (propagator.f:3205-4137) for a batch.
 ! A failing call ends the run with STOP and the library's message, as the
@@ -147,6 +150,81 @@ module mqct_hip_bridge
       integer(c_int) :: indx_chann(*), j12(*), m12(*), chan_qn(*), ind_mat(*)
       real(c_double) :: R_com(*), mat_el(*)
     end function
+    integer(c_int) function mqct_hip_set_mij_shift(h, on) bind(C, name="mqct_hip_set_mij_shift")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: on
+    end function
+    integer(c_int) function mqct_hip_mij_spline(h, n_r, R_com, total_size, mat_el, mat_el_der) &
+        bind(C, name="mqct_hip_mij_spline")
+      import :: c_int, c_ptr, c_double, c_int64_t
+      type(c_ptr), value :: h
+      integer(c_int), value :: n_r
+      integer(c_int64_t), value :: total_size
+      real(c_double) :: R_com(*), mat_el(*), mat_el_der(*)
+    end function
+    integer(c_int) function mqct_hip_use_built_matrix(h, Ej, mass, theta_on) bind(C, name="mqct_hip_use_built_matrix")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double) :: Ej(*)
+      real(c_double), value :: mass
+      integer(c_int), value :: theta_on
+    end function
+    integer(c_int) function mqct_hip_set_identical(h, identical_pes, parity_state, parity_state_bk, same_level) &
+        bind(C, name="mqct_hip_set_identical")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: identical_pes
+      type(c_ptr), value :: parity_state, parity_state_bk, same_level
+    end function
+    integer(c_int) function mqct_hip_set_expansion_terms(h, n_terms, a_top) &
+        bind(C, name="mqct_hip_set_expansion_terms")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: n_terms
+      integer(c_int) :: a_top(*)
+    end function
+    integer(c_int) function mqct_hip_expansion_project(h, terms) bind(C, name="mqct_hip_expansion_project")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double) :: terms(*)
+    end function
+    integer(c_int) function mqct_hip_expansion_build(h, n_r, R_com, terms, total_size, ind_mat, conv_unit_e, &
+        mij_zero_cut, ir_cut1, ir_cut2, mat_el, mat_el_der, skipped) bind(C, name="mqct_hip_expansion_build")
+      import :: c_int, c_ptr, c_double, c_int64_t
+      type(c_ptr), value :: h
+      integer(c_int), value :: n_r, ir_cut1, ir_cut2
+      integer(c_int64_t), value :: total_size
+      type(c_ptr), value :: R_com, terms
+      integer(c_int) :: ind_mat(*)
+      real(c_double), value :: conv_unit_e, mij_zero_cut
+      real(c_double) :: mat_el(*), mat_el_der(*)
+      type(c_ptr), value :: skipped
+    end function
+    integer(c_int) function mqct_hip_cross_sections(h, n_traj, n_states, ldq, q, n_channels, indx_chann, &
+        n_groups, weight, sigma, opacity) bind(C, name="mqct_hip_cross_sections")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      integer(c_int), value :: n_traj, n_states, ldq, n_channels, n_groups
+      real(c_double) :: q(*), weight(*), sigma(*)
+      integer(c_int) :: indx_chann(*)
+      type(c_ptr), value :: opacity
+    end function
+    integer(c_int) function mqct_hip_write_at_path(dir, j12, m12, l, parity, n_rec, records) &
+        bind(C, name="mqct_hip_write_at_path")
+      import :: c_int, c_char, c_double
+      character(kind=c_char) :: dir(*)
+      integer(c_int), value :: j12, m12, l, parity, n_rec
+      real(c_double) :: records(*)
+    end function
+    integer(c_int) function mqct_hip_read_at_path(dir, j12, m12, l, parity, cap, records, n_rec) &
+        bind(C, name="mqct_hip_read_at_path")
+      import :: c_int, c_char, c_double
+      character(kind=c_char) :: dir(*)
+      integer(c_int), value :: j12, m12, l, parity, cap
+      real(c_double) :: records(*)
+      integer(c_int) :: n_rec
+    end function
   end interface
 
 contains
@@ -218,6 +296,34 @@ contains
     end if
     call mqct_check(mqct_hip_matrix_build(h, int(precision, c_int), total_size, im, conv_unit_e, mij_zero_cut, &
                     int(ir_cut1, c_int), int(ir_cut2, c_int), Mat_el, Mat_el_der, c_null_ptr), "matrix_build")
+  end subroutine
+
+  ! MATRIX_MAKE's expansion branch (matrix_ini.f:1950-1956) on the GPU for the
+  ! basis already on the handle: the terms A_TOP(5, nterms) and either the
+  ! coefficients expansion_terms(n_r, nterms) as READ_EXPANSION_TERMS read
+  ! them (project = .false.), or their projection from the handle's V grid,
+  ! returned in expansion_terms (CALC_EXPANSION_TERMS, project = .true.).
+  subroutine mqct_gpu_expansion_matrix(h, nterms, A_TOP, n_r, R_COM, expansion_terms, project, total_size, &
+      ind_mat, conv_unit_e, mij_zero_cut, ir_cut1, ir_cut2, Mat_el, Mat_el_der)
+    type(c_ptr), intent(in) :: h
+    integer, intent(in) :: nterms, n_r, ir_cut1, ir_cut2
+    integer, intent(in) :: A_TOP(5, nterms)
+    real(c_double), intent(in), target :: R_COM(n_r)
+    real(c_double), intent(inout), target :: expansion_terms(n_r, nterms)
+    logical, intent(in) :: project
+    integer(c_int64_t), intent(in) :: total_size
+    integer, intent(in) :: ind_mat(2, total_size)
+    real(c_double), intent(in) :: conv_unit_e, mij_zero_cut
+    real(c_double), intent(out) :: Mat_el(n_r, total_size), Mat_el_der(n_r, total_size)
+    integer(c_int), allocatable :: at(:), im(:)
+    allocate(at(5 * nterms), im(2 * total_size))
+    at = reshape(A_TOP, [5 * nterms])
+    im = reshape(ind_mat, [2 * total_size])
+    call mqct_check(mqct_hip_set_expansion_terms(h, int(nterms, c_int), at), "set_expansion_terms")
+    if (project) call mqct_check(mqct_hip_expansion_project(h, expansion_terms), "expansion_project")
+    call mqct_check(mqct_hip_expansion_build(h, int(n_r, c_int), c_loc(R_COM), c_loc(expansion_terms), &
+                    total_size, im, conv_unit_e, mij_zero_cut, int(ir_cut1, c_int), int(ir_cut2, c_int), &
+                    Mat_el, Mat_el_der, c_null_ptr), "expansion_build")
   end subroutine
 
   ! DOTRAJECT for a batch of CC-MQCT trajectories from the matrix in memory

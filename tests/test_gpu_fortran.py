@@ -2,8 +2,10 @@
 the iso_c_binding bridge (mqct_amd/fortran).  Checks: the MTRX_UF.DAT that
 Fortran's own unformatted WRITEs produce in the PRINT_MIJ_USER layout is
 byte-identical to the library's writer; the matrix the Fortran host got
-matches the oracle to 1e-10 relative; after the driver's MIJ_SHIFT its
-CC-MQCT RK4 trajectories match the oracle's to 1e-9."""
+matches the oracle to 1e-10 relative; the expansion path through the
+bridge (projection and matrix) matches the oracle's to 1e-12; after the
+driver's MIJ_SHIFT its CC-MQCT RK4 trajectories match the oracle's to
+1e-9."""
 import os
 import subprocess
 
@@ -41,10 +43,15 @@ def test_fortran_host_drives_the_library(tmp_path):
     T, n = b.total_size, len(wl.R)
     N = b.n_states
     raw = np.fromfile(tmp_path / "demo_out.bin", dtype=np.float64)
-    assert raw.size == 2 * T * n + 2 * (2 * N + 8)
+    nq = 2 * (2 * N + 8)
+    assert raw.size == 4 * T * n + nq + 7 * n
     M = raw[:T * n].reshape(T, n)            # Mat_el(n_r, T) column-major
     D = raw[T * n:2 * T * n].reshape(T, n)
-    q = raw[2 * T * n:].reshape(2, 2 * N + 8)
+    q = raw[2 * T * n:2 * T * n + nq].reshape(2, 2 * N + 8)
+    o0 = 2 * T * n + nq
+    Me = raw[o0:o0 + T * n].reshape(T, n)
+    De = raw[o0 + T * n:o0 + 2 * T * n].reshape(T, n)
+    Pe = raw[o0 + 2 * T * n:].reshape(7, n)  # expansion_terms(n_r, 7)
     orc = O.oracle_for_workload(wl)
     Mo, Do, So = orc.matrix_make(b.ind_mat, wl.conv_unit_e, wl.mij_zero_cut, *wl.ir_cut)
     # the demo applies the driver's MIJ_SHIFT (matrix_ini.f:2226-2230) before
@@ -56,6 +63,14 @@ def test_fortran_host_drives_the_library(tmp_path):
     Mf = read_mtrx_uf(str(tmp_path / "MTRX_UF.DAT"))["mat_el"]
     assert np.max(np.abs(Mf - Mo) / np.maximum(np.abs(Mo), 1e-300)) <= 1e-10
     assert np.max(np.abs(D - Do)) <= 1e-10 * np.abs(Do).max()
+    # the expansion path through the bridge: projection and matrix against
+    # the oracle's CALC_EXPANSION_TERMS and EXPANSION_MATRIX_ELEMENT
+    terms = [(l,) for l in range(7)]
+    Po = orc.expansion_terms(terms)
+    assert np.max(np.abs(Pe - Po)) <= 1e-12 * np.abs(Po).max()
+    Meo, Deo, _ = orc.expansion_matrix_make(terms, Po, b.ind_mat, wl.conv_unit_e, wl.mij_zero_cut, *wl.ir_cut)
+    assert np.max(np.abs(Me - Meo)) <= 1e-12 * np.abs(Meo).max()
+    assert np.max(np.abs(De - Deo)) <= 1e-9 * np.abs(Deo).max()
     dyn = O.DynOracle(b.j12, b.m12, b.Ej * CM_TO_HARTREE, b.ind_mat, wl.R, M, D, 3.6 * AMUTOAU)
     dt = 20.0 / np.sqrt(500.0 / 300.0)
     for i, l in enumerate(ls):
