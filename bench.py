@@ -122,6 +122,10 @@ def main():
                     help="two-tops trajectories per propagate call (each call is logged when it ends)")
     ap.add_argument("--c3-only", action="store_true",
                     help="run only the two-asymmetric-tops section (no headline line)")
+    ap.add_argument("--c3-dt-study", type=str, default="",
+                    help="comma-separated TIME_STEPs (a.u.): only propagate the FP64 two-tops build's sample "
+                         "(c3-traj trajectories) at each and report the cross-section change between "
+                         "consecutive ones (no headline line)")
     ap.add_argument("--sur-samples", type=int, default=4_000_000)
     ap.add_argument("--sur-steps", type=int, default=30000)
     ap.add_argument("--sur-batch", type=int, default=8192)
@@ -144,6 +148,11 @@ def main():
     from mqct_amd.propagate import Propagator, at_first_step
 
     dev = Device(0)
+    if args.c3_dt_study:
+        rec = c3_dt_study(dev, args, [float(x) for x in args.c3_dt_study.split(",")])
+        dev.close()
+        print(json.dumps({"two_asym_tops_dt_study": rec}), flush=True)
+        return 0
     if args.c3_only:
         c3 = two_tops_section(dev, args)
         dev.close()
@@ -568,6 +577,59 @@ def two_tops_section(dev, args):
     out["surrogate_bf16x3"] = stats
     log("two tops surrogate: fit %.1f s, grid fill %.3f s, xsec err %.2e, elem err max %.2e" % (
         stats["fit_s"], fill_s, e, stats["elem_err"]["max"]))
+    return out
+
+
+def c3_dt_study(dev, args, dts):
+    """Step-size convergence of the two-tops rk4 propagation: the sample of
+    the 1024-trajectory batch (c3_traj trajectories, the same selection as
+    the two-tops section) on the FP64 build, batched with FP64
+    coefficients, at each TIME_STEP of dts.  rk4's global error goes as
+    dt^4, so halving dt leaves about 1/16 of the error; the change between
+    consecutive dts is reported as the largest relative change of the
+    sample's cross sections over the channels above 1e-6, 1e-4 and 1e-2 of
+    the largest, with the norm drift."""
+    from mqct_amd import _lib as L
+    from mqct_amd import workloads as W
+    from mqct_amd.matrix import MatrixBuilder
+    from mqct_amd.propagate import Propagator, at_first_step
+    n = args.c3_traj
+    L.check(dev.lib.mqct_hip_set_progress(dev.h, 60.0), "set_progress")
+    wl = W.two_asym_tops_c3()
+    bb, E = W.two_tops_at_batch(1024)
+    sel = np.linspace(0, 1023, n).astype(int) if n < 1024 else np.arange(1024)
+    bb, E = bb[sel], E[sel]
+    paths, l, st0 = at_first_step(dev, W.two_asym_tops(jmax1=1, jmax2=1), bb, E, time_step=40.0, rec_cap=8000)
+    MatrixBuilder(dev, wl).build("fp64", copy_out=False)
+    prop = Propagator(dev, wl.basis, wl.R, mass_amu=wl.mass_amu, from_build=True)
+    prop.load_paths(paths)
+    out = {"states": int(wl.basis.n_states), "n_traj": n, "sample_of": 1024, "integrator": "AT rk4",
+           "runs": []}
+    prev = None
+    for dt in dts:
+        rows = []
+        for i in range(n):
+            rows += prop.trajectories([l[i]], float(E[i]), 1, time_step=10.0, at_paths=[i], max_steps=3000000)
+            rows[-1]["dt"] = dt
+        t0 = time.perf_counter()
+        q, st = prop.run(L.MQCT_PROP_AT_RK4, rows, batched=1)
+        el = time.perf_counter() - t0
+        sig = xsec_functional(prop, q, l, E)
+        run = {"dt": dt, "traj_per_s": round(n / el, 3), "norm_err_max": float(st["norm_err"].max()),
+               "rhs_mean": float(st["n_rhs"].mean())}
+        if prev is not None:
+            run["xsec_rel_change_vs_dt"] = prev[0]
+            for fl in (1e-6, 1e-4, 1e-2):
+                e, nch = rel_err(sig, prev[1], floor=fl)
+                run["xsec_rel_change_floor_%g" % fl] = e
+                run["channels_floor_%g" % fl] = nch
+        out["runs"].append(run)
+        log("two tops dt study: dt %g, %.3f traj/s, norm drift %.2e%s" % (
+            dt, run["traj_per_s"], run["norm_err_max"],
+            "" if prev is None else ", cross sections change from dt %g by %.2e / %.2e / %.2e (floors 1e-6 / "
+            "1e-4 / 1e-2)" % (prev[0], run["xsec_rel_change_floor_1e-06"], run["xsec_rel_change_floor_0.0001"],
+                              run["xsec_rel_change_floor_0.01"])))
+        prev = (dt, sig)
     return out
 
 
