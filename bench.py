@@ -123,9 +123,12 @@ def main():
     ap.add_argument("--c3-only", action="store_true",
                     help="run only the two-asymmetric-tops section (no headline line)")
     ap.add_argument("--c3-dt-study", type=str, default="",
-                    help="comma-separated TIME_STEPs (a.u.): only propagate the FP64 two-tops build's sample "
-                         "(c3-traj trajectories) at each and report the cross-section change between "
-                         "consecutive ones (no headline line)")
+                    help="comma-separated integrator settings: a number is an rk4 TIME_STEP (a.u.), e<eps> is "
+                         "odeint with that EPS_ODEINT (MIN_TMSTP --c3-hmin); only propagate the FP64 two-tops "
+                         "build's sample (c3-traj trajectories) with each and report the cross-section change "
+                         "between consecutive ones (no headline line)")
+    ap.add_argument("--c3-hmin", type=float, default=0.25,
+                    help="MIN_TMSTP (a.u.) of the two-tops odeint runs of the integrator study")
     ap.add_argument("--sur-samples", type=int, default=4_000_000)
     ap.add_argument("--sur-steps", type=int, default=30000)
     ap.add_argument("--sur-batch", type=int, default=8192)
@@ -149,7 +152,7 @@ def main():
 
     dev = Device(0)
     if args.c3_dt_study:
-        rec = c3_dt_study(dev, args, [float(x) for x in args.c3_dt_study.split(",")])
+        rec = c3_dt_study(dev, args, args.c3_dt_study.split(","))
         dev.close()
         print(json.dumps({"two_asym_tops_dt_study": rec}), flush=True)
         return 0
@@ -580,15 +583,17 @@ def two_tops_section(dev, args):
     return out
 
 
-def c3_dt_study(dev, args, dts):
-    """Step-size convergence of the two-tops rk4 propagation: the sample of
+def c3_dt_study(dev, args, specs):
+    """Integrator convergence of the two-tops propagation: the sample of
     the 1024-trajectory batch (c3_traj trajectories, the same selection as
     the two-tops section) on the FP64 build, batched with FP64
-    coefficients, at each TIME_STEP of dts.  rk4's global error goes as
-    dt^4, so halving dt leaves about 1/16 of the error; the change between
-    consecutive dts is reported as the largest relative change of the
-    sample's cross sections over the channels above 1e-6, 1e-4 and 1e-2 of
-    the largest, with the norm drift."""
+    coefficients, with each integrator setting of specs: a number is an
+    rk4 TIME_STEP (rk4's global error goes as dt^4, so halving dt leaves
+    about 1/16 of it), e<eps> is odeint (Cash-Karp with error control) at
+    EPS_ODEINT eps and MIN_TMSTP c3_hmin.  The change between consecutive
+    settings is reported as the largest relative change of the sample's
+    cross sections over the channels above 1e-6, 1e-4 and 1e-2 of the
+    largest, with the norm drift and the right-hand sides per trajectory."""
     from mqct_amd import _lib as L
     from mqct_amd import workloads as W
     from mqct_amd.matrix import MatrixBuilder
@@ -603,20 +608,27 @@ def c3_dt_study(dev, args, dts):
     MatrixBuilder(dev, wl).build("fp64", copy_out=False)
     prop = Propagator(dev, wl.basis, wl.R, mass_amu=wl.mass_amu, from_build=True)
     prop.load_paths(paths)
-    out = {"states": int(wl.basis.n_states), "n_traj": n, "sample_of": 1024, "integrator": "AT rk4",
-           "runs": []}
+    out = {"states": int(wl.basis.n_states), "n_traj": n, "sample_of": 1024, "runs": []}
     prev = None
-    for dt in dts:
+    for spec in specs:
+        ode = spec.startswith("e")
         rows = []
         for i in range(n):
-            rows += prop.trajectories([l[i]], float(E[i]), 1, time_step=10.0, at_paths=[i], max_steps=3000000)
-            rows[-1]["dt"] = dt
+            if ode:
+                rows += prop.trajectories([l[i]], float(E[i]), 1, time_step=10.0, eps=float(spec[1:]),
+                                          hmin=args.c3_hmin, at_paths=[i])
+            else:
+                rows += prop.trajectories([l[i]], float(E[i]), 1, time_step=10.0, at_paths=[i], max_steps=3000000)
+                rows[-1]["dt"] = float(spec)
         t0 = time.perf_counter()
-        q, st = prop.run(L.MQCT_PROP_AT_RK4, rows, batched=1)
+        q, st = prop.run(L.MQCT_PROP_AT_ODEINT if ode else L.MQCT_PROP_AT_RK4, rows, batched=1)
         el = time.perf_counter() - t0
         sig = xsec_functional(prop, q, l, E)
+        dt = ("odeint eps %s hmin %g" % (spec[1:], args.c3_hmin)) if ode else float(spec)
         run = {"dt": dt, "traj_per_s": round(n / el, 3), "norm_err_max": float(st["norm_err"].max()),
-               "rhs_mean": float(st["n_rhs"].mean())}
+               "rhs_mean": float(st["n_rhs"].mean()), "rhs_max": int(st["n_rhs"].max())}
+        if ode:
+            run["maxstp_reached"] = int((st["n_steps"] >= 10000).sum())
         if prev is not None:
             run["xsec_rel_change_vs_dt"] = prev[0]
             for fl in (1e-6, 1e-4, 1e-2):
@@ -624,9 +636,9 @@ def c3_dt_study(dev, args, dts):
                 run["xsec_rel_change_floor_%g" % fl] = e
                 run["channels_floor_%g" % fl] = nch
         out["runs"].append(run)
-        log("two tops dt study: dt %g, %.3f traj/s, norm drift %.2e%s" % (
-            dt, run["traj_per_s"], run["norm_err_max"],
-            "" if prev is None else ", cross sections change from dt %g by %.2e / %.2e / %.2e (floors 1e-6 / "
+        log("two tops integrator study: %s, %.3f traj/s, rhs mean %.0f, norm drift %.2e%s" % (
+            dt, run["traj_per_s"], run["rhs_mean"], run["norm_err_max"],
+            "" if prev is None else ", cross sections change from %s by %.2e / %.2e / %.2e (floors 1e-6 / "
             "1e-4 / 1e-2)" % (prev[0], run["xsec_rel_change_floor_1e-06"], run["xsec_rel_change_floor_0.0001"],
                               run["xsec_rel_change_floor_0.01"])))
         prev = (dt, sig)

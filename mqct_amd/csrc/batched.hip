@@ -124,6 +124,8 @@ struct BDev {
   int *cnt;                   // [2][n_r]
   Entry *bucket;              // [2][n_r][bcap]
   int *active;                // running trajectories
+  int *alist;                 // [n_list] the trajectories the step kernel's workgroups take (compacted launches)
+  int *alist_tmp;             // [n_traj] compaction space
   unsigned long long *nbucketed;  // right-hand sides with R inside the grid
   unsigned long long *ntouched;   // (GEMM launch, knot) pairs with a non-empty bucket
 };
@@ -170,6 +172,12 @@ __device__ inline int interval(const double *xa, int stride, int n, double x, in
   while (lo > 0 && xa[(long long)lo * stride] > x) lo--;
   while (lo < n - 2 && xa[(long long)(lo + 1) * stride] <= x) lo++;
   return lo;
+}
+
+// Run by one workgroup of the GEMM with bucket parity par: clears the bucket
+// counts that the following step kernel (parity par ^ 1) appends to.
+__device__ inline void clear_next_counts(const BDev &d, int par) {
+  for (int i = threadIdx.x; i < d.n_r; i += blockDim.x) d.cnt[(par ^ 1) * d.n_r + i] = 0;
 }
 
 // The time of stage c.stage (Cash-Karp abscissae for odeint; x, x+h/2,
@@ -374,8 +382,7 @@ template <typename KT, int GE, int GK, int GRT>
 __global__ __launch_bounds__(BS) void gemm_kernel(BDev d, int par) {
   constexpr int GROWS = 4 * GRT * 16;  // rows per workgroup
   const int r = blockIdx.x, b = blockIdx.y;
-  if (r == 0 && b == 0 && blockIdx.z == 0)
-    for (int i = threadIdx.x; i < d.n_r; i += BS) d.cnt[(par ^ 1) * d.n_r + i] = 0;
+  if (r == 0 && b == 0 && blockIdx.z == 0) clear_next_counts(d, par);
   const int ne = d.cnt[par * d.n_r + r];
   if (ne == 0) return;
   const int nb = d.blk_start[b + 1] - d.blk_start[b];
@@ -395,8 +402,7 @@ __global__ __launch_bounds__(BS) void gemm_kernel(BDev d, int par) {
 template <typename KT, int NCT>
 __global__ __launch_bounds__(BS) void gemm_direct_kernel(BDev d, int par) {
   const int r = blockIdx.x, b = blockIdx.y;
-  if (r == 0 && b == 0 && blockIdx.z == 0)
-    for (int i = threadIdx.x; i < d.n_r; i += BS) d.cnt[(par ^ 1) * d.n_r + i] = 0;
+  if (r == 0 && b == 0 && blockIdx.z == 0) clear_next_counts(d, par);
   const int ne = d.cnt[par * d.n_r + r];
   if (ne == 0) return;
   const int p0 = d.blk_start[b], nb = d.blk_start[b + 1] - p0;
@@ -489,8 +495,7 @@ __global__ __launch_bounds__(BS) void gemm_stream_kernel(BDev d, int par) {
   using FV = typename F::V;
   constexpr int U = F::U, KB = 4 * U;
   const int r = blockIdx.x;
-  if (r == 0 && blockIdx.y == 0)
-    for (int i = threadIdx.x; i < d.n_r; i += BS) d.cnt[(par ^ 1) * d.n_r + i] = 0;
+  if (r == 0 && blockIdx.y == 0) clear_next_counts(d, par);
   const int ne = d.cnt[par * d.n_r + r];
   if (ne == 0) return;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -576,8 +581,7 @@ __device__ int chunk_prefix(const BDev &d, int par, int CH, int *pre, int *wtot)
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int nr = d.n_r;
   const int *cnt = d.cnt + par * nr;
-  if (blockIdx.x == 0)
-    for (int i = tid; i < nr; i += BS) d.cnt[(par ^ 1) * nr + i] = 0;
+  if (blockIdx.x == 0) clear_next_counts(d, par);
   const int per = (nr + BS - 1) / BS, ra = min(nr, tid * per), rb = min(nr, ra + per);
   int own = 0, busy = 0;
   for (int r = ra; r < rb; r++) {
@@ -719,15 +723,16 @@ __global__ __launch_bounds__(BS) void gemm_lds_queue_kernel(BDev d, int par) {
 // The software-pipelined product: the whole workgroup takes an item (knot
 // r, m-block b, chunk c of GE bucket entries).  Wave w owns the block's
 // 16-row tiles w and w + 4 (up to 128 rows) and keeps their fragment-ordered
-// [Y | Y2] operands in registers, one slice of GK = 16 contraction rows at
-// a time; the weighted amplitudes of the chunk pass through a
-// double-buffered LDS slice.  The next slice's matrix fragments and
-// amplitudes are loaded while the MFMAs of the current one run, so there is
-// one barrier per slice and the loads overlap the math.
+// [Y | Y2] operands in registers, one slice at a time; a slice is GK = 16
+// (or 32) contraction rows of the Y half and the same rows of the Y2 half,
+// so each amplitude of the chunk is loaded once and staged twice, weighted
+// by w_Y and by w_Y2, into a double-buffered LDS slice.  The next slice's
+// matrix fragments and amplitudes are loaded while the MFMAs of the current
+// one run, so there is one barrier per slice and the loads overlap the math.
 template <typename KT, int GE, int GKS = 16>
 struct PipeShape {
   static constexpr int U = Frag<KT>::U, KB = 4 * U;
-  static constexpr int GK = GKS, LPS = GK / KB;  // contraction rows per slice, fragment loads per tile per slice
+  static constexpr int GK = GKS, LPS = GK / KB;  // rows per slice and half, fragment loads per tile, half and slice
   static constexpr int GC = 2 * GE / 16;        // MFMA column tiles
   // LDS row stride (doubles): the rows lane groups q and q + 1 read at one
   // MFMA step are U apart; U BW = 16 (mod 32) puts them 32 banks apart
@@ -735,14 +740,15 @@ struct PipeShape {
   // the same with FP32 staging (FP32 products): lane groups q read rows
   // 4 apart, 4 BW32 = 16 (mod 64) puts them 16 banks apart
   static constexpr int BW32 = 2 * GE + 4;
-  static constexpr int SPT = GK * GE / BS;      // (row, entry) pairs staged per thread per slice
+  static constexpr int SPT = GK * GE / BS;      // (row, entry) amplitudes loaded per thread per slice
   static_assert(SPT * BS == GK * GE && LPS * KB == GK, "slice shape");
 };
 
 template <typename KT, int GE, int GKS = 16, bool M32 = false>
 struct PipeShared {
   using BT = typename std::conditional<M32, float, double>::type;
-  BT Bs[2][PipeShape<KT, GE, GKS>::GK][M32 ? PipeShape<KT, GE, GKS>::BW32 : PipeShape<KT, GE, GKS>::BW];
+  // rows 0 .. GK-1: w_Y v of the slice's rows; GK .. 2 GK-1: w_Y2 v
+  BT Bs[2][2 * PipeShape<KT, GE, GKS>::GK][M32 ? PipeShape<KT, GE, GKS>::BW32 : PipeShape<KT, GE, GKS>::BW];
   int t[GE], side[GE];
   double wy[GE], wy2[GE];
 };
@@ -768,10 +774,10 @@ __device__ void pipe_item(const BDev &d, int par, int r, int b, int tg, int c,
   const int nr = d.n_r;
   const int *cnt = d.cnt + par * nr;
   const int p0 = d.blk_start[b], nb = d.blk_start[b + 1] - p0;
-  // [Y | Y2] rows of the tile, in fragment loads of KB rows; the last
-  // slice may run past them (GK = 32 with 8-row FP64 fragments): its
+  // each half ([Y | Y2]) of a tile's rows is nbp = nfh KB contraction rows;
+  // the last slice may run past them (GK = 32 with KB = 16 or 8): its
   // fragments and amplitudes are zero there
-  const int nbp = (nb + KB - 1) / KB * KB, nfr = 2 * nbp / KB, nsl = (2 * nbp + GK - 1) / GK;
+  const int nbp = (nb + KB - 1) / KB * KB, nfh = nbp / KB, nsl = (nbp + GK - 1) / GK;
   const int e0 = c * GE, nE = min(GE, cnt[r] - e0), nct = (2 * nE + 15) / 16;
   const Entry *bk = d.bucket + ((long long)par * nr + r) * d.bcap;
   __syncthreads();  // the previous item is done with the entry table and Bs
@@ -803,32 +809,36 @@ __device__ void pipe_item(const BDev &d, int par, int r, int b, int tg, int c,
     A[h] = reinterpret_cast<const FV *>(static_cast<const KT *>(d.KT) + (long long)r * d.KS + d.tile_off[g]) +
            lane;
   }
-  FV an[2][LPS];
+  FV an[2][2][LPS];  // [tile][half][load]
   double2 bn[SPT];
-  double bw[SPT];
-  // loads of slice s into registers: fragments of the owned tiles and the
-  // chunk's amplitudes for rows s GK .. s GK + GK - 1 of [Y | Y2]
+  // loads of slice s into registers: fragments of the owned tiles for rows
+  // s GK .. s GK + GK - 1 of both halves, and the chunk's amplitudes of those rows
   auto load = [&](int s) {
 #pragma unroll
     for (int h = 0; h < 2; h++)
 #pragma unroll
-      for (int l = 0; l < LPS; l++) an[h][l] = (on[h] && s * LPS + l < nfr) ? A[h][(s * LPS + l) * 64] : FV{};
+      for (int hf = 0; hf < 2; hf++)
+#pragma unroll
+        for (int l = 0; l < LPS; l++) {
+          const int f = s * LPS + l;
+          an[h][hf][l] = (on[h] && f < nfh) ? A[h][(hf * nfh + f) * 64] : FV{};
+        }
 #pragma unroll
     for (int j = 0; j < SPT; j++) {
       const int idx = tid + j * BS, kk = idx % GK, e = idx / GK, k = s * GK + kk;
-      const bool y2 = k >= nbp;
-      const int kv = y2 ? k - nbp : k;
-      const bool ok = e < nE && kv < nb;
-      bn[j] = ok ? d.V[(long long)sh.t[e] * d.N + p0 + kv] : make_double2(0.0, 0.0);
-      bw[j] = y2 ? sh.wy2[e] : sh.wy[e];
+      const bool ok = e < nE && k < nb;
+      bn[j] = ok ? d.V[(long long)sh.t[e] * d.N + p0 + k] : make_double2(0.0, 0.0);
     }
   };
   auto stage = [&](int buf) {
 #pragma unroll
     for (int j = 0; j < SPT; j++) {
       const int idx = tid + j * BS, kk = idx % GK, e = idx / GK;
-      sh.Bs[buf][kk][2 * e] = (BT)(bw[j] * bn[j].x);
-      sh.Bs[buf][kk][2 * e + 1] = (BT)(bw[j] * bn[j].y);
+      const double wy = sh.wy[e], wy2 = sh.wy2[e];
+      sh.Bs[buf][kk][2 * e] = (BT)(wy * bn[j].x);
+      sh.Bs[buf][kk][2 * e + 1] = (BT)(wy * bn[j].y);
+      sh.Bs[buf][GK + kk][2 * e] = (BT)(wy2 * bn[j].x);
+      sh.Bs[buf][GK + kk][2 * e + 1] = (BT)(wy2 * bn[j].y);
     }
   };
   AccT acc[2][GC];
@@ -841,38 +851,42 @@ __device__ void pipe_item(const BDev &d, int par, int r, int b, int tg, int c,
   __syncthreads();
   for (int s = 0; s < nsl; s++) {
     const int buf = s & 1;
-    FV ac[2][LPS];
+    FV ac[2][2][LPS];
 #pragma unroll
     for (int h = 0; h < 2; h++)
 #pragma unroll
-      for (int l = 0; l < LPS; l++) ac[h][l] = an[h][l];
+      for (int hf = 0; hf < 2; hf++)
+#pragma unroll
+        for (int l = 0; l < LPS; l++) ac[h][hf][l] = an[h][hf][l];
     const bool more = s + 1 < nsl;
     if (more) load(s + 1);
 #pragma unroll
-    for (int l = 0; l < LPS; l++)
+    for (int hf = 0; hf < 2; hf++)
 #pragma unroll
-      for (int u = 0; u < U; u++) {
-        // fragment order: lane group q holds row KB l + U q + u at step u
-        const BT *brow = &sh.Bs[buf][KB * l + U * q + u][col];
-        BT bf[GC];
+      for (int l = 0; l < LPS; l++)
 #pragma unroll
-        for (int ct = 0; ct < GC; ct++) bf[ct] = (ct < nct) ? brow[16 * ct] : (BT)0;
+        for (int u = 0; u < U; u++) {
+          // fragment order: lane group q holds row KB l + U q + u at step u
+          const BT *brow = &sh.Bs[buf][hf * GK + KB * l + U * q + u][col];
+          BT bf[GC];
 #pragma unroll
-        for (int h = 0; h < 2; h++) {
-          if (!on[h]) continue;
-          if constexpr (M32) {
-            const float a = F::getf(ac[h][l], u);
+          for (int ct = 0; ct < GC; ct++) bf[ct] = (ct < nct) ? brow[16 * ct] : (BT)0;
 #pragma unroll
-            for (int ct = 0; ct < GC; ct++)
-              if (ct < nct) acc[h][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bf[ct], acc[h][ct], 0, 0, 0);
-          } else {
-            const double a = F::get(ac[h][l], u);
+          for (int h = 0; h < 2; h++) {
+            if (!on[h]) continue;
+            if constexpr (M32) {
+              const float a = F::getf(ac[h][hf][l], u);
 #pragma unroll
-            for (int ct = 0; ct < GC; ct++)
-              if (ct < nct) acc[h][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bf[ct], acc[h][ct], 0, 0, 0);
+              for (int ct = 0; ct < GC; ct++)
+                if (ct < nct) acc[h][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bf[ct], acc[h][ct], 0, 0, 0);
+            } else {
+              const double a = F::get(ac[h][hf][l], u);
+#pragma unroll
+              for (int ct = 0; ct < GC; ct++)
+                if (ct < nct) acc[h][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bf[ct], acc[h][ct], 0, 0, 0);
+            }
           }
         }
-      }
     if (more) stage(buf ^ 1);
     __syncthreads();
   }
@@ -1037,12 +1051,15 @@ void launch_gemm(int shape, dim3 knots_blocks, int maxnb, hipStream_t st, const 
 // phases -- the right-hand side k of the completed stage, the integrator
 // update, the next stage vector -- so k is stored only when a later stage
 // reads it, and a finished step's y feeds the next stage without a reload.
-template <int PT, int TB = BS>
+// LIST: workgroup b takes entry b of the compacted list of running
+// trajectories (the grid is the list's length); otherwise trajectory b.
+template <int PT, int TB = BS, bool LIST = false>
 __global__ __launch_bounds__(TB) void step_kernel(BDev d, int par, int w32) {
   extern __shared__ double sys[];  // 2N: the completed stage's vector (Coriolis neighbours)
   __shared__ double red[8];
-  const int t = blockIdx.x, tid = threadIdx.x, N = d.N, n2 = 2 * N;
-  if (t == 0 && tid == 0) *d.qctr = 0;  // item counter of the next GEMM
+  const int tid = threadIdx.x, N = d.N, n2 = 2 * N;
+  if (blockIdx.x == 0 && tid == 0) *d.qctr = 0;  // item counter of the next GEMM
+  const int t = LIST ? __builtin_amdgcn_readfirstlane(d.alist[blockIdx.x]) : (int)blockIdx.x;
   Ctl c = d.ctl[t];
   if (c.status != 0) return;
   double *S = d.S + (long long)t * NSLOT * n2;
@@ -1363,13 +1380,14 @@ __device__ void prepare_stage_cc(const BDev &d, int t, Ctl &c, const CtlCC &cc, 
 // interval of DOTRAJECT with its checks (propagator.f:3995-4050: turning
 // points of R, orbiting periods of Phi, time limit, R past R_start, step
 // cap), and prepares the next stage.
-template <int PT, int TB = BS>
+template <int PT, int TB = BS, bool LIST = false>
 __global__ __launch_bounds__(TB) void cc_step_kernel(BDev d, int par, int w32) {
   extern __shared__ double sys[];  // 2N: the completed stage's vector (Coriolis neighbours)
   __shared__ double red[8];
   __shared__ Ctl s_ctl;            // the control after an interval's end, from thread 0
-  const int t = blockIdx.x, tid = threadIdx.x, N = d.N, n2 = 2 * N, ns = n2 + 8;
-  if (t == 0 && tid == 0) *d.qctr = 0;  // item counter of the next GEMM
+  const int tid = threadIdx.x, N = d.N, n2 = 2 * N, ns = n2 + 8;
+  if (blockIdx.x == 0 && tid == 0) *d.qctr = 0;  // item counter of the next GEMM
+  const int t = LIST ? __builtin_amdgcn_readfirstlane(d.alist[blockIdx.x]) : (int)blockIdx.x;  // as in step_kernel
   Ctl c = d.ctl[t];
   if (c.status != 0) return;
   CtlCC cc = d.cc[t];
@@ -1813,7 +1831,10 @@ __global__ __launch_bounds__(BS) void init_kernel(BDev d, const mqct_traj_t *__r
   c.eps = p.eps;
   c.path = p.at_path;
   prepare_stage<PT>(d, t, c, xr, xi, 0, Ej);
-  if (tid == 0) d.ctl[t] = c;
+  if (tid == 0) {
+    d.ctl[t] = c;
+    d.alist[t] = t;
+  }
 }
 
 // CC initial conditions (propagator.f:2431-2566): the initial state, R =
@@ -1861,7 +1882,38 @@ __global__ __launch_bounds__(BS) void cc_init_kernel(BDev d, const int *__restri
   if (tid == 0) {
     d.ctl[t] = c;
     d.cc[t] = cc;
+    d.alist[t] = t;
   }
+}
+
+// Order-preserving compaction of the step kernel's trajectory list (one
+// workgroup): the entries of alist[0 .. n) whose trajectories still run
+// move to its front, in order.
+__global__ __launch_bounds__(1024) void compact_kernel(const Ctl *__restrict__ ctl, int *__restrict__ alist,
+                                                       int *__restrict__ tmp, int n) {
+  __shared__ int wsum[16];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int per = (n + 1023) / 1024, a = min(n, tid * per), b = min(n, a + per);
+  int own = 0;
+  for (int i = a; i < b; i++) own += ctl[alist[i]].status == 0;
+  int inc = own;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += y;
+  }
+  if (lane == 63) wsum[wave] = inc;
+  __syncthreads();
+  int pos = inc - own, tot = 0;
+  for (int w = 0; w < 16; w++) {
+    if (w < wave) pos += wsum[w];
+    tot += wsum[w];
+  }
+  for (int i = a; i < b; i++) {
+    const int v = alist[i];
+    if (ctl[v].status == 0) tmp[pos++] = v;
+  }
+  __syncthreads();
+  for (int i = tid; i < tot; i += 1024) alist[i] = tmp[i];
 }
 
 __global__ __launch_bounds__(BS) void out_kernel(BDev d, const int *__restrict__ pos_state,
@@ -2066,7 +2118,8 @@ int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *t
   const size_t off_cnt = off_CC + (ccm ? al((size_t)n_traj * sizeof(CtlCC)) : 0);
   const size_t off_bk = off_cnt + al((size_t)2 * n_r * 4);
   const size_t off_act = off_bk + al((size_t)2 * n_r * bcap * sizeof(Entry));
-  const size_t total = off_act + 256;
+  const size_t off_al = off_act + 256;
+  const size_t total = off_al + 2 * al((size_t)n_traj * 4);
   if (ds->d_bat.ensure(total)) return mqct_fail(MQCT_ERR_NOMEM, "batched: trajectory state");
   char *base = ds->d_bat.as<char>();
   BDev d;
@@ -2116,6 +2169,8 @@ int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *t
   d.nbucketed = ds->prof_on ? reinterpret_cast<unsigned long long *>(base + off_act + 8) : nullptr;
   d.ntouched = reinterpret_cast<unsigned long long *>(base + off_act + 16);
   d.qctr = reinterpret_cast<int *>(base + off_act + 24);
+  d.alist = reinterpret_cast<int *>(base + off_al);
+  d.alist_tmp = reinterpret_cast<int *>(base + off_al + al((size_t)n_traj * 4));
   HIP_TRY(hipMemsetAsync(d.cnt, 0, (size_t)2 * n_r * 4, h->stream));
   HIP_TRY(hipMemsetAsync(d.active, 0, 32, h->stream));
   HIP_TRY(hipMemcpyAsync(d.active, &n_traj, 4, hipMemcpyHostToDevice, h->stream));
@@ -2127,29 +2182,40 @@ int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *t
   if (stb != 256 && stb != 512) return mqct_fail(MQCT_ERR_ARG, "batched: step threads must be 256 or 512");
   // the step kernel reads FP32 products after the FP32-product GEMM forms
   auto w32_of = [&](int sh) -> int { return prod_fp32 && sh >= 0 && kShapes[sh].form >= kPipe ? 1 : 0; };
-  auto step = [&](int par, int w32) {
-    if (ccm) {
-      if (stb == 512) {
-        if (N <= 4 * 512) cc_step_kernel<4, 512><<<n_traj, 512, lds_step, h->stream>>>(d, par, w32);
-        else cc_step_kernel<8, 512><<<n_traj, 512, lds_step, h->stream>>>(d, par, w32);
-      } else if (PT == 4) {
-        cc_step_kernel<4><<<n_traj, BS, lds_step, h->stream>>>(d, par, w32);
-      } else if (PT == 8) {
-        cc_step_kernel<8><<<n_traj, BS, lds_step, h->stream>>>(d, par, w32);
-      } else {
-        cc_step_kernel<MAXPT><<<n_traj, BS, lds_step, h->stream>>>(d, par, w32);
+  // step launches: one workgroup per entry of the trajectory list; `grid` is
+  // the list's length (the init kernel lists every trajectory, and the list
+  // is compacted to the running ones when fewer than half of it still run)
+  auto step = [&](int par, int w32, int grid) {
+    const int sg = std::max(1, std::min(n_traj, grid));
+    // the list was never compacted: workgroup b takes trajectory b
+    auto go = [&](auto listc) {
+      constexpr bool LI = decltype(listc)::value;
+      if (ccm) {
+        if (stb == 512) {
+          if (N <= 4 * 512) cc_step_kernel<4, 512, LI><<<sg, 512, lds_step, h->stream>>>(d, par, w32);
+          else cc_step_kernel<8, 512, LI><<<sg, 512, lds_step, h->stream>>>(d, par, w32);
+        } else if (PT == 4) {
+          cc_step_kernel<4, BS, LI><<<sg, BS, lds_step, h->stream>>>(d, par, w32);
+        } else if (PT == 8) {
+          cc_step_kernel<8, BS, LI><<<sg, BS, lds_step, h->stream>>>(d, par, w32);
+        } else {
+          cc_step_kernel<MAXPT, BS, LI><<<sg, BS, lds_step, h->stream>>>(d, par, w32);
+        }
+        return;
       }
-      return;
-    }
-    if (stb == 512) {
-      if (N <= 4 * 512) step_kernel<4, 512><<<n_traj, 512, lds_step, h->stream>>>(d, par, w32);
-      else step_kernel<8, 512><<<n_traj, 512, lds_step, h->stream>>>(d, par, w32);
-      return;
-    }
-    if (PT == 4) step_kernel<4><<<n_traj, BS, lds_step, h->stream>>>(d, par, w32);
-    else if (PT == 8) step_kernel<8><<<n_traj, BS, lds_step, h->stream>>>(d, par, w32);
-    else step_kernel<MAXPT><<<n_traj, BS, lds_step, h->stream>>>(d, par, w32);
+      if (stb == 512) {
+        if (N <= 4 * 512) step_kernel<4, 512, LI><<<sg, 512, lds_step, h->stream>>>(d, par, w32);
+        else step_kernel<8, 512, LI><<<sg, 512, lds_step, h->stream>>>(d, par, w32);
+        return;
+      }
+      if (PT == 4) step_kernel<4, BS, LI><<<sg, BS, lds_step, h->stream>>>(d, par, w32);
+      else if (PT == 8) step_kernel<8, BS, LI><<<sg, BS, lds_step, h->stream>>>(d, par, w32);
+      else step_kernel<MAXPT, BS, LI><<<sg, BS, lds_step, h->stream>>>(d, par, w32);
+    };
+    if (grid >= n_traj) go(std::false_type{});
+    else go(std::true_type{});
   };
+
   if (ccm) {
     if (PT == 4) cc_init_kernel<4><<<n_traj, BS, 0, h->stream>>>(d, d_sp.as<int>());
     else if (PT == 8) cc_init_kernel<8><<<n_traj, BS, 0, h->stream>>>(d, d_sp.as<int>());
@@ -2171,7 +2237,11 @@ int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *t
     if (knots_fp32) launch_gemm<float>(sh, kb, maxnb, h->stream, d, par, prod_fp32);
     else launch_gemm<double>(sh, kb, maxnb, h->stream, d, par);
   };
-  const int ITER = 16;  // even: the bucket parity pattern repeats per replay
+  // iterations per graph replay (even: the bucket parity pattern repeats per
+  // replay); more per replay once few trajectories run and an iteration is
+  // short, so the host's read of the running count between replays costs
+  // less (iterations after the last trajectory ends find empty lists)
+  const int ITER = 16, ITER_LOW = 64, LOW_ACTIVE = 512;
   const bool prof = ds->prof_on;
   int active = n_traj;
   long long it_done = 0;  // iterations run so far (parity of the next one)
@@ -2189,7 +2259,23 @@ int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *t
   // products written (2 x 16 N bytes, or 2 x 8 N as FP32 products); each
   // touched knot's matrices are read once (2 S2 values)
   unsigned long long prof_seen[2] = {0, 0};
+  // the step grid: every trajectory, one workgroup each, until fewer than
+  // COMPACT_BELOW run; then the list of the running ones, compacted again
+  // whenever fewer than half of it still run (a list entry costs each
+  // workgroup one more dependent load, which pays only once most workgroups
+  // of a full grid would find their trajectory ended)
+  const int COMPACT_BELOW = 512;
+  int n_list = n_traj;
+  auto compact = [&]() -> int {
+    if (active > 0 && active < COMPACT_BELOW && 2 * active < n_list) {
+      compact_kernel<<<1, 1024, 0, h->stream>>>(d.ctl, d.alist, d.alist_tmp, n_list);
+      HIP_TRY(hipGetLastError());
+      n_list = active;
+    }
+    return MQCT_OK;
+  };
   auto timed_iters = [&](int sh, int n_it, double &gemm_ms) -> int {
+    if (int rc = compact()) return rc;
     const int active0 = active;
     unsigned long long cnts[2] = {0, 0};
     for (int it = 0; it < n_it; it++) {
@@ -2197,7 +2283,7 @@ int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *t
       HIP_TRY(hipEventRecord(ds->prof_ev[3 * it], h->stream));
       gemm(sh, par);
       HIP_TRY(hipEventRecord(ds->prof_ev[3 * it + 1], h->stream));
-      step(par ^ 1, w32_of(sh));
+      step(par ^ 1, w32_of(sh), n_list);
       HIP_TRY(hipEventRecord(ds->prof_ev[3 * it + 2], h->stream));
     }
     HIP_TRY(hipGetLastError());
@@ -2290,18 +2376,18 @@ int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *t
   };
   // every trajectory ends after at most MAXSTP accepted steps and a bounded
   // number of step-size reductions; the cap only guards the host loop
-  const int max_rounds = 1 << 20;  // a safety bound: 16.7 M iterations
+  const int max_rounds = 1 << 20;  // a safety bound: at least 16.7 M iterations
   HIP_TRY(hipEventRecord(h->ev[5], h->stream));
   int shape = -1;
   if (int rc = shape_for(shape)) return rc;
   ds->gemm_first[pi_] = shape;
-  // ITER iterations (GEMM, step) captured per shape and bucket parity,
+  // ITER (or ITER_LOW) iterations (GEMM, step) captured per shape, bucket parity and step grid,
   // replayed until every trajectory has ended; without graphs (profiling)
   // the same iterations are launched one by one between HIP events
   struct GraphGuard {
     hipGraph_t g = nullptr;
     hipGraphExec_t e = nullptr;
-    int shape = -1, par = -1;
+    int shape = -1, par = -1, grid = 0, iters = 0;
     void reset() {
       if (e) (void)hipGraphExecDestroy(e);
       if (g) (void)hipGraphDestroy(g);
@@ -2332,13 +2418,18 @@ int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *t
       continue;
     }
     const int p0 = (int)(it_done & 1);
-    if (gg.shape != shape || gg.par != p0) {
+    // captured again when the shape, parity, step grid or replay length changes
+    const int n_it = active < LOW_ACTIVE ? ITER_LOW : ITER;
+    if (int rc = compact()) return rc;
+    if (gg.shape != shape || gg.par != p0 || gg.grid != n_list || gg.iters != n_it) {
       gg.reset();
+      gg.grid = n_list;
+      gg.iters = n_it;
       HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-      for (int it = 0; it < ITER; it++) {
+      for (int it = 0; it < n_it; it++) {
         const int par = (it + p0) & 1;
         gemm(shape, par);
-        step(par ^ 1, w32_of(shape));
+        step(par ^ 1, w32_of(shape), gg.grid);
       }
       const hipError_t cap = hipStreamEndCapture(h->stream, &gg.g);
       HIP_TRY(cap);
@@ -2347,7 +2438,7 @@ int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *t
       gg.par = p0;
     }
     HIP_TRY(hipGraphLaunch(gg.e, h->stream));
-    it_done += ITER;
+    it_done += n_it;
     HIP_TRY(hipMemcpyAsync(&active, d.active, 4, hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
   }

@@ -464,8 +464,12 @@ __global__ __launch_bounds__(256) void contract_bf16x3_kernel(
 }
 
 // The three-term split-BF16 contraction with 16-byte staging and a
-// double-buffered LDS (needs G to be a multiple of 8).
-template <int KC, int NW = WAVES, int NRB = NR>
+// double-buffered LDS (needs G to be a multiple of 8).  IL: the six products
+// are issued term by term over all of the wave's accumulators, so
+// consecutive MFMAs write different accumulators instead of a chain of six
+// into one (each accumulator still receives its terms in the same order, so
+// the sums are the same).
+template <int KC, int NW = WAVES, int NRB = NR, bool IL = false>
 __global__ __launch_bounds__(64 * NW) void contract_bf16x3_v_kernel(
     const float *__restrict__ pa_re, const float *__restrict__ pa_im,
     const float *__restrict__ pb_re, const float *__restrict__ pb_im,
@@ -550,39 +554,67 @@ __global__ __launch_bounds__(64 * NW) void contract_bf16x3_v_kernel(
     if (more) load((ch + 1) * KC);
     for (int ks = 0; ks < KC; ks += 32) {
       const int k0 = ks + 8 * kq;
-      float br[8], bi[8];
-#pragma unroll
-      for (int j = 0; j < 8; j++) {
-        br[j] = sB[buf][0][row][k0 + j];
-        bi[j] = has_im ? sB[buf][1][row][k0 + j] : 0.0f;
-      }
+      // eight consecutive grid points per lane, read as two 16-byte vectors
+      // (rows are 16-byte aligned and k0 is a multiple of 8)
+      auto row8 = [&](const float *src, float (&dst)[8]) {
+        const float4 u = reinterpret_cast<const float4 *>(src)[0], v = reinterpret_cast<const float4 *>(src)[1];
+        dst[0] = u.x; dst[1] = u.y; dst[2] = u.z; dst[3] = u.w;
+        dst[4] = v.x; dst[5] = v.y; dst[6] = v.z; dst[7] = v.w;
+      };
+      float br[8], bi[8] = {};
+      row8(&sB[buf][0][row][k0], br);
+      if (has_im) row8(&sB[buf][1][row][k0], bi);
       // the split pair products of all the wave's a-states first, then each
       // R tile's three weight fragments read from LDS once for all of them
       bf16x8 p0[AWn], p1[AWn], p2[AWn];
 #pragma unroll
       for (int a = 0; a < AWn; a++) {
         const int al = wave * AWn + a;
-        float pv[8];
+        float ar[8], ai[8], pv[8];
+        row8(&sA[buf][0][al][k0], ar);
+        if (has_im) row8(&sA[buf][1][al][k0], ai);
 #pragma unroll
         for (int j = 0; j < 8; j++) {
-          pv[j] = sA[buf][0][al][k0 + j] * br[j];
-          if (has_im) pv[j] += sA[buf][1][al][k0 + j] * bi[j];
+          pv[j] = ar[j] * br[j];
+          if (has_im) pv[j] += ai[j] * bi[j];
         }
         split3_pack8(pv, p0[a], p1[a], p2[a]);
       }
+      if constexpr (IL) {
+        bf16x8 w[3][NTB];
 #pragma unroll
-      for (int n = 0; n < NTB; n++) {
-        const bf16x8 w0 = *reinterpret_cast<const bf16x8 *>(&sW[buf][0][n * 16 + row][k0]);
-        const bf16x8 w1 = *reinterpret_cast<const bf16x8 *>(&sW[buf][1][n * 16 + row][k0]);
-        const bf16x8 w2 = *reinterpret_cast<const bf16x8 *>(&sW[buf][2][n * 16 + row][k0]);
+        for (int n = 0; n < NTB; n++)
 #pragma unroll
-        for (int a = 0; a < AWn; a++) {
-          acc[a][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p2[a], w0, acc[a][n], 0, 0, 0);
-          acc[a][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p1[a], w1, acc[a][n], 0, 0, 0);
-          acc[a][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p0[a], w2, acc[a][n], 0, 0, 0);
-          acc[a][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p1[a], w0, acc[a][n], 0, 0, 0);
-          acc[a][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p0[a], w1, acc[a][n], 0, 0, 0);
-          acc[a][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p0[a], w0, acc[a][n], 0, 0, 0);
+          for (int q = 0; q < 3; q++) w[q][n] = *reinterpret_cast<const bf16x8 *>(&sW[buf][q][n * 16 + row][k0]);
+        // terms p2 w0, p1 w1, p0 w2, p1 w0, p0 w1, p0 w0, each over all accumulators
+        auto term = [&](const bf16x8 (&p)[AWn], const bf16x8 (&wq)[NTB]) {
+#pragma unroll
+          for (int n = 0; n < NTB; n++)
+#pragma unroll
+            for (int a = 0; a < AWn; a++)
+              acc[a][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p[a], wq[n], acc[a][n], 0, 0, 0);
+        };
+        term(p2, w[0]);
+        term(p1, w[1]);
+        term(p0, w[2]);
+        term(p1, w[0]);
+        term(p0, w[1]);
+        term(p0, w[0]);
+      } else {
+#pragma unroll
+        for (int n = 0; n < NTB; n++) {
+          const bf16x8 w0 = *reinterpret_cast<const bf16x8 *>(&sW[buf][0][n * 16 + row][k0]);
+          const bf16x8 w1 = *reinterpret_cast<const bf16x8 *>(&sW[buf][1][n * 16 + row][k0]);
+          const bf16x8 w2 = *reinterpret_cast<const bf16x8 *>(&sW[buf][2][n * 16 + row][k0]);
+#pragma unroll
+          for (int a = 0; a < AWn; a++) {
+            acc[a][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p2[a], w0, acc[a][n], 0, 0, 0);
+            acc[a][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p1[a], w1, acc[a][n], 0, 0, 0);
+            acc[a][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p0[a], w2, acc[a][n], 0, 0, 0);
+            acc[a][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p1[a], w0, acc[a][n], 0, 0, 0);
+            acc[a][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p0[a], w1, acc[a][n], 0, 0, 0);
+            acc[a][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p0[a], w0, acc[a][n], 0, 0, 0);
+          }
         }
       }
     }
@@ -780,8 +812,8 @@ struct CVariant {
 };
 constexpr CVariant kCVar[3][4] = {{{32, 4, 64}, {32, 8, 64}, {32, 8, 128}, {0, 0, 0}},  // FP64
                                   {{64, 4, 64}, {64, 8, 64}, {32, 4, 64}, {64, 8, 128}},  // FP32
-                                  {{64, 4, 64}, {64, 8, 64}, {32, 4, 64}, {0, 0, 0}}};    // split-BF16
-constexpr int kNCVar[3] = {3, 4, 3};
+                                  {{64, 4, 64}, {64, 8, 64}, {32, 4, 64}, {64, 8, 64}}};  // split-BF16 (3: term-interleaved)
+constexpr int kNCVar[3] = {3, 4, 4};
 
 // One launch of variant v of a precision form over tiles [0, ntiles).
 static void launch_contraction(mqct_hip *h, int precision, int v, unsigned ntiles, const OutMap &om, double *out) {
@@ -827,6 +859,7 @@ static void launch_contraction(mqct_hip *h, int precision, int v, unsigned ntile
   }
   if (v == 1) contract_bf16x3_v_kernel<64, 8><<<grid, 512, 0, st>>>(ar, ai, br, bi, w0, w1, w2, h->G, h->n_r, tl, om, out);
   else if (v == 2) contract_bf16x3_v_kernel<32, 4><<<grid, 256, 0, st>>>(ar, ai, br, bi, w0, w1, w2, h->G, h->n_r, tl, om, out);
+  else if (v == 3) contract_bf16x3_v_kernel<64, 8, 64, true><<<grid, 512, 0, st>>>(ar, ai, br, bi, w0, w1, w2, h->G, h->n_r, tl, om, out);
   else contract_bf16x3_v_kernel<64, 4><<<grid, 256, 0, st>>>(ar, ai, br, bi, w0, w1, w2, h->G, h->n_r, tl, om, out);
 }
 

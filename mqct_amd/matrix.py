@@ -234,7 +234,7 @@ def _main(argv=None):
     import argparse
     from . import workloads as W
     ap = argparse.ArgumentParser(description=_main.__doc__)
-    ap.add_argument("workload", choices=("diatom_atom", "asym_top_atom", "two_asym_tops"))
+    ap.add_argument("workload", choices=("diatom_atom", "asym_top_atom", "two_asym_tops", "two_asym_tops_c3"))
     ap.add_argument("precisions", nargs="?", default="fp64,fp32,bf16x3")
     ap.add_argument("--reps", type=int, default=2)
     a = ap.parse_args(argv)
