@@ -629,6 +629,7 @@ def c3_dt_study(dev, args, specs):
                "rhs_mean": float(st["n_rhs"].mean()), "rhs_max": int(st["n_rhs"].max())}
         if ode:
             run["maxstp_reached"] = int((st["n_steps"] >= 10000).sum())
+        run["sigma"] = [float(x) for x in sig]
         if prev is not None:
             run["xsec_rel_change_vs_dt"] = prev[0]
             for fl in (1e-6, 1e-4, 1e-2):

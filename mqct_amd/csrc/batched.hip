@@ -62,6 +62,9 @@ struct Ctl {
   int stage, nstp, status, zero;  // status: 0 running, 1 done
   int klo, path, nrhs, rk4;       // rk4: the stages are rk4's (AT rk4, or odeint's last step)
   int last, rec;                  // last: rk4 step that ends the path; rec: path-record hint
+  // the bounds of the current path-record and knot intervals, so a stage
+  // inside them finds its intervals without loads (zero: not yet known)
+  double rec_t0, rec_t1, knot_r0, knot_r1;
 };
 
 // A bucket entry: trajectory, product slot (0: the knot below R, 1: above;
@@ -174,6 +177,20 @@ __device__ inline int interval(const double *xa, int stride, int n, double x, in
   return lo;
 }
 
+// The same with the bounds of the hinted interval cached in (b0, b1): an x
+// with b0 <= x < b1 lies in it without a load.  The bounds kept are those
+// the interval's conditions test (-inf for lo = 0, +inf for lo = n-2), so
+// the cached answer is the one the walk would give.
+__device__ inline int interval_cached(const double *xa, int stride, int n, double x, int &hint, double &b0,
+                                      double &b1) {
+  if (b0 <= x && x < b1) return hint;
+  const int lo = interval(xa, stride, n, x, hint);
+  hint = lo;
+  b0 = lo == 0 ? -INFINITY : xa[(long long)lo * stride];
+  b1 = lo == n - 2 ? INFINITY : xa[(long long)(lo + 1) * stride];
+  return lo;
+}
+
 // Run by one workgroup of the GEMM with bucket parity par: clears the bucket
 // counts that the following step kernel (parity par ^ 1) appends to.
 __device__ inline void clear_next_counts(const BDev &d, int par) {
@@ -216,8 +233,7 @@ __device__ void prepare_stage(const BDev &d, int t, Ctl &c, const double *xr, co
     const int n = (int)(d.poff[c.path + 1] - o0);
     const double *rec = d.prec + o0 * 8;
     const double *der = d.pder + o0 * 7;
-    const int lo = interval(rec, 8, n, tau, c.rec);
-    c.rec = lo;
+    const int lo = interval_cached(rec, 8, n, tau, c.rec, c.rec_t0, c.rec_t1);
     const int hi = lo + 1;
     const double hr = rec[hi * 8] - rec[lo * 8];
     const double a = (rec[hi * 8] - tau) / hr, b = (tau - rec[lo * 8]) / hr;
@@ -233,11 +249,10 @@ __device__ void prepare_stage(const BDev &d, int t, Ctl &c, const double *xr, co
     const int nr = d.n_r;
     c.tau = tau;
     c.zero = R > d.R[nr - 1];
-    const int klo = interval(d.R, 1, nr, R, c.klo), khi = klo + 1;
+    const int klo = interval_cached(d.R, 1, nr, R, c.klo, c.knot_r0, c.knot_r1), khi = klo + 1;
     const double hk = d.R[khi] - d.R[klo];
     const double aa = (d.R[khi] - R) / hk, bb = (R - d.R[klo]) / hk;
     const double cc5 = (hk * hk) / 6.0;
-    c.klo = klo;
     c.aa = aa;
     c.bb = bb;
     c.c1 = (aa * aa * aa - aa) * cc5;
@@ -246,9 +261,8 @@ __device__ void prepare_stage(const BDev &d, int t, Ctl &c, const double *xr, co
       if (d.nbucketed) atomicAdd(d.nbucketed, 1ULL);  // counted only in profiled runs
       int *cnt = d.cnt + par * nr;
       Entry *bk = d.bucket + (long long)par * nr * d.bcap;
-      const int p0 = atomicAdd(&cnt[klo], 1);
+      const int p0 = atomicAdd(&cnt[klo], 1), p1 = atomicAdd(&cnt[khi], 1);
       bk[(long long)klo * d.bcap + p0] = Entry{t, 0, aa, c.c1};
-      const int p1 = atomicAdd(&cnt[khi], 1);
       bk[(long long)khi * d.bcap + p1] = Entry{t, 1, bb, c.c2};
     }
   }
@@ -1351,11 +1365,10 @@ __device__ void prepare_stage_cc(const BDev &d, int t, Ctl &c, const CtlCC &cc, 
     const double R = cc.xc[0];
     c.tau = tau;
     c.zero = R > d.R[nr - 1];
-    const int klo = interval(d.R, 1, nr, R, c.klo), khi = klo + 1;
+    const int klo = interval_cached(d.R, 1, nr, R, c.klo, c.knot_r0, c.knot_r1), khi = klo + 1;
     const double hk = d.R[khi] - d.R[klo];
     const double aa = (d.R[khi] - R) / hk, bb = (R - d.R[klo]) / hk;
     const double cc5 = (hk * hk) / 6.0, cc6 = cc5 / hk;
-    c.klo = klo;
     c.aa = aa;
     c.bb = bb;
     c.c1 = (aa * aa * aa - aa) * cc5;
@@ -1364,10 +1377,9 @@ __device__ void prepare_stage_cc(const BDev &d, int t, Ctl &c, const CtlCC &cc, 
       if (d.nbucketed) atomicAdd(d.nbucketed, 2ULL);  // twice an AT right-hand side's products
       int *cnt = d.cnt + par * nr;
       Entry *bk = d.bucket + (long long)par * nr * d.bcap;
-      const int p0 = atomicAdd(&cnt[klo], 2);
+      const int p0 = atomicAdd(&cnt[klo], 2), p1 = atomicAdd(&cnt[khi], 2);
       bk[(long long)klo * d.bcap + p0] = Entry{t, 0, aa, c.c1};
       bk[(long long)klo * d.bcap + p0 + 1] = Entry{t, 2, -1.0 / hk, -(3.0 * aa * aa - 1.0) * cc6};
-      const int p1 = atomicAdd(&cnt[khi], 2);
       bk[(long long)khi * d.bcap + p1] = Entry{t, 1, bb, c.c2};
       bk[(long long)khi * d.bcap + p1 + 1] = Entry{t, 3, 1.0 / hk, (3.0 * bb * bb - 1.0) * cc6};
     }

@@ -254,6 +254,13 @@ def test_mixed_precision_diatom_atom(dev, prec):
 
 
 @pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+def test_mixed_precision_odd_grid(dev, prec):
+    """An odd number of grid points (63): the contractions take their forms
+    without 16-byte staging."""
+    check_mixed(dev, W.diatom_atom(jmax=5, n_beta=63, n_r=30), prec)
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
 def test_mixed_precision_asym_top(dev, prec):
     check_mixed(dev, W.asym_top_atom(jmax=3, n_beta=12, n_gamma=24, n_r=20), prec)
 
