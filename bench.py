@@ -88,7 +88,7 @@ def main():
     ap.add_argument("--knots", choices=("fp32", "fp64", "fp32p"), default="fp32p",
                     help="storage precision of the spline-coefficient matrices of the timed form "
                          "(fp32p: FP32 storage and FP32 MFMA products in the pipelined GEMM forms)")
-    ap.add_argument("--step-threads", type=int, default=0, choices=(0, 256, 512),
+    ap.add_argument("--step-threads", type=int, default=0, choices=(0, 256, 512, 1024),
                     help="threads per trajectory of the batched step kernel (0: the library's choice)")
     ap.add_argument("--tile-levels", default="",
                     help="comma-separated interval-GEMM shape per level floor(log2(running trajectories)), "

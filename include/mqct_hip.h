@@ -333,8 +333,9 @@ int mqct_hip_batched_set_tile_levels(mqct_hip_t *h, int fp32, int n, const int *
 /* Fixes the interval-GEMM tile shape (index into the candidate list,
  * 0..21) for the following batched runs, or -1 to choose by timing again. */
 int mqct_hip_batched_set_tile(mqct_hip_t *h, int shape);
-/* Threads per trajectory of the batched step kernel: 256 or 512, or 0 for
- * the automatic choice (512 when the states exceed 1024). */
+/* Threads per trajectory of the batched step kernel: 256, 512 or 1024
+ * (AT modes, at most 2048 states; the CC modes run 512), or 0 for the
+ * automatic choice (512 when the states exceed 1024). */
 int mqct_hip_batched_set_step_threads(mqct_hip_t *h, int threads);
 int mqct_hip_batched_kernel_stats(mqct_hip_t *h, int64_t *launches,
                                   double *gemm_ms, double *gemm_flops,

@@ -1070,7 +1070,7 @@ void launch_gemm(int shape, dim3 knots_blocks, int maxnb, hipStream_t st, const 
 template <int PT, int TB = BS, bool LIST = false>
 __global__ __launch_bounds__(TB) void step_kernel(BDev d, int par, int w32) {
   extern __shared__ double sys[];  // 2N: the completed stage's vector (Coriolis neighbours)
-  __shared__ double red[8];
+  __shared__ double red[TB / 64];
   const int tid = threadIdx.x, N = d.N, n2 = 2 * N;
   if (blockIdx.x == 0 && tid == 0) *d.qctr = 0;  // item counter of the next GEMM
   const int t = LIST ? __builtin_amdgcn_readfirstlane(d.alist[blockIdx.x]) : (int)blockIdx.x;
@@ -1079,6 +1079,37 @@ __global__ __launch_bounds__(TB) void step_kernel(BDev d, int par, int w32) {
   double *S = d.S + (long long)t * NSLOT * n2;
   double *y = S;
   const int s = c.stage;
+  // the earlier k slots this stage reads, loaded with the other loads below
+  // (1024 threads, two states per thread): odeint stage s = 1..4 reads k1..ks,
+  // stage 5 k1, k3, k4, k5; rk4's last stage k1, k2, k3
+  constexpr bool PF = PT <= 2 && TB == 1024;
+  int sl[4] = {1, 2, 3, 4}, nsl = 0;
+  if (c.rk4) {
+    nsl = s == 3 ? 3 : 0;
+  } else if (s == 5) {
+    sl[1] = 3;
+    sl[2] = 4;
+    sl[3] = 5;
+    nsl = 4;
+  } else {
+    nsl = s;
+  }
+  double2 kp[PF ? 4 : 1][PF ? PT : 1];
+  if constexpr (PF) {
+#pragma unroll
+    for (int m = 0; m < 4; m++)
+#pragma unroll
+      for (int j = 0; j < PT; j++) {
+        const int i = tid + j * TB;
+        kp[m][j] = (m < nsl && i < N) ? make_double2(S[sl[m] * n2 + i], S[sl[m] * n2 + i + N])
+                                      : make_double2(0.0, 0.0);
+      }
+  }
+  // component p (0 Re, 1 Im) of state tid + j TB of the m-th slot read (slot sl[m])
+  auto K = [&](int m, int j, int p) -> double {
+    if constexpr (PF) return p ? kp[m][j].y : kp[m][j].x;
+    else return S[sl[m] * n2 + tid + j * TB + p * N];
+  };
   // -- k = -i exp(i E tau) (W_lo + W_hi) + Coriolis(stage vector), the stage
   //    vector formed again from its rotated amplitudes
   double kr[PT], ki[PT];
@@ -1188,14 +1219,14 @@ __global__ __launch_bounds__(TB) void step_kernel(BDev d, int par, int w32) {
       stage_from_k(s < 2 ? 0.5 * c.h : c.h);
     } else {
       const double h6 = c.h / 6.0;
-      const double *k1 = S + n2, *k2 = S + 2 * n2, *k3 = S + 3 * n2;
       double nrm = 0.0;
 #pragma unroll
       for (int j = 0; j < PT; j++) {
         const int i = tid + j * TB;
         if (i < N) {
-          xr[j] = yr[j] + h6 * (k1[i] + kr[j] + 2.0 * (k2[i] + k3[i]));
-          xi[j] = yi[j] + h6 * (k1[i + N] + ki[j] + 2.0 * (k2[i + N] + k3[i + N]));
+          // k1, k2, k3 are reads 0, 1, 2
+          xr[j] = yr[j] + h6 * (K(0, j, 0) + kr[j] + 2.0 * (K(1, j, 0) + K(2, j, 0)));
+          xi[j] = yi[j] + h6 * (K(0, j, 1) + ki[j] + 2.0 * (K(1, j, 1) + K(2, j, 1)));
           y[i] = xr[j];
           y[i + N] = xi[j];
           nrm += xr[j] * xr[j] + xi[j] * xi[j];
@@ -1243,9 +1274,12 @@ __global__ __launch_bounds__(TB) void step_kernel(BDev d, int par, int w32) {
       const int i = tid + j * TB;
       if (i < N) {
         double ar = 0.0, ai = 0.0;
-        for (int q = 0; q < s; q++) {
-          ar += CK_B[s + 1][q] * S[(1 + q) * n2 + i];
-          ai += CK_B[s + 1][q] * S[(1 + q) * n2 + i + N];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          if (q < s) {  // k_{q+1} is read q
+            ar += CK_B[s + 1][q] * K(q, j, 0);
+            ai += CK_B[s + 1][q] * K(q, j, 1);
+          }
         }
         ar += CK_B[s + 1][s] * kr[j];
         ai += CK_B[s + 1][s] * ki[j];
@@ -1259,7 +1293,6 @@ __global__ __launch_bounds__(TB) void step_kernel(BDev d, int par, int w32) {
     const double C1 = 37.0 / 378.0, C3 = 250.0 / 621.0, C4 = 125.0 / 594.0, C6 = 512.0 / 1771.0;
     const double DC1 = C1 - 2825.0 / 27648.0, DC3 = C3 - 18575.0 / 48384.0,
                  DC4 = C4 - 13525.0 / 55296.0, DC5 = -277.0 / 14336.0, DC6 = C6 - .25;
-    const double *k1 = S + n2, *k3 = S + 3 * n2, *k4 = S + 4 * n2, *k5 = S + 5 * n2;
     double em = 0.0;
 #pragma unroll
     for (int j = 0; j < PT; j++) {
@@ -1267,13 +1300,14 @@ __global__ __launch_bounds__(TB) void step_kernel(BDev d, int par, int w32) {
       if (i < N) {
 #pragma unroll
         for (int p = 0; p < 2; p++) {
-          const int ii = i + p * N;
+          // k1, k3, k4, k5 are reads 0..3
+          const double k1 = K(0, j, p), k3 = K(1, j, p), k4 = K(2, j, p), k5 = K(3, j, p);
           const double k6 = p ? ki[j] : kr[j];
-          const double yerr = c.h * (DC1 * k1[ii] + DC3 * k3[ii] + DC4 * k4[ii] + DC5 * k5[ii] + DC6 * k6);
+          const double yerr = c.h * (DC1 * k1 + DC3 * k3 + DC4 * k4 + DC5 * k5 + DC6 * k6);
           const double yv = p ? yi[j] : yr[j];
-          const double ysc = fabs(yv) + fabs(c.h0 * k1[ii]) + TINY;
+          const double ysc = fabs(yv) + fabs(c.h0 * k1) + TINY;
           em = fmax(em, fabs(yerr / ysc));
-          const double yn = yv + c.h * (C1 * k1[ii] + C3 * k3[ii] + C4 * k4[ii] + C6 * k6);
+          const double yn = yv + c.h * (C1 * k1 + C3 * k3 + C4 * k4 + C6 * k6);
           if (p) xi[j] = yn;
           else xr[j] = yn;
         }
@@ -1291,8 +1325,8 @@ __global__ __launch_bounds__(TB) void step_kernel(BDev d, int par, int w32) {
       for (int j = 0; j < PT; j++) {
         const int i = tid + j * TB;
         if (i < N) {
-          xr[j] = yr[j] + CK_B[1][0] * c.h * k1[i];
-          xi[j] = yi[j] + CK_B[1][0] * c.h * k1[i + N];
+          xr[j] = yr[j] + CK_B[1][0] * c.h * K(0, j, 0);
+          xi[j] = yi[j] + CK_B[1][0] * c.h * K(0, j, 1);
         }
       }
     } else {
@@ -2191,7 +2225,10 @@ int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *t
   // step kernel: 256 threads per trajectory, or 512 (half the states per
   // thread, twice the loads in flight) when the states fill them
   const int stb = ds->step_threads > 0 ? ds->step_threads : (N > 4 * BS ? 512 : 256);
-  if (stb != 256 && stb != 512) return mqct_fail(MQCT_ERR_ARG, "batched: step threads must be 256 or 512");
+  if (stb != 256 && stb != 512 && stb != 1024)
+    return mqct_fail(MQCT_ERR_ARG, "batched: step threads must be 256, 512 or 1024");
+  if (stb == 1024 && N > 2 * 1024)
+    return mqct_fail(MQCT_ERR_ARG, "batched: 1024 step threads take at most 2048 states");
   // the step kernel reads FP32 products after the FP32-product GEMM forms
   auto w32_of = [&](int sh) -> int { return prod_fp32 && sh >= 0 && kShapes[sh].form >= kPipe ? 1 : 0; };
   // step launches: one workgroup per entry of the trajectory list; `grid` is
@@ -2199,11 +2236,10 @@ int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *t
   // is compacted to the running ones when fewer than half of it still run)
   auto step = [&](int par, int w32, int grid) {
     const int sg = std::max(1, std::min(n_traj, grid));
-    // the list was never compacted: workgroup b takes trajectory b
     auto go = [&](auto listc) {
       constexpr bool LI = decltype(listc)::value;
       if (ccm) {
-        if (stb == 512) {
+        if (stb >= 512) {  // the CC step kernel runs 512 threads at most
           if (N <= 4 * 512) cc_step_kernel<4, 512, LI><<<sg, 512, lds_step, h->stream>>>(d, par, w32);
           else cc_step_kernel<8, 512, LI><<<sg, 512, lds_step, h->stream>>>(d, par, w32);
         } else if (PT == 4) {
@@ -2215,6 +2251,13 @@ int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *t
         }
         return;
       }
+      // 1024 threads (two states per thread) when asked for, or by the
+      // automatic choice once fewer than 256 trajectories run (there it
+      // shortens each workgroup's run; with a full grid 512 is faster)
+      if (stb == 1024 || (ds->step_threads == 0 && stb == 512 && N <= 2 * 1024 && grid < 256)) {
+        step_kernel<2, 1024, LI><<<sg, 1024, lds_step, h->stream>>>(d, par, w32);
+        return;
+      }
       if (stb == 512) {
         if (N <= 4 * 512) step_kernel<4, 512, LI><<<sg, 512, lds_step, h->stream>>>(d, par, w32);
         else step_kernel<8, 512, LI><<<sg, 512, lds_step, h->stream>>>(d, par, w32);
@@ -2224,7 +2267,7 @@ int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *t
       else if (PT == 8) step_kernel<8, BS, LI><<<sg, BS, lds_step, h->stream>>>(d, par, w32);
       else step_kernel<MAXPT, BS, LI><<<sg, BS, lds_step, h->stream>>>(d, par, w32);
     };
-    if (grid >= n_traj) go(std::false_type{});
+    if (grid >= n_traj) go(std::false_type{});  // never compacted: workgroup b takes trajectory b
     else go(std::true_type{});
   };
 
@@ -2496,8 +2539,8 @@ extern "C" int mqct_hip_batched_prof_trace(mqct_hip_t *h, int cap, double *rows,
 
 extern "C" int mqct_hip_batched_set_step_threads(mqct_hip_t *h, int threads) {
   if (!h || !h->dyn) return mqct_fail(MQCT_ERR_STATE, "batched_set_step_threads: no matrix loaded");
-  if (threads != 0 && threads != 256 && threads != 512)
-    return mqct_fail(MQCT_ERR_ARG, "batched_set_step_threads: 0, 256 or 512");
+  if (threads != 0 && threads != 256 && threads != 512 && threads != 1024)
+    return mqct_fail(MQCT_ERR_ARG, "batched_set_step_threads: 0, 256, 512 or 1024");
   h->dyn->step_threads = threads;
   return MQCT_OK;
 }

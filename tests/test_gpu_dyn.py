@@ -328,11 +328,14 @@ def test_at_batched_full_blocks(dev, shape, knots):
     assert np.array_equal(s0["n_steps"], s1["n_steps"])
 
 
+@pytest.mark.parametrize("threads", [512, 1024])
 @pytest.mark.parametrize("mode", [L.MQCT_PROP_AT_RK4, L.MQCT_PROP_AT_ODEINT])
-def test_at_batched_step_threads(dev, mode):
+def test_at_batched_step_threads(dev, mode, threads):
     """The step kernel with 512 threads per trajectory (two states per
-    thread fewer, block reductions over eight waves) against the plain
-    kernel, on 969 states where the automatic choice would take 256."""
+    thread fewer, block reductions over eight waves) and with 1024 (two
+    states per thread, the earlier k slots loaded with the stage's other
+    loads) against the plain kernel, on 969 states where the automatic
+    choice would take 256."""
     from mqct_amd.matrix import MatrixBuilder
     from mqct_amd.propagate import at_first_step
     wl = W.asym_top_atom(jmax=8, n_beta=20, n_gamma=40, n_r=120)
@@ -346,7 +349,7 @@ def test_at_batched_step_threads(dev, mode):
     for i in range(len(bb)):
         rows += prop.trajectories([l[i]], float(E[i]), 1, time_step=10.0, eps=1e-8, hmin=20.0, at_paths=[i])
     q0, s0 = prop.run(mode, rows, batched=0)
-    L.check(dev.lib.mqct_hip_batched_set_step_threads(dev.h, 512), "set_step_threads")
+    L.check(dev.lib.mqct_hip_batched_set_step_threads(dev.h, threads), "set_step_threads")
     try:
         q1, s1 = prop.run(mode, rows, batched=1)
     finally:
@@ -354,7 +357,7 @@ def test_at_batched_step_threads(dev, mode):
     N = wl.basis.n_states
     d = np.max(np.abs(q1[:, :2 * N] - q0[:, :2 * N]))
     tol = 1e-9 if mode == L.MQCT_PROP_AT_RK4 else 1e-6
-    assert d <= tol, f"512-thread step kernel vs plain amplitude diff {d}"
+    assert d <= tol, f"{threads}-thread step kernel vs plain amplitude diff {d}"
     if mode == L.MQCT_PROP_AT_RK4:
         assert np.array_equal(s0["n_steps"], s1["n_steps"])
 
