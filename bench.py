@@ -252,7 +252,7 @@ def main():
                                                                              v["step_us_per_it"])
                                    for k, v in levels.items()))
     L.check(dev.lib.mqct_hip_batched_profile(dev.h, 0), "batched_profile")
-    FORMS = ("lds", "direct", "stream", "queue", "lds_queue", "pipe", "pipe_dyn")
+    FORMS = ("lds", "direct", "stream", "queue", "lds_queue", "pipe", "pipe_dyn", "stream_list")
 
     def shape_desc(sh):
         fo, ge, gk, grt = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()

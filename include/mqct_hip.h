@@ -303,7 +303,7 @@ int mqct_hip_batched_profile(mqct_hip_t *h, int on);
  * ms, shape index, summed step-kernel ms); copies min(cap, n) rows of 5
  * doubles into rows and returns their number in n. */
 int mqct_hip_batched_prof_trace(mqct_hip_t *h, int cap, double *rows, int *n);
-/* The interval-GEMM candidates: shape index 0..21 (shape = -1: the shape
+/* The interval-GEMM candidates: shape index 0..22 (shape = -1: the shape
  * the last batched run started with, for the coefficient precision of that
  * run).  form: 0 LDS-staged per knot, 1 per-wave loads per knot, 2 the same
  * over the fragment-ordered knot tiles, 3 fragment-ordered work queue of
@@ -312,7 +312,8 @@ int mqct_hip_batched_prof_trace(mqct_hip_t *h, int cap, double *rows, int *n);
  * software-pipelined work queue of (knot, row group, entry chunk) items
  * (a row group: up to 128 rows of one m-block, contracted over the whole
  * block), 6 the same with the items handed out one at a time from a device
- * counter, largest row groups first;
+ * counter, largest row groups first, 7 form 2 over the list of knots with
+ * bucket entries (a grid of two workgroup columns per running trajectory);
  * entries: bucket entries per super-tile (per wave for forms 1-3); k_rows:
  * K rows per LDS slice (0 without LDS); row_tiles: 16-row tiles per wave.
  * The runner chooses a shape per level floor(log2(running trajectories)) by
@@ -331,7 +332,7 @@ int mqct_hip_batched_tile_levels(mqct_hip_t *h, int fp32, int n, int *shapes);
  * knots). */
 int mqct_hip_batched_set_tile_levels(mqct_hip_t *h, int fp32, int n, const int *shapes);
 /* Fixes the interval-GEMM tile shape (index into the candidate list,
- * 0..21) for the following batched runs, or -1 to choose by timing again. */
+ * 0..22) for the following batched runs, or -1 to choose by timing again. */
 int mqct_hip_batched_set_tile(mqct_hip_t *h, int shape);
 /* Threads per trajectory of the batched step kernel: 256, 512 or 1024
  * (AT modes, at most 2048 states; the CC modes run 512), or 0 for the

@@ -128,6 +128,8 @@ struct BDev {
   Entry *bucket;              // [2][n_r][bcap]
   int *active;                // running trajectories
   int *alist;                 // [n_list] the trajectories the step kernel's workgroups take (compacted launches)
+  int *klist;                 // [2][n_r] the knots with bucket entries, by bucket parity
+  int *kcnt;                  // [2] their counts
   int *alist_tmp;             // [n_traj] compaction space
   unsigned long long *nbucketed;  // right-hand sides with R inside the grid
   unsigned long long *ntouched;   // (GEMM launch, knot) pairs with a non-empty bucket
@@ -195,6 +197,7 @@ __device__ inline int interval_cached(const double *xa, int stride, int n, doubl
 // counts that the following step kernel (parity par ^ 1) appends to.
 __device__ inline void clear_next_counts(const BDev &d, int par) {
   for (int i = threadIdx.x; i < d.n_r; i += blockDim.x) d.cnt[(par ^ 1) * d.n_r + i] = 0;
+  if (threadIdx.x == 0) d.kcnt[par ^ 1] = 0;
 }
 
 // The time of stage c.stage (Cash-Karp abscissae for odeint; x, x+h/2,
@@ -264,6 +267,9 @@ __device__ void prepare_stage(const BDev &d, int t, Ctl &c, const double *xr, co
       const int p0 = atomicAdd(&cnt[klo], 1), p1 = atomicAdd(&cnt[khi], 1);
       bk[(long long)klo * d.bcap + p0] = Entry{t, 0, aa, c.c1};
       bk[(long long)khi * d.bcap + p1] = Entry{t, 1, bb, c.c2};
+      // a knot's first entry lists the knot for the knot-list GEMM form
+      if (p0 == 0) d.klist[par * nr + atomicAdd(&d.kcnt[par], 1)] = klo;
+      if (p1 == 0) d.klist[par * nr + atomicAdd(&d.kcnt[par], 1)] = khi;
     }
   }
 }
@@ -503,13 +509,20 @@ struct Frag<float> {
 // permuted (lane group q holds k = 4U s + U q + u for step u) and the
 // weighted amplitudes follow the same order.  Tile rows past the block and
 // columns past nb are zero; amplitude reads clamp k to the block.
-template <typename KT, int NCT>
+// KLIST: blockIdx.x indexes the list of knots with entries (the grid's x
+// extent bounds the list: two knots per running trajectory), not the knots,
+// so few running trajectories launch few workgroups.
+template <typename KT, int NCT, bool KLIST = false>
 __global__ __launch_bounds__(BS) void gemm_stream_kernel(BDev d, int par) {
   using F = Frag<KT>;
   using FV = typename F::V;
   constexpr int U = F::U, KB = 4 * U;
-  const int r = blockIdx.x;
-  if (r == 0 && blockIdx.y == 0) clear_next_counts(d, par);
+  if (blockIdx.x == 0 && blockIdx.y == 0) clear_next_counts(d, par);
+  int r = blockIdx.x;
+  if (KLIST) {
+    if ((int)blockIdx.x >= __builtin_amdgcn_readfirstlane(d.kcnt[par])) return;
+    r = __builtin_amdgcn_readfirstlane(d.klist[par * d.n_r + blockIdx.x]);
+  }
   const int ne = d.cnt[par * d.n_r + r];
   if (ne == 0) return;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -991,7 +1004,7 @@ __global__ __launch_bounds__(BS, GE >= 64 ? 1 : (GE >= 32 ? 2 : 3)) void gemm_pi
 // The GEMM forms and tile shapes the runner chooses from: form, bucket
 // entries per super-tile (per wave for the LDS-free forms), K rows per LDS
 // slice, 16-row tiles per wave.
-enum GemmForm { kLds = 0, kDirect = 1, kStream = 2, kQueue = 3, kLdsQueue = 4, kPipe = 5, kPipeDyn = 6 };
+enum GemmForm { kLds = 0, kDirect = 1, kStream = 2, kQueue = 3, kLdsQueue = 4, kPipe = 5, kPipeDyn = 6, kStreamList = 7 };
 struct GemmShape {
   int form, ge, gk, grt;
 };
@@ -1001,13 +1014,18 @@ constexpr GemmShape kShapes[] = {
     {kStream, 8, 0, 1},     {kStream, 16, 0, 1},  {kStream, 32, 0, 1},  {kQueue, 16, 0, 1},
     {kQueue, 32, 0, 1},     {kLdsQueue, 64, 16, 1}, {kLdsQueue, 32, 16, 1}, {kPipe, 64, 16, 2},
     {kPipe, 32, 16, 2},     {kPipeDyn, 64, 16, 2},  {kPipeDyn, 32, 16, 2},  {kPipeDyn, 16, 16, 2},
-    {kPipeDyn, 32, 32, 2},  {kPipeDyn, 16, 32, 2}};
+    {kPipeDyn, 32, 32, 2},  {kPipeDyn, 16, 32, 2},  {kStreamList, 8, 0, 1}};
 constexpr int kNShapes = sizeof(kShapes) / sizeof(kShapes[0]);
 // the work-queue forms keep the chunk prefix of all knots in LDS
-constexpr bool uses_queue(int shape) { return kShapes[shape].form >= kQueue; }
+constexpr bool uses_queue(int shape) {
+  return kShapes[shape].form == kQueue || kShapes[shape].form == kLdsQueue || kShapes[shape].form == kPipe ||
+         kShapes[shape].form == kPipeDyn;
+}
+// the software-pipelined forms (FP32 products with batched = 3)
+constexpr bool pipelined(int shape) { return kShapes[shape].form == kPipe || kShapes[shape].form == kPipeDyn; }
 
 template <typename KT>
-void launch_gemm(int shape, dim3 knots_blocks, int maxnb, hipStream_t st, const BDev &d, int par,
+void launch_gemm(int shape, dim3 knots_blocks, int maxnb, hipStream_t st, const BDev &d, int par, int kx,
                  bool m32 = false) {
   const size_t shq = (size_t)(d.n_r + 1) * 4;
   const dim3 gs(knots_blocks.x, (d.ntile + 3) / 4);
@@ -1027,6 +1045,7 @@ void launch_gemm(int shape, dim3 knots_blocks, int maxnb, hipStream_t st, const 
   }
   switch (shape) {
     case 8: gemm_stream_kernel<KT, 1><<<gs, BS, 0, st>>>(d, par); return;
+    case 22: gemm_stream_kernel<KT, 1, true><<<dim3(kx, gs.y), BS, 0, st>>>(d, par); return;
     case 9: gemm_stream_kernel<KT, 2><<<gs, BS, 0, st>>>(d, par); return;
     case 10: gemm_stream_kernel<KT, 4><<<gs, BS, 0, st>>>(d, par); return;
     case 11: gemm_queue_kernel<KT, 2><<<QGRID, BS, shq, st>>>(d, par); return;
@@ -1412,6 +1431,8 @@ __device__ void prepare_stage_cc(const BDev &d, int t, Ctl &c, const CtlCC &cc, 
       int *cnt = d.cnt + par * nr;
       Entry *bk = d.bucket + (long long)par * nr * d.bcap;
       const int p0 = atomicAdd(&cnt[klo], 2), p1 = atomicAdd(&cnt[khi], 2);
+      if (p0 == 0) d.klist[par * nr + atomicAdd(&d.kcnt[par], 1)] = klo;
+      if (p1 == 0) d.klist[par * nr + atomicAdd(&d.kcnt[par], 1)] = khi;
       bk[(long long)klo * d.bcap + p0] = Entry{t, 0, aa, c.c1};
       bk[(long long)klo * d.bcap + p0 + 1] = Entry{t, 2, -1.0 / hk, -(3.0 * aa * aa - 1.0) * cc6};
       bk[(long long)khi * d.bcap + p1] = Entry{t, 1, bb, c.c2};
@@ -2165,7 +2186,8 @@ int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *t
   const size_t off_bk = off_cnt + al((size_t)2 * n_r * 4);
   const size_t off_act = off_bk + al((size_t)2 * n_r * bcap * sizeof(Entry));
   const size_t off_al = off_act + 256;
-  const size_t total = off_al + 2 * al((size_t)n_traj * 4);
+  const size_t off_kl = off_al + 2 * al((size_t)n_traj * 4);
+  const size_t total = off_kl + al((size_t)2 * n_r * 4);
   if (ds->d_bat.ensure(total)) return mqct_fail(MQCT_ERR_NOMEM, "batched: trajectory state");
   char *base = ds->d_bat.as<char>();
   BDev d;
@@ -2217,8 +2239,10 @@ int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *t
   d.qctr = reinterpret_cast<int *>(base + off_act + 24);
   d.alist = reinterpret_cast<int *>(base + off_al);
   d.alist_tmp = reinterpret_cast<int *>(base + off_al + al((size_t)n_traj * 4));
+  d.klist = reinterpret_cast<int *>(base + off_kl);
+  d.kcnt = reinterpret_cast<int *>(base + off_act + 32);
   HIP_TRY(hipMemsetAsync(d.cnt, 0, (size_t)2 * n_r * 4, h->stream));
-  HIP_TRY(hipMemsetAsync(d.active, 0, 32, h->stream));
+  HIP_TRY(hipMemsetAsync(d.active, 0, 40, h->stream));
   HIP_TRY(hipMemcpyAsync(d.active, &n_traj, 4, hipMemcpyHostToDevice, h->stream));
   // states per thread of the step kernel
   const int PT = N <= 4 * BS ? 4 : (N <= 8 * BS ? 8 : MAXPT);
@@ -2230,7 +2254,7 @@ int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *t
   if (stb == 1024 && N > 2 * 1024)
     return mqct_fail(MQCT_ERR_ARG, "batched: 1024 step threads take at most 2048 states");
   // the step kernel reads FP32 products after the FP32-product GEMM forms
-  auto w32_of = [&](int sh) -> int { return prod_fp32 && sh >= 0 && kShapes[sh].form >= kPipe ? 1 : 0; };
+  auto w32_of = [&](int sh) -> int { return prod_fp32 && sh >= 0 && pipelined(sh) ? 1 : 0; };
   // step launches: one workgroup per entry of the trajectory list; `grid` is
   // the list's length (the init kernel lists every trajectory, and the list
   // is compacted to the running ones when fewer than half of it still run)
@@ -2288,9 +2312,13 @@ int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *t
   // tile shapes are kept per product form, for AT (0..2) and CC (3..5)
   const int pi_ = (prod_fp32 ? 2 : (knots_fp32 ? 1 : 0)) + (ccm ? 3 : 0);
   ds->last_form = pi_;
+  int n_list = n_traj;  // the step grid (the trajectory list's length; see compact below)
+  // (the knot-list form launches two workgroup columns per trajectory of the
+  // step grid: each running trajectory lists at most two knots)
   auto gemm = [&](int sh, int par) {
-    if (knots_fp32) launch_gemm<float>(sh, kb, maxnb, h->stream, d, par, prod_fp32);
-    else launch_gemm<double>(sh, kb, maxnb, h->stream, d, par);
+    const int kx = std::max(1, (int)std::min<long long>(n_r, 2LL * n_list));
+    if (knots_fp32) launch_gemm<float>(sh, kb, maxnb, h->stream, d, par, kx, prod_fp32);
+    else launch_gemm<double>(sh, kb, maxnb, h->stream, d, par, kx);
   };
   // iterations per graph replay (even: the bucket parity pattern repeats per
   // replay); more per replay once few trajectories run and an iteration is
@@ -2320,7 +2348,6 @@ int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *t
   // workgroup one more dependent load, which pays only once most workgroups
   // of a full grid would find their trajectory ended)
   const int COMPACT_BELOW = 512;
-  int n_list = n_traj;
   auto compact = [&]() -> int {
     if (active > 0 && active < COMPACT_BELOW && 2 * active < n_list) {
       compact_kernel<<<1, 1024, 0, h->stream>>>(d.ctl, d.alist, d.alist_tmp, n_list);

@@ -24,7 +24,7 @@ import oracle.oracle as O
 pytestmark = pytest.mark.gpu
 
 U_CM = 500.0
-N_SHAPES = 22
+N_SHAPES = 23
 
 
 @pytest.fixture(scope="module")

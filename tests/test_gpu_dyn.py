@@ -201,7 +201,7 @@ def test_at_batched_fp32_coefficients(dev):
     assert np.all(s32["norm_err"] < 1e-6)
 
 
-N_SHAPES = 22
+N_SHAPES = 23
 
 
 @pytest.mark.parametrize("shape", list(range(N_SHAPES)))
