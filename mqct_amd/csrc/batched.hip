@@ -215,63 +215,91 @@ __device__ inline double stage_time(const Ctl &c) {
 // (dPhi/dt for the Coriolis term), finds the knots and spline weights,
 // records them in its c and appends the trajectory to the buckets of
 // parity `par`.
-template <int PT, int TB = BS>
+// Thread 0's part of preparing stage c.stage: evaluates the recorded path at
+// the stage time (dPhi/dt for the Coriolis term), finds the knots and spline
+// weights, records them in c and appends the trajectory to the buckets of
+// parity `par`.
+__device__ void stage_chain(const BDev &d, int t, Ctl &c, int par) {
+  const double tau = stage_time(c);
+  // recorded path at tau: R, Theta, P_Phi (DERIVS_BK_adia, mqct_eqtns.f:748-759)
+  const long long o0 = d.poff[c.path];
+  const int n = (int)(d.poff[c.path + 1] - o0);
+  const double *rec = d.prec + o0 * 8;
+  const double *der = d.pder + o0 * 7;
+  const int lo = interval_cached(rec, 8, n, tau, c.rec, c.rec_t0, c.rec_t1);
+  const int hi = lo + 1;
+  const double hr = rec[hi * 8] - rec[lo * 8];
+  const double a = (rec[hi * 8] - tau) / hr, b = (tau - rec[lo * 8]) / hr;
+  const double ca = a * a * a - a, cb = b * b * b - b;
+  double pv[3];
+  const int col[3] = {0, 2, 5};
+  for (int q = 0; q < 3; q++)
+    pv[q] = a * rec[lo * 8 + 1 + col[q]] + b * rec[hi * 8 + 1 + col[q]] +
+            (ca * der[lo * 7 + col[q]] + cb * der[hi * 7 + col[q]]) * (hr * hr) / 6.0;
+  const double R = pv[0], st = sin(pv[1]);
+  c.dphidt = pv[2] / R / R / d.R[-1] / (st * st);
+  // knots of R (splint_bk, mqct_eqtns.f:575-607); R > R_n: no coupling
+  const int nr = d.n_r;
+  c.tau = tau;
+  c.zero = R > d.R[nr - 1];
+  const int klo = interval_cached(d.R, 1, nr, R, c.klo, c.knot_r0, c.knot_r1), khi = klo + 1;
+  const double hk = d.R[khi] - d.R[klo];
+  const double aa = (d.R[khi] - R) / hk, bb = (R - d.R[klo]) / hk;
+  const double cc5 = (hk * hk) / 6.0;
+  c.aa = aa;
+  c.bb = bb;
+  c.c1 = (aa * aa * aa - aa) * cc5;
+  c.c2 = (bb * bb * bb - bb) * cc5;
+  if (!c.zero) {
+    if (d.nbucketed) atomicAdd(d.nbucketed, 1ULL);  // counted only in profiled runs
+    int *cnt = d.cnt + par * nr;
+    Entry *bk = d.bucket + (long long)par * nr * d.bcap;
+    const int p0 = atomicAdd(&cnt[klo], 1), p1 = atomicAdd(&cnt[khi], 1);
+    bk[(long long)klo * d.bcap + p0] = Entry{t, 0, aa, c.c1};
+    bk[(long long)khi * d.bcap + p1] = Entry{t, 1, bb, c.c2};
+    // a knot's first entry lists the knot for the knot-list GEMM form
+    if (p0 == 0) d.klist[par * nr + atomicAdd(&d.kcnt[par], 1)] = klo;
+    if (p1 == 0) d.klist[par * nr + atomicAdd(&d.kcnt[par], 1)] = khi;
+  }
+}
+
+// The fields stage_chain sets in a control record.
+__device__ inline void copy_chain(Ctl &dst, const Ctl &src) {
+  dst.tau = src.tau;
+  dst.dphidt = src.dphidt;
+  dst.zero = src.zero;
+  dst.aa = src.aa;
+  dst.bb = src.bb;
+  dst.c1 = src.c1;
+  dst.c2 = src.c2;
+  dst.klo = src.klo;
+  dst.rec = src.rec;
+  dst.rec_t0 = src.rec_t0;
+  dst.rec_t1 = src.rec_t1;
+  dst.knot_r0 = src.knot_r0;
+  dst.knot_r1 = src.knot_r1;
+}
+
+// Prepares stage c.stage from its vector x (the thread's states in
+// registers: position i = sid + j TS, Re at i, Im at i + N; Ej their
+// energies): writes v = exp(-i E tau) x to V, and thread 0 runs stage_chain
+// unless the control wave has run it already (chain false).
+template <int PT, int TS>
 __device__ void prepare_stage(const BDev &d, int t, Ctl &c, const double *xr, const double *xi, int par,
-                              const double *Ej) {
-  const int tid = threadIdx.x, N = d.N;
+                              const double *Ej, int sid, bool chain) {
+  const int N = d.N;
   const double tau = stage_time(c);
   double2 *v = d.V + (long long)t * N;
 #pragma unroll
   for (int j = 0; j < PT; j++) {
-    const int i = tid + j * TB;
+    const int i = sid + j * TS;
     if (i < N) {
       double sn, cs;
       sincos(tau * Ej[j], &sn, &cs);
       v[i] = make_double2(xr[j] * cs + xi[j] * sn, xi[j] * cs - xr[j] * sn);
     }
   }
-  if (tid == 0) {
-    // recorded path at tau: R, Theta, P_Phi (DERIVS_BK_adia, mqct_eqtns.f:748-759)
-    const long long o0 = d.poff[c.path];
-    const int n = (int)(d.poff[c.path + 1] - o0);
-    const double *rec = d.prec + o0 * 8;
-    const double *der = d.pder + o0 * 7;
-    const int lo = interval_cached(rec, 8, n, tau, c.rec, c.rec_t0, c.rec_t1);
-    const int hi = lo + 1;
-    const double hr = rec[hi * 8] - rec[lo * 8];
-    const double a = (rec[hi * 8] - tau) / hr, b = (tau - rec[lo * 8]) / hr;
-    const double ca = a * a * a - a, cb = b * b * b - b;
-    double pv[3];
-    const int col[3] = {0, 2, 5};
-    for (int q = 0; q < 3; q++)
-      pv[q] = a * rec[lo * 8 + 1 + col[q]] + b * rec[hi * 8 + 1 + col[q]] +
-              (ca * der[lo * 7 + col[q]] + cb * der[hi * 7 + col[q]]) * (hr * hr) / 6.0;
-    const double R = pv[0], st = sin(pv[1]);
-    c.dphidt = pv[2] / R / R / d.R[-1] / (st * st);
-    // knots of R (splint_bk, mqct_eqtns.f:575-607); R > R_n: no coupling
-    const int nr = d.n_r;
-    c.tau = tau;
-    c.zero = R > d.R[nr - 1];
-    const int klo = interval_cached(d.R, 1, nr, R, c.klo, c.knot_r0, c.knot_r1), khi = klo + 1;
-    const double hk = d.R[khi] - d.R[klo];
-    const double aa = (d.R[khi] - R) / hk, bb = (R - d.R[klo]) / hk;
-    const double cc5 = (hk * hk) / 6.0;
-    c.aa = aa;
-    c.bb = bb;
-    c.c1 = (aa * aa * aa - aa) * cc5;
-    c.c2 = (bb * bb * bb - bb) * cc5;
-    if (!c.zero) {
-      if (d.nbucketed) atomicAdd(d.nbucketed, 1ULL);  // counted only in profiled runs
-      int *cnt = d.cnt + par * nr;
-      Entry *bk = d.bucket + (long long)par * nr * d.bcap;
-      const int p0 = atomicAdd(&cnt[klo], 1), p1 = atomicAdd(&cnt[khi], 1);
-      bk[(long long)klo * d.bcap + p0] = Entry{t, 0, aa, c.c1};
-      bk[(long long)khi * d.bcap + p1] = Entry{t, 1, bb, c.c2};
-      // a knot's first entry lists the knot for the knot-list GEMM form
-      if (p0 == 0) d.klist[par * nr + atomicAdd(&d.kcnt[par], 1)] = klo;
-      if (p1 == 0) d.klist[par * nr + atomicAdd(&d.kcnt[par], 1)] = khi;
-    }
-  }
+  if (chain && threadIdx.x == 0) stage_chain(d, t, c, par);
 }
 
 // Interval GEMM.  One workgroup per (knot r, m-block b, row group):
@@ -1078,23 +1106,113 @@ void launch_gemm(int shape, dim3 knots_blocks, int maxnb, hipStream_t st, const 
   }
 }
 
+// The control update after one evaluated stage: odeint/rkqs/rkck's step
+// rules (suppl_routines.f:1229-1283, 1355-1391) and DOTRAJECT's AT rk4 loop
+// (propagator.f:3862-3880).  em is rkqs's scaled error, read only at a
+// Cash-Karp step's last stage with h above hmin.
+__device__ inline void ctl_advance(Ctl &c, int mode, double em) {
+  const double TINY = 1.e-30;
+  const int s = c.stage;
+  if (c.rk4) {
+    if (s < 3) {
+      c.stage = s + 1;
+      return;
+    }
+    c.nstp++;
+    c.stage = 0;
+    if (mode == MQCT_PROP_AT_ODEINT) {
+      // odeint's MAXSTP-th step: one rk4 step to x2 (suppl_routines.f:1385-1391)
+      c.x = c.x2 + copysign(TINY, c.x2 - c.x1);
+      c.status = 1;
+    } else {
+      // a step that would reach the path end is shortened to end there and
+      // is the last
+      c.x += c.h;
+      if (c.last) c.status = 1;
+      else if (c.x + c.h >= c.x2) {
+        c.h = c.x2 - c.x;
+        c.last = 1;
+      }
+    }
+    return;
+  }
+  if (s == 0) {
+    if (c.nstp == MAXSTP - 1) {
+      // continue as rk4 with k1 = dydx at x
+      c.rk4 = 1;
+      c.h = copysign(fabs(c.x2 - c.x), c.x2 - c.x1);
+    } else {
+      // odeint keeps h at the start of the step (error scale), then clamps h
+      c.h0 = c.h;
+      if ((c.x + c.h - c.x2) * (c.x + c.h - c.x1) > 0.0) c.h = c.x2 - c.x;
+    }
+    c.stage = 1;
+    return;
+  }
+  if (s < 5) {
+    c.stage = s + 1;
+    return;
+  }
+  // a step at or below hmin is taken as is
+  if (!((c.h <= c.hmin) || !(em > 1.0))) {
+    const double htemp = 0.9 * c.h * pow(em, -.25);
+    const double a = fabs(htemp), b = 0.1 * fabs(c.h);
+    c.h = copysign(a > b ? a : b, c.h);
+    c.stage = 1;  // retry with the same k1
+    return;
+  }
+  if (c.h > c.hmin) c.hnext = (em > 1.89e-4) ? 0.9 * c.h * pow(em, -.2) : 5.0 * c.h;
+  c.x += c.h;
+  c.nstp++;
+  c.stage = 0;
+  if ((c.x - c.x2) * (c.x2 - c.x1) >= 0.0) {
+    c.status = 1;
+  } else {
+    if (fabs(c.hnext) < c.hmin) c.hnext = copysign(c.hmin, c.x2 - c.x1);
+    c.h = c.hnext;
+  }
+}
+
+// Whether ctl_advance's result is known before the stage's states are: at
+// every stage but a Cash-Karp step's last one with h above hmin, whose
+// error decides it.
+__device__ inline bool ctl_known(const Ctl &c) { return c.rk4 || c.stage < 5 || c.h <= c.hmin; }
+
 // Completes the stage the GEMM evaluated, advances the integrator and
 // prepares the next stage (into the buckets of parity `par`).  Every thread
-// keeps the states i = tid + j BS (j < PT) in registers through the three
+// keeps the states i = sid + j TS (j < PT) in registers through the three
 // phases -- the right-hand side k of the completed stage, the integrator
 // update, the next stage vector -- so k is stored only when a later stage
 // reads it, and a finished step's y feeds the next stage without a reload.
 // LIST: workgroup b takes entry b of the compacted list of running
 // trajectories (the grid is the list's length); otherwise trajectory b.
-template <int PT, int TB = BS, bool LIST = false>
+// CW: wave 0 holds no states.  When the next stage's control record is
+// known from this one (ctl_known), wave 0 advances a copy of it and runs the
+// next stage's stage_chain (path record, knots, bucket appends) at once,
+// while the other waves load and update the states; thread 0 keeps the
+// chain's fields in LDS for the final control record.  Otherwise thread 0
+// runs the chain at the end, as without CW.
+template <int PT, int TB = BS, bool LIST = false, bool CW = false>
 __global__ __launch_bounds__(TB) void step_kernel(BDev d, int par, int w32) {
   extern __shared__ double sys[];  // 2N: the completed stage's vector (Coriolis neighbours)
   __shared__ double red[TB / 64];
+  __shared__ Ctl cw_next;  // thread 0: the control wave's stage_chain fields
+  constexpr int TS = CW ? TB - 64 : TB;  // threads that hold states
   const int tid = threadIdx.x, N = d.N, n2 = 2 * N;
+  const int sid = CW ? (tid >= 64 ? tid - 64 : N) : tid;  // wave 0 of CW: every i >= N
   if (blockIdx.x == 0 && tid == 0) *d.qctr = 0;  // item counter of the next GEMM
   const int t = LIST ? __builtin_amdgcn_readfirstlane(d.alist[blockIdx.x]) : (int)blockIdx.x;
   Ctl c = d.ctl[t];
   if (c.status != 0) return;
+  const bool early = CW && ctl_known(c);
+  if (early && tid == 0) {
+    Ctl n = c;
+    ctl_advance(n, d.mode, 0.0);
+    if (n.status == 0) {
+      stage_chain(d, t, n, par);
+      cw_next = n;
+    }
+  }
   double *S = d.S + (long long)t * NSLOT * n2;
   double *y = S;
   const int s = c.stage;
@@ -1119,15 +1237,15 @@ __global__ __launch_bounds__(TB) void step_kernel(BDev d, int par, int w32) {
     for (int m = 0; m < 4; m++)
 #pragma unroll
       for (int j = 0; j < PT; j++) {
-        const int i = tid + j * TB;
+        const int i = sid + j * TS;
         kp[m][j] = (m < nsl && i < N) ? make_double2(S[sl[m] * n2 + i], S[sl[m] * n2 + i + N])
                                       : make_double2(0.0, 0.0);
       }
   }
-  // component p (0 Re, 1 Im) of state tid + j TB of the m-th slot read (slot sl[m])
+  // component p (0 Re, 1 Im) of state sid + j TS of the m-th slot read (slot sl[m])
   auto K = [&](int m, int j, int p) -> double {
     if constexpr (PF) return p ? kp[m][j].y : kp[m][j].x;
-    else return S[sl[m] * n2 + tid + j * TB + p * N];
+    else return S[sl[m] * n2 + sid + j * TS + p * N];
   };
   // -- k = -i exp(i E tau) (W_lo + W_hi) + Coriolis(stage vector), the stage
   //    vector formed again from its rotated amplitudes
@@ -1145,7 +1263,7 @@ __global__ __launch_bounds__(TB) void step_kernel(BDev d, int par, int w32) {
     int2 nj[PT];
 #pragma unroll
     for (int j = 0; j < PT; j++) {
-      const int i = tid + j * TB;
+      const int i = sid + j * TS;
       Ej[j] = yr[j] = yi[j] = 0.0;
       vj[j] = w0j[j] = w1j[j] = fj[j] = make_double2(0.0, 0.0);
       nj[j] = make_int2(-1, -1);
@@ -1170,7 +1288,7 @@ __global__ __launch_bounds__(TB) void step_kernel(BDev d, int par, int w32) {
     }
 #pragma unroll
     for (int j = 0; j < PT; j++) {
-      const int i = tid + j * TB;
+      const int i = sid + j * TS;
       kr[j] = ki[j] = 0.0;
       if (i < N) {
         double sn, cs;
@@ -1189,7 +1307,7 @@ __global__ __launch_bounds__(TB) void step_kernel(BDev d, int par, int w32) {
     const double hd = c.dphidt / 2.0;
 #pragma unroll
     for (int j = 0; j < PT; j++) {
-      const int i = tid + j * TB;
+      const int i = sid + j * TS;
       if (i < N) {
         // state-diagonal Coriolis coupling (mqct_eqtns.f:845-877)
         const int2 nbr = nj[j];
@@ -1213,7 +1331,7 @@ __global__ __launch_bounds__(TB) void step_kernel(BDev d, int par, int w32) {
     double *k = S + (1 + s) * n2;
 #pragma unroll
     for (int j = 0; j < PT; j++) {
-      const int i = tid + j * TB;
+      const int i = sid + j * TS;
       if (i < N) {
         k[i] = kr[j];
         k[i + N] = ki[j];
@@ -1224,24 +1342,25 @@ __global__ __launch_bounds__(TB) void step_kernel(BDev d, int par, int w32) {
   auto stage_from_k = [&](double f) {
 #pragma unroll
     for (int j = 0; j < PT; j++) {
-      const int i = tid + j * TB;
+      const int i = sid + j * TS;
       if (i < N) {
         xr[j] = yr[j] + f * kr[j];
         xi[j] = yi[j] + f * ki[j];
       }
     }
   };
+  // the integrator's vector part; ctl_advance makes the control update
   if (c.rk4) {
     if (s < 3) {
       store_k();
-      c.stage = s + 1;
       stage_from_k(s < 2 ? 0.5 * c.h : c.h);
+      ctl_advance(c, d.mode, 0.0);
     } else {
       const double h6 = c.h / 6.0;
       double nrm = 0.0;
 #pragma unroll
       for (int j = 0; j < PT; j++) {
-        const int i = tid + j * TB;
+        const int i = sid + j * TS;
         if (i < N) {
           // k1, k2, k3 are reads 0, 1, 2
           xr[j] = yr[j] + h6 * (K(0, j, 0) + kr[j] + 2.0 * (K(1, j, 0) + K(2, j, 0)));
@@ -1251,46 +1370,24 @@ __global__ __launch_bounds__(TB) void step_kernel(BDev d, int par, int w32) {
           nrm += xr[j] * xr[j] + xi[j] * xi[j];
         }
       }
-      c.nstp++;
-      c.stage = 0;
-      if (d.mode == MQCT_PROP_AT_ODEINT) {
-        // odeint's MAXSTP-th step: one rk4 step to x2 (suppl_routines.f:1385-1391)
-        c.x = c.x2 + copysign(TINY, c.x2 - c.x1);
-        c.status = 1;
-      } else {
-        // DOTRAJECT's AT rk4 loop (propagator.f:3862-3880): a step that
-        // would reach the path end is shortened to end there and is the last
-        c.x += c.h;
-        if (c.last) c.status = 1;
-        else if (c.x + c.h >= c.x2) { c.h = c.x2 - c.x; c.last = 1; }
-      }
+      ctl_advance(c, d.mode, 0.0);
       nrm = block_sum<TB>(nrm, red);
       c.norm_err = fmax(c.norm_err, fabs(1.0 - nrm));
     }
   } else if (s == 0) {
+    // odeint: yscal = |y| + |h dydx| + TINY with h at the start of the step
+    // (formed again from y and k1 when the error is measured); the first
+    // stage uses h as clamped, or rk4's half step after MAXSTP - 1 steps
     store_k();
-    if (c.nstp == MAXSTP - 1) {
-      // continue as rk4 with k1 = dydx at x
-      c.rk4 = 1;
-      c.h = copysign(fabs(c.x2 - c.x), c.x2 - c.x1);
-      c.stage = 1;
-      stage_from_k(0.5 * c.h);
-    } else {
-      // odeint: yscal = |y| + |h dydx| + TINY with h at the start of the
-      // step (formed again from y and k1 when the error is measured), then
-      // clamp h
-      c.h0 = c.h;
-      if ((c.x + c.h - c.x2) * (c.x + c.h - c.x1) > 0.0) c.h = c.x2 - c.x;
-      c.stage = 1;
-      stage_from_k(CK_B[1][0] * c.h);
-    }
+    ctl_advance(c, d.mode, 0.0);
+    stage_from_k((c.rk4 ? 0.5 : CK_B[1][0]) * c.h);
   } else if (s < 5) {
     // Cash-Karp stage s + 1 = y + h sum_q B[s+1][q] k_{q+1}, the last k in registers
     store_k();
-    c.stage = s + 1;
+    ctl_advance(c, d.mode, 0.0);
 #pragma unroll
     for (int j = 0; j < PT; j++) {
-      const int i = tid + j * TB;
+      const int i = sid + j * TS;
       if (i < N) {
         double ar = 0.0, ai = 0.0;
 #pragma unroll
@@ -1315,7 +1412,7 @@ __global__ __launch_bounds__(TB) void step_kernel(BDev d, int par, int w32) {
     double em = 0.0;
 #pragma unroll
     for (int j = 0; j < PT; j++) {
-      const int i = tid + j * TB;
+      const int i = sid + j * TS;
       if (i < N) {
 #pragma unroll
         for (int p = 0; p < 2; p++) {
@@ -1335,37 +1432,24 @@ __global__ __launch_bounds__(TB) void step_kernel(BDev d, int par, int w32) {
     em = block_max<TB>(em, red) / c.eps;
     // a step at or below hmin is taken as is
     const bool accept = (c.h <= c.hmin) || !(em > 1.0);
+    ctl_advance(c, d.mode, em);  // a rejected step retries with the same k1 and a smaller h
     if (!accept) {
-      const double htemp = 0.9 * c.h * pow(em, -.25);
-      const double a = fabs(htemp), b = 0.1 * fabs(c.h);
-      c.h = copysign(a > b ? a : b, c.h);
-      c.stage = 1;  // retry with the same k1
 #pragma unroll
       for (int j = 0; j < PT; j++) {
-        const int i = tid + j * TB;
+        const int i = sid + j * TS;
         if (i < N) {
           xr[j] = yr[j] + CK_B[1][0] * c.h * K(0, j, 0);
           xi[j] = yi[j] + CK_B[1][0] * c.h * K(0, j, 1);
         }
       }
     } else {
-      if (c.h > c.hmin) c.hnext = (em > 1.89e-4) ? 0.9 * c.h * pow(em, -.2) : 5.0 * c.h;
 #pragma unroll
       for (int j = 0; j < PT; j++) {
-        const int i = tid + j * TB;
+        const int i = sid + j * TS;
         if (i < N) {
           y[i] = xr[j];
           y[i + N] = xi[j];
         }
-      }
-      c.x += c.h;
-      c.nstp++;
-      c.stage = 0;
-      if ((c.x - c.x2) * (c.x2 - c.x1) >= 0.0) {
-        c.status = 1;
-      } else {
-        if (fabs(c.hnext) < c.hmin) c.hnext = copysign(c.hmin, c.x2 - c.x1);
-        c.h = c.hnext;
       }
     }
   }
@@ -1376,8 +1460,11 @@ __global__ __launch_bounds__(TB) void step_kernel(BDev d, int par, int w32) {
     }
     return;
   }
-  prepare_stage<PT, TB>(d, t, c, xr, xi, par, Ej);
-  if (tid == 0) d.ctl[t] = c;
+  prepare_stage<PT, TS>(d, t, c, xr, xi, par, Ej, sid, !early);
+  if (tid == 0) {
+    if (early) copy_chain(c, cw_next);
+    d.ctl[t] = c;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1897,7 +1984,7 @@ __global__ __launch_bounds__(BS) void init_kernel(BDev d, const mqct_traj_t *__r
   c.hmin = p.hmin;
   c.eps = p.eps;
   c.path = p.at_path;
-  prepare_stage<PT>(d, t, c, xr, xi, 0, Ej);
+  prepare_stage<PT, BS>(d, t, c, xr, xi, 0, Ej, tid, true);
   if (tid == 0) {
     d.ctl[t] = c;
     d.alist[t] = t;
@@ -2278,8 +2365,12 @@ int mqct_dyn_run_batched(mqct_hip *h, int mode, int n_traj, const mqct_traj_t *t
       // 1024 threads (two states per thread) when asked for, or by the
       // automatic choice once fewer than 256 trajectories run (there it
       // shortens each workgroup's run; with a full grid 512 is faster)
+      // (with 1024 threads wave 0 runs the control chain when the other
+      // waves hold every state; at 512 the 448-thread state stride costs
+      // registers that the full grid's occupancy cannot spare)
       if (stb == 1024 || (ds->step_threads == 0 && stb == 512 && N <= 2 * 1024 && grid < 256)) {
-        step_kernel<2, 1024, LI><<<sg, 1024, lds_step, h->stream>>>(d, par, w32);
+        if (N <= 2 * (1024 - 64)) step_kernel<2, 1024, LI, true><<<sg, 1024, lds_step, h->stream>>>(d, par, w32);
+        else step_kernel<2, 1024, LI><<<sg, 1024, lds_step, h->stream>>>(d, par, w32);
         return;
       }
       if (stb == 512) {

@@ -329,13 +329,18 @@ def test_at_batched_full_blocks(dev, shape, knots):
 
 
 @pytest.mark.parametrize("threads", [512, 1024])
-@pytest.mark.parametrize("mode", [L.MQCT_PROP_AT_RK4, L.MQCT_PROP_AT_ODEINT])
-def test_at_batched_step_threads(dev, mode, threads):
+@pytest.mark.parametrize("mode,time_step,hmin", [(L.MQCT_PROP_AT_RK4, 10.0, 20.0),
+                                                  (L.MQCT_PROP_AT_ODEINT, 10.0, 20.0),
+                                                  (L.MQCT_PROP_AT_ODEINT, 60.0, 10.0)])
+def test_at_batched_step_threads(dev, mode, time_step, hmin, threads):
     """The step kernel with 512 threads per trajectory (two states per
     thread fewer, block reductions over eight waves) and with 1024 (two
     states per thread, the earlier k slots loaded with the stage's other
-    loads) against the plain kernel, on 969 states where the automatic
-    choice would take 256."""
+    loads, wave 0 running the next stage's path and bucket chain ahead of
+    the states) against the plain kernel, on 969 states where the automatic
+    choice would take 256.  At TIME_STEP 60 and hmin 10 rkqs's error
+    control decides the steps (the chain then waits for the error at a
+    step's last stage)."""
     from mqct_amd.matrix import MatrixBuilder
     from mqct_amd.propagate import at_first_step
     wl = W.asym_top_atom(jmax=8, n_beta=20, n_gamma=40, n_r=120)
@@ -347,13 +352,19 @@ def test_at_batched_step_threads(dev, mode, threads):
     prop.load_paths(paths)
     rows = []
     for i in range(len(bb)):
-        rows += prop.trajectories([l[i]], float(E[i]), 1, time_step=10.0, eps=1e-8, hmin=20.0, at_paths=[i])
+        rows += prop.trajectories([l[i]], float(E[i]), 1, time_step=time_step, eps=1e-8, hmin=hmin,
+                                  at_paths=[i])
     q0, s0 = prop.run(mode, rows, batched=0)
     L.check(dev.lib.mqct_hip_batched_set_step_threads(dev.h, threads), "set_step_threads")
     try:
         q1, s1 = prop.run(mode, rows, batched=1)
     finally:
         L.check(dev.lib.mqct_hip_batched_set_step_threads(dev.h, 0), "set_step_threads")
+    rej = (s1["n_rhs"] - 6 * s1["n_steps"]) / 5
+    print("steps", s1["n_steps"].tolist(), "plain", s0["n_steps"].tolist(), "rejected trials",
+          rej.astype(int).tolist(), flush=True)
+    if time_step > hmin and mode == L.MQCT_PROP_AT_ODEINT:
+        assert np.any(rej > 0)  # rkqs's error control runs
     N = wl.basis.n_states
     d = np.max(np.abs(q1[:, :2 * N] - q0[:, :2 * N]))
     tol = 1e-9 if mode == L.MQCT_PROP_AT_RK4 else 1e-6
