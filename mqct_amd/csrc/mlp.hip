@@ -564,10 +564,9 @@ extern "C" int mqct_hip_surrogate_vgrid(mqct_hip_t *h, int n_r, const double *R_
     if (Vplain.ensure(n * 8)) return mqct_fail(MQCT_ERR_NOMEM, "surrogate_vgrid");
     HIP_TRY(hipMemcpyAsync(Vplain.p, h->d_V.p, n * 8, hipMemcpyDeviceToDevice, h->stream));
   }
-  h->n_r = n_r;
-  h->R.assign(R_com, R_com + n_r);
-  if (h->d_R.ensure(n_r * 8) || h->d_V.ensure(n * 8)) return mqct_fail(MQCT_ERR_NOMEM, "surrogate_vgrid");
-  HIP_TRY(hipMemcpyAsync(h->d_R.p, R_com, n_r * 8, hipMemcpyHostToDevice, h->stream));
+  int rc = mqct_set_r(h, n_r, R_com);  // the grid, its device copy and spline factors
+  if (rc) return rc;
+  if (h->d_V.ensure(n * 8)) return mqct_fail(MQCT_ERR_NOMEM, "surrogate_vgrid");
   const long long CH = 1 << 20;
   if (ensure_batch(s, CH, true)) return mqct_fail(MQCT_ERR_NOMEM, "surrogate batch");
   for (long long g0 = 0; g0 < n; g0 += CH) {

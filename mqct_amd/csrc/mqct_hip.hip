@@ -382,6 +382,8 @@ extern "C" int mqct_hip_set_grid(mqct_hip_t *h, const mqct_grid_t *g) {
   return MQCT_OK;
 }
 
+static std::vector<double> spline_tables(const std::vector<double> &R);
+
 int mqct_set_r(mqct_hip *h, int n_r, const double *R) {
   if (n_r < 2 || !R) return mqct_fail(MQCT_ERR_ARG, "R grid needs >= 2 points");
   for (int i = 1; i < n_r; i++)
@@ -392,7 +394,11 @@ int mqct_set_r(mqct_hip *h, int n_r, const double *R) {
   if (!same) h->terms_nr = 0;  // expansion coefficients were on another R grid
   h->n_r = n_r;
   h->R.assign(R, R + n_r);
-  return upload(h, h->d_R, R, n_r);
+  int rc = upload(h, h->d_R, R, n_r);
+  if (rc) return rc;
+  // the spline's forward factors on this grid, for every build's post_kernel
+  const std::vector<double> tab = spline_tables(h->R);
+  return upload(h, h->d_spl, tab.data(), tab.size());
 }
 
 extern "C" int mqct_hip_set_vgrid(mqct_hip_t *h, int n_r, const double *R_com,
@@ -471,29 +477,143 @@ __device__ inline void spline_column(int n, const double *__restrict__ R,
 }
 
 // Cutoff (matrix_ini.f:1991-1997) and clamped spline of the compute branch.
-// One thread per element k.
-__global__ void post_kernel(int64_t T, int n, const double *__restrict__ R,
-                            const double *__restrict__ sig,
-                            const double *__restrict__ pp,
-                            const double *__restrict__ y2f, double cut,
-                            int ir1, int ir2,
-                            const double *__restrict__ cutv,
-                            double *__restrict__ M,
-                            double *__restrict__ D,
-                            unsigned char *__restrict__ skip) {
-  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= T) return;
+// One wave per element (column k, n R-contiguous values), PW per workgroup.
+// spline_column's two sweeps are first-order linear recurrences,
+//   forward   u_i = (X_i - sig_i u_{i-1}) / pp_i = X_i / pp_i + (-sig_i / pp_i) u_{i-1},
+//   backward  d_i = y2f_i d_{i+1} + u_i,
+// so each 64-point chunk is an inclusive scan over the lanes of the affine
+// maps x -> A + B x (composition (A, B) after (A', B') = (A + B A', B B')),
+// applied to the value carried from the previous chunk.  The end rows are
+// the maps of spline_column's first and last points (B = 0 starts a chain).
+// The sums are reassociated, so the values differ from the sequential
+// sweeps in the last bits.  A column below the cutoff is zeroed.
+constexpr int PW = 4;  // elements (waves) per workgroup
+constexpr int PR = 8;  // 64-point chunks kept in registers (n <= 512)
+__device__ inline void affine_scan_up(double &A, double &B, int lane) {
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const double Ap = __shfl_up(A, o, 64), Bp = __shfl_up(B, o, 64);
+    if (lane >= o) {
+      A = A + B * Ap;
+      B = B * Bp;
+    }
+  }
+}
+__device__ inline void affine_scan_down(double &A, double &B, int lane) {
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const double An = __shfl_down(A, o, 64), Bn = __shfl_down(B, o, 64);
+    if (lane + o < 64) {
+      A = A + B * An;
+      B = B * Bn;
+    }
+  }
+}
+__global__ __launch_bounds__(64 * PW) void post_kernel(int64_t T, int n, const double *__restrict__ R,
+                                                       const double *__restrict__ sig,
+                                                       const double *__restrict__ pp,
+                                                       const double *__restrict__ y2f, double cut,
+                                                       int ir1, int ir2,
+                                                       const double *__restrict__ cutv,
+                                                       double *__restrict__ M,
+                                                       double *__restrict__ D,
+                                                       unsigned char *__restrict__ skip) {
+  const int lane = threadIdx.x & 63;
+  const int64_t k = (int64_t)blockIdx.x * PW + (threadIdx.x >> 6);
+  if (k >= T) return;  // the whole wave
   double *y = M + k * n, *d = D + k * n;
   // cutv (mixed precision): FP64 values at the two cutoff points
-  double a1 = cutv ? fabs(cutv[2 * k]) : fabs(y[ir1]);
-  double a2 = cutv ? fabs(cutv[2 * k + 1]) : fabs(y[ir2]);
+  const double a1 = cutv ? fabs(cutv[2 * k]) : fabs(y[ir1]);
+  const double a2 = cutv ? fabs(cutv[2 * k + 1]) : fabs(y[ir2]);
   if ((a1 > a2 ? a1 : a2) < cut) {
-    for (int i = 0; i < n; i++) { y[i] = 0.0; d[i] = 0.0; }
-    skip[k] = 1;
+    for (int i = lane; i < n; i += 64) {
+      y[i] = 0.0;
+      d[i] = 0.0;
+    }
+    if (lane == 0) skip[k] = 1;
     return;
   }
-  skip[k] = 0;
-  spline_column(n, R, sig, pp, y2f, y, d);
+  if (lane == 0) skip[k] = 0;
+  // the affine map of forward step i (spline_column's first and last
+  // points at i = 0 and n - 1; the identity past the end)
+  auto fwd_map = [&](int i, double &A, double &B) {
+    A = 0.0;
+    B = 1.0;
+    if (i == 0) {
+      const double y0 = y[0], y1 = y[1];
+      const double yp1 = (y1 - y0) / (R[1] - R[0]);
+      A = (3.0 / (R[1] - R[0])) * ((y1 - y0) / (R[1] - R[0]) - yp1);
+      B = 0.0;
+    } else if (i < n - 1) {
+      const double ym = y[i - 1], yi = y[i], yp = y[i + 1];
+      const double X = 6.0 * ((yp - yi) / (R[i + 1] - R[i]) - (yi - ym) / (R[i] - R[i - 1])) /
+                       (R[i + 1] - R[i - 1]);
+      A = X / pp[i];
+      B = -sig[i] / pp[i];
+    } else if (i == n - 1) {
+      const double yl = y[n - 1], yk = y[n - 2], h = R[n - 1] - R[n - 2];
+      const double ypn = (yl - yk) / h;
+      const double qn = 0.5, un = (3.0 / h) * (ypn - (yl - yk) / h);
+      const double den = qn * y2f[n - 2] + 1.0;
+      A = un / den;
+      B = -qn / den;
+    }
+  };
+  if (n <= 64 * PR) {
+    // u in registers, PR chunks per lane
+    double ur[PR];
+    double carry = 0.0;
+#pragma unroll
+    for (int c = 0; c < PR; c++) {
+      ur[c] = 0.0;
+      if (64 * c >= n) continue;
+      double A, B;
+      fwd_map(64 * c + lane, A, B);
+      affine_scan_up(A, B, lane);
+      ur[c] = A + B * carry;
+      carry = __shfl(ur[c], 63, 64);
+    }
+    carry = 0.0;
+#pragma unroll
+    for (int c = PR - 1; c >= 0; c--) {
+      if (64 * c >= n) continue;
+      const int i = 64 * c + lane;
+      double A = 0.0, B = 1.0;
+      if (i < n) {
+        A = ur[c];
+        B = i < n - 1 ? y2f[i] : 0.0;  // d_{n-1} = u_{n-1}
+      }
+      affine_scan_down(A, B, lane);
+      const double v = A + B * carry;
+      if (i < n) d[i] = v;
+      carry = __shfl(v, 0, 64);
+    }
+    return;
+  }
+  // longer grids: u_i goes to d[i] until the backward sweep replaces it
+  double carry = 0.0;
+  for (int i0 = 0; i0 < n; i0 += 64) {
+    const int i = i0 + lane;
+    double A, B;
+    fwd_map(i, A, B);
+    affine_scan_up(A, B, lane);
+    const double u = A + B * carry;
+    if (i < n) d[i] = u;
+    carry = __shfl(u, 63, 64);
+  }
+  carry = 0.0;
+  for (int i0 = (n - 1) / 64 * 64; i0 >= 0; i0 -= 64) {
+    const int i = i0 + lane;
+    double A = 0.0, B = 1.0;
+    if (i < n) {
+      A = d[i];
+      B = i < n - 1 ? y2f[i] : 0.0;
+    }
+    affine_scan_down(A, B, lane);
+    const double v = A + B * carry;
+    if (i < n) d[i] = v;
+    carry = __shfl(v, 0, 64);
+  }
 }
 
 // READ_MIJ_USER after each column is read (matrix_ini.f:6603-6621): the
@@ -534,12 +654,9 @@ static std::vector<double> spline_tables(const std::vector<double> &R) {
 int mqct_run_post(mqct_hip *h, double cut, int ir1, int ir2,
                   const double *cutv) {
   int n = h->n_r;
-  std::vector<double> tab = spline_tables(h->R);
-  int rc = upload(h, h->d_spl, tab.data(), tab.size());
-  if (rc) return rc;
-  const double *t = h->d_spl.as<double>();
+  const double *t = h->d_spl.as<double>();  // uploaded with the R grid
   int64_t T = h->built_T;
-  post_kernel<<<(unsigned)((T + 255) / 256), 256, 0, h->stream>>>(
+  post_kernel<<<(unsigned)((T + PW - 1) / PW), 64 * PW, 0, h->stream>>>(
       T, n, h->d_R.as<double>(), t, t + n, t + 2 * n, cut, ir1 - 1, ir2 - 1,
       cutv, h->d_mat.as<double>(), h->d_der.as<double>(),
       h->d_skip.as<unsigned char>());

@@ -100,6 +100,17 @@ def test_diatom_atom_fp64_full_size(dev):
     assert rel_err(D, Do, 1e-10) <= 1e-8
 
 
+def test_diatom_atom_fp64_long_grid(dev):
+    """700 R points: the spline's scans run past the 512 points kept in
+    registers, through the derivative table."""
+    wl = W.diatom_atom(jmax=4, n_r=700)
+    (M, D, S), (Mo, Do, So), orc = build_both(dev, wl)
+    assert np.array_equal(S, So)
+    A = abs_sums(orc, wl)
+    assert fp64_ok(M, Mo, A) <= 1.0
+    assert rel_err(D, Do, 1e-10) <= 1e-8
+
+
 def test_asym_top_atom_fp64_small(dev):
     wl = W.asym_top_atom(jmax=3, n_beta=12, n_gamma=24, n_r=20)
     (M, D, S), (Mo, Do, So), orc = build_both(dev, wl)
