@@ -362,12 +362,15 @@ def main():
             xsec["channels"] = nch
             log("xsec of the %s build vs fp64: max rel err %.2e over %d channels" % (prec, e, nch))
 
-    # -- matrix-build times ----------------------------------------------------
+    # -- matrix-build times: a warm build of each form (the first build of a
+    #    basis and grid also forms the basis tables, and the first of each
+    #    form converts the weights and times the contraction variants) ------
     if not args.no_build:
-        for prec in ("fp32", "bf16x3"):
+        for prec in ("fp64", "fp32", "bf16x3"):
             if prec not in build["asym_top_atom"]:
                 mb.build(prec, copy_out=False)
-                build["asym_top_atom"][prec] = mb.timing()
+            mb.build(prec, copy_out=False)
+            build["asym_top_atom"][prec] = mb.timing()
         log("matrix build total ms: " + ", ".join(
             "%s %s %.1f" % (w, p, build[w][p]["total_ms"]) for w in build for p in build[w]))
     build = {w: {p: {k: round(float(v), 3) for k, v in t.items()} for p, t in d.items()} for w, d in build.items()}
